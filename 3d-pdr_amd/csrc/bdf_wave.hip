@@ -1,0 +1,88 @@
+// Wavefront-per-system BDF kernels, instantiated per network, and their
+// launchers.  See bdf_wave_impl.h for the algorithm and layout.
+#include <hip/hip_runtime.h>
+
+#include "gen/net_reduced.h"
+#include "pdr_internal.h"
+
+namespace bdfw_reduced {
+namespace NET = pdrnet_reduced;
+#include "bdf_wave_impl.h"
+}  // namespace bdfw_reduced
+
+namespace pdrchem {
+
+template <typename Args>
+static void fill_args(Args &a, const SolveRequest &r) {
+  a.abundance = r.abundance;
+  a.rate = r.rate;
+  a.density = r.density;
+  a.nsys = r.npts;
+  a.ab_stride = r.nspec;
+  a.rate_stride = r.nreac;
+  a.rtol = r.rtol;
+  a.atol = r.atol;
+  a.t0 = r.t0;
+  a.tend = r.tend;
+  a.tout0 = r.tout0;
+  a.nelect0 = r.nelect0;
+  a.mxstep = r.mxstep;
+  a.status = r.status;
+  a.stats = r.stats;
+  a.counter = r.counter;
+}
+
+template <typename Kern, typename Args>
+static int launch_persistent(Kern kern, const Args &a, int npts, hipStream_t s, int max_blocks) {
+  int dev = 0, ncu = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return -1;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return -1;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 256, 0) != hipSuccess) return -1;
+  if (per_cu < 1) per_cu = 1;
+  long long want = ((long long)npts + 3) / 4;  // 4 wavefronts per block
+  long long full = (long long)ncu * per_cu;
+  long long nb = want < full ? want : full;
+  if (max_blocks > 0 && nb > max_blocks) nb = max_blocks;
+  if (nb < 1) nb = 1;
+  hipLaunchKernelGGL(kern, dim3((unsigned)nb), dim3(256), 0, s, a);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int launch_wave_solver(int net, const SolveRequest &r, hipStream_t s) {
+  switch (net) {
+    case NET_REDUCED: {
+      bdfw_reduced::PointArgs a;
+      fill_args(a, r);
+      if (r.mixed) return launch_persistent(bdfw_reduced::bdf_wave_kernel<float>, a, r.npts, s, r.max_blocks);
+      return launch_persistent(bdfw_reduced::bdf_wave_kernel<double>, a, r.npts, s, r.max_blocks);
+    }
+    default:
+      return -2;
+  }
+}
+
+template <typename Kern>
+static int kernel_info(Kern fn, KernelInfo *info) {
+  hipFuncAttributes attr;
+  if (hipFuncGetAttributes(&attr, (const void *)fn) != hipSuccess) return -1;
+  info->num_regs = attr.numRegs;
+  info->shared_bytes = (int)attr.sharedSizeBytes;
+  info->local_bytes = (int)attr.localSizeBytes;
+  info->max_threads = attr.maxThreadsPerBlock;
+  int per_cu = 0;
+  (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, 0);
+  info->blocks_per_cu = per_cu;
+  return 0;
+}
+
+int wave_solver_info(int net, int mixed, KernelInfo *info) {
+  switch (net) {
+    case NET_REDUCED:
+      return mixed ? kernel_info(bdfw_reduced::bdf_wave_kernel<float>, info)
+                   : kernel_info(bdfw_reduced::bdf_wave_kernel<double>, info);
+    default:
+      return -2;
+  }
+}
+
+}  // namespace pdrchem

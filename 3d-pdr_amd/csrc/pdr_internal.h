@@ -1,0 +1,49 @@
+// Internal interfaces between the C-ABI layer and the kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pdrchem {
+
+enum NetId { NET_REDUCED = 0, NET_FULL = 1, NET_MYNETWORK = 2, NET_COUNT = 3 };
+
+// one batched solve: every pointer is device memory
+struct SolveRequest {
+  double *abundance;      // [npts][nspec], point-major (calculate_abundances.c:155)
+  const double *rate;     // [npts][nreac]
+  const double *density;  // [npts]
+  int npts, nspec, nreac;
+  double rtol, atol;
+  double t0, tend, tout0; // seconds
+  int nelect0;            // 0-based electron slot
+  long long mxstep;
+  int *status;            // [npts]
+  int *stats;             // [npts][8] or null
+  unsigned int *counter;  // work-queue counter, zero on entry
+  int max_blocks;         // 0 = fill the device
+  int mixed;              // 0: FP64 iteration matrix; 1: FP32 matrix, FP64 Newton
+};
+
+struct KernelInfo {
+  int num_regs, shared_bytes, local_bytes, max_threads, blocks_per_cu;
+};
+
+int launch_wave_solver(int net, const SolveRequest &r, hipStream_t s);
+int wave_solver_info(int net, int mixed, KernelInfo *info);
+
+// rate evaluation (calc_reac_rates.F90)
+struct PhysParams {
+  double zeta, v_turb, omega, grain_radius, metallicity, g2d;
+};
+struct RateRequest {
+  const double *T, *Td;   // [npts]
+  const double *G, *Av;   // [npts][nrays]
+  const double *cols;     // [npts][nrays][5] (H2, HD, CO, C, S) or null
+  int npts, nrays;
+  PhysParams ph;
+  double *rate;           // [npts][nreac]
+  int *neg;               // count of points with a negative rate (device int)
+};
+int launch_rates(int net, const RateRequest &r, hipStream_t s);
+
+}  // namespace pdrchem

@@ -1,0 +1,265 @@
+// C ABI of libpdrchem.so (declarations and reference citations in
+// include/pdrchem.h).
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/pdrchem.h"
+#include "gen/net_reduced.h"
+#include "pdr_internal.h"
+
+using namespace pdrchem;
+
+// ---------------------------------------------------------------------------
+// Globals of the reference interface (weak: a Fortran host's bind(c)
+// definitions take precedence).  Defaults as set by the reference driver:
+// tolerances 3DPDR.F90:462-463, start time 3DPDR.F90:452, end time params.dat.
+// ---------------------------------------------------------------------------
+extern "C" {
+__attribute__((weak)) double chemistry_module_mp_relative_abundance_tolerance_ = 1.0e-8;
+__attribute__((weak)) double chemistry_module_mp_absolute_abundance_tolerance_ = 1.0e-30;
+__attribute__((weak)) double maincode_module_mp_start_time_ = 0.0;
+__attribute__((weak)) double maincode_module_mp_end_time_ = 1.0e7;
+__attribute__((weak)) int global_module_mp_nelect_ = 0;
+}
+
+namespace {
+
+constexpr double SECONDS_IN_YEAR = 3.1556926e7;  // calculate_abundances.c:90
+
+thread_local std::string g_err;
+int g_max_blocks = 0;
+int g_precision = -1;  // -1: not set, read PDRCHEM_PRECISION on first use
+
+int host_precision() {
+  if (g_precision < 0) {
+    const char *e = std::getenv("PDRCHEM_PRECISION");
+    g_precision = (e && std::strcmp(e, "fp64") == 0) ? PDRCHEM_FP64 : PDRCHEM_MIXED;
+  }
+  return g_precision;
+}
+
+void set_err(const std::string &s) { g_err = s; }
+
+bool hip_ok(hipError_t e, const char *what) {
+  if (e == hipSuccess) return true;
+  set_err(std::string(what) + ": " + hipGetErrorString(e));
+  return false;
+}
+
+struct NetShape {
+  int nspec, nreac;
+};
+const NetShape kNets[NET_COUNT] = {
+    {pdrnet_reduced::NSPEC, pdrnet_reduced::NREAC},
+    {215, 2926},
+    {58, 634},
+};
+const bool kCompiled[NET_COUNT] = {true, false, false};
+
+// device buffers reused across calculate_abundances_ calls
+struct HostPathBuffers {
+  std::mutex mu;
+  double *ab = nullptr, *rate = nullptr, *dens = nullptr;
+  int *status = nullptr;
+  unsigned int *counter = nullptr;
+  size_t cap_ab = 0, cap_rate = 0, cap_pts = 0;
+  hipStream_t stream = nullptr;
+
+  bool reserve(size_t n_ab, size_t n_rate, size_t npts) {
+    if (!stream && !hip_ok(hipStreamCreate(&stream), "hipStreamCreate")) return false;
+    if (!counter && !hip_ok(hipMalloc(&counter, sizeof(unsigned int)), "hipMalloc")) return false;
+    if (n_ab > cap_ab) {
+      if (ab) (void)hipFree(ab);
+      ab = nullptr;
+      if (!hip_ok(hipMalloc(&ab, n_ab * sizeof(double)), "hipMalloc(abundance)")) return false;
+      cap_ab = n_ab;
+    }
+    if (n_rate > cap_rate) {
+      if (rate) (void)hipFree(rate);
+      rate = nullptr;
+      if (!hip_ok(hipMalloc(&rate, n_rate * sizeof(double)), "hipMalloc(rate)")) return false;
+      cap_rate = n_rate;
+    }
+    if (npts > cap_pts) {
+      if (dens) (void)hipFree(dens);
+      if (status) (void)hipFree(status);
+      dens = nullptr;
+      status = nullptr;
+      if (!hip_ok(hipMalloc(&dens, npts * sizeof(double)), "hipMalloc(density)")) return false;
+      if (!hip_ok(hipMalloc(&status, npts * sizeof(int)), "hipMalloc(status)")) return false;
+      cap_pts = npts;
+    }
+    return true;
+  }
+};
+HostPathBuffers g_buf;
+
+// cvode.log entries for failed points (calculate_abundances.c:108, 235-298)
+void log_failures(const std::vector<int> &st, double start_time, double end_time) {
+  FILE *fh = nullptr;
+  for (size_t n = 0; n < st.size(); n++) {
+    if (st[n] == 0) continue;
+    if (!fh) fh = std::fopen("cvode.log", "a");
+    if (!fh) return;
+    std::fprintf(fh, "ERROR! Integration failed: particle = %zu, flag = %d, t = 0 .. %8.2le yr\n\n", n,
+                 st[n], end_time - start_time);
+  }
+  if (fh) std::fclose(fh);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *pdrchem_last_error(void) { return g_err.c_str(); }
+
+void pdrchem_set_max_blocks(int max_blocks) { g_max_blocks = max_blocks < 0 ? 0 : max_blocks; }
+
+void pdrchem_set_precision(int precision) { g_precision = precision == PDRCHEM_FP64 ? PDRCHEM_FP64 : PDRCHEM_MIXED; }
+
+int pdrchem_network_id(int nspec, int nreac) {
+  for (int i = 0; i < NET_COUNT; i++)
+    if (kCompiled[i] && kNets[i].nspec == nspec && kNets[i].nreac == nreac) return i;
+  return -1;
+}
+
+int pdrchem_kernel_info(int net, int precision, int *num_regs, int *lds_bytes, int *scratch_bytes,
+                        int *blocks_per_cu) {
+  KernelInfo ki{};
+  if (wave_solver_info(net, precision == PDRCHEM_MIXED, &ki) != 0) {
+    set_err("no solver kernel for this network");
+    return -1;
+  }
+  if (num_regs) *num_regs = ki.num_regs;
+  if (lds_bytes) *lds_bytes = ki.shared_bytes;
+  if (scratch_bytes) *scratch_bytes = ki.local_bytes;
+  if (blocks_per_cu) *blocks_per_cu = ki.blocks_per_cu;
+  return 0;
+}
+
+int pdrchem_solve_device(int net, double *d_abundance, const double *d_rate, const double *d_density,
+                         int npts, double rtol, double atol, double t_start, double t_end, int nelect,
+                         int *d_status, int *d_stats, unsigned int *d_counter, int precision, void *stream) {
+  if (net < 0 || net >= NET_COUNT || !kCompiled[net]) {
+    set_err("unsupported network id");
+    return -1;
+  }
+  if (npts <= 0) return 0;
+  if (!d_abundance || !d_rate || !d_density || !d_status || !d_counter) {
+    set_err("null device pointer");
+    return -1;
+  }
+  if (nelect < 1 || nelect > kNets[net].nspec) {
+    set_err("electron index out of range");
+    return -1;
+  }
+  SolveRequest r{};
+  r.abundance = d_abundance;
+  r.rate = d_rate;
+  r.density = d_density;
+  r.npts = npts;
+  r.nspec = kNets[net].nspec;
+  r.nreac = kNets[net].nreac;
+  r.rtol = rtol;
+  r.atol = atol;
+  r.t0 = t_start * SECONDS_IN_YEAR;
+  r.tend = t_end * SECONDS_IN_YEAR;
+  r.tout0 = 1.0e-4 * SECONDS_IN_YEAR;  // calculate_abundances.c:214
+  r.nelect0 = nelect - 1;
+  r.mxstep = 10000000LL;               // calculate_abundances.c:111
+  r.status = d_status;
+  r.stats = d_stats;
+  r.counter = d_counter;
+  r.max_blocks = g_max_blocks;
+  r.mixed = (precision == PDRCHEM_MIXED) ? 1 : 0;
+  int rc = launch_wave_solver(net, r, (hipStream_t)stream);
+  if (rc != 0) {
+    set_err("solver launch failed");
+    return -1;
+  }
+  return 0;
+}
+
+int pdrchem_rates_device(int net, const pdrchem_phys *phys, const double *d_T, const double *d_Td, int nrays,
+                         const double *d_G, const double *d_Av, const double *d_cols, int npts, double *d_rate,
+                         int *d_neg, void *stream) {
+  if (net < 0 || net >= NET_COUNT || !kCompiled[net]) {
+    set_err("unsupported network id");
+    return -1;
+  }
+  if (npts <= 0) return 0;
+  if (!phys || !d_T || !d_Td || !d_G || !d_Av || !d_rate || !d_neg || nrays < 0) {
+    set_err("bad argument to pdrchem_rates_device");
+    return -1;
+  }
+  RateRequest q{};
+  q.T = d_T;
+  q.Td = d_Td;
+  q.G = d_G;
+  q.Av = d_Av;
+  q.cols = d_cols;
+  q.npts = npts;
+  q.nrays = nrays;
+  q.ph = PhysParams{phys->zeta, phys->v_turb, phys->omega, phys->grain_radius, phys->metallicity, phys->g2d};
+  q.rate = d_rate;
+  q.neg = d_neg;
+  if (launch_rates(net, q, (hipStream_t)stream) != 0) {
+    set_err("rate kernel launch failed");
+    return -1;
+  }
+  return 0;
+}
+
+int calculate_abundances_(double *abundance, double *rate, double *density, double *temperature, int *npart,
+                          int *nspec, int *nreac) {
+  (void)temperature;  // stored as T_g but unused by the RHS (odes_*.c)
+  const double reltol = chemistry_module_mp_relative_abundance_tolerance_;
+  const double abstol = chemistry_module_mp_absolute_abundance_tolerance_;
+  const double start_time = maincode_module_mp_start_time_;
+  const double end_time = maincode_module_mp_end_time_;
+  int nelect = global_module_mp_nelect_;
+  const int np = *npart, ns = *nspec, nr = *nreac;
+  if (np <= 0) return 0;
+  int net = pdrchem_network_id(ns, nr);
+  if (net < 0) {
+    set_err("no compiled network for (nspec, nreac)");
+    return 1;
+  }
+  if (nelect <= 0) nelect = ns;  // electron is the last species (read_species.F90:164)
+  std::lock_guard<std::mutex> lock(g_buf.mu);
+  const size_t n_ab = (size_t)np * ns, n_rate = (size_t)np * nr;
+  if (!g_buf.reserve(n_ab, n_rate, (size_t)np)) return 1;
+  hipStream_t s = g_buf.stream;
+  if (!hip_ok(hipMemcpyAsync(g_buf.ab, abundance, n_ab * sizeof(double), hipMemcpyHostToDevice, s), "H2D"))
+    return 1;
+  if (!hip_ok(hipMemcpyAsync(g_buf.rate, rate, n_rate * sizeof(double), hipMemcpyHostToDevice, s), "H2D"))
+    return 1;
+  if (!hip_ok(hipMemcpyAsync(g_buf.dens, density, (size_t)np * sizeof(double), hipMemcpyHostToDevice, s), "H2D"))
+    return 1;
+  if (!hip_ok(hipMemsetAsync(g_buf.counter, 0, sizeof(unsigned int), s), "memset")) return 1;
+  if (pdrchem_solve_device(net, g_buf.ab, g_buf.rate, g_buf.dens, np, reltol, abstol, start_time, end_time,
+                           nelect, g_buf.status, nullptr, g_buf.counter, host_precision(), s) != 0)
+    return 1;
+  std::vector<int> st((size_t)np);
+  if (!hip_ok(hipMemcpyAsync(abundance, g_buf.ab, n_ab * sizeof(double), hipMemcpyDeviceToHost, s), "D2H"))
+    return 1;
+  if (!hip_ok(hipMemcpyAsync(st.data(), g_buf.status, (size_t)np * sizeof(int), hipMemcpyDeviceToHost, s), "D2H"))
+    return 1;
+  if (!hip_ok(hipStreamSynchronize(s), "hipStreamSynchronize")) return 1;
+  int status = 0;
+  for (int n = 0; n < np; n++)
+    if (st[n] != 0) {
+      status = st[n];
+      break;
+    }
+  if (status != 0) log_failures(st, start_time, end_time);
+  return status;
+}
+
+}  // extern "C"

@@ -1,0 +1,201 @@
+"""ctypes binding of libpdrchem.so (declared in include/pdrchem.h).
+
+The product path: every call here goes through the HIP library.  If the
+library is missing or cannot be loaded this module raises -- there is no CPU
+fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.environ.get("PDRCHEM_LIB", os.path.join(PKG_DIR, "libpdrchem.so"))
+
+FP64 = 0   # PDRCHEM_FP64
+MIXED = 1  # PDRCHEM_MIXED
+
+NETWORKS = {"reduced": (33, 329), "full": (215, 2926), "mynetwork": (58, 634)}
+
+# every function and global declared in include/pdrchem.h
+EXPORTED_FUNCTIONS = [
+    "calculate_abundances_", "pdrchem_network_id", "pdrchem_solve_device", "pdrchem_set_precision",
+    "pdrchem_rates_device", "pdrchem_kernel_info", "pdrchem_set_max_blocks", "pdrchem_last_error",
+]
+EXPORTED_GLOBALS = [
+    ("chemistry_module_mp_relative_abundance_tolerance_", C.c_double),
+    ("chemistry_module_mp_absolute_abundance_tolerance_", C.c_double),
+    ("maincode_module_mp_start_time_", C.c_double),
+    ("maincode_module_mp_end_time_", C.c_double),
+    ("global_module_mp_nelect_", C.c_int),
+]
+
+
+class Phys(C.Structure):
+    _fields_ = [("zeta", C.c_double), ("v_turb", C.c_double), ("omega", C.c_double),
+                ("grain_radius", C.c_double), ("metallicity", C.c_double), ("g2d", C.c_double)]
+
+
+def default_phys(**kw) -> Phys:
+    """params.dat defaults (params.dat:11-18; zeta normalised as
+    input_parameters.F90:31, v_turb in cm/s as 3DPDR.F90:47)."""
+    d = dict(zeta=5.0e-17 / 1.3e-17, v_turb=1.0e5, omega=0.42, grain_radius=1.0e-7,
+             metallicity=1.0, g2d=100.0)
+    d.update(kw)
+    return Phys(**d)
+
+
+class PdrChemError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load(path: Optional[str] = None):
+    """Load (once) and return the ctypes handle of libpdrchem.so."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise PdrChemError(f"{p} not built: run __graft_entry__.build() or `make -C 3d-pdr_amd`")
+    L = C.CDLL(p, mode=C.RTLD_GLOBAL)
+    dp, ip, vp = C.POINTER(C.c_double), C.POINTER(C.c_int), C.c_void_p
+    L.calculate_abundances_.argtypes = [dp, dp, dp, dp, ip, ip, ip]
+    L.calculate_abundances_.restype = C.c_int
+    L.pdrchem_network_id.argtypes = [C.c_int, C.c_int]
+    L.pdrchem_solve_device.argtypes = [C.c_int, vp, vp, vp, C.c_int, C.c_double, C.c_double, C.c_double,
+                                       C.c_double, C.c_int, vp, vp, vp, C.c_int, vp]
+    L.pdrchem_rates_device.argtypes = [C.c_int, C.POINTER(Phys), vp, vp, C.c_int, vp, vp, vp, C.c_int, vp, vp,
+                                       vp]
+    L.pdrchem_kernel_info.argtypes = [C.c_int, C.c_int, ip, ip, ip, ip]
+    L.pdrchem_set_max_blocks.argtypes = [C.c_int]
+    L.pdrchem_set_max_blocks.restype = None
+    L.pdrchem_set_precision.argtypes = [C.c_int]
+    L.pdrchem_set_precision.restype = None
+    L.pdrchem_last_error.restype = C.c_char_p
+    _lib = L
+    return L
+
+
+def last_error() -> str:
+    return load().pdrchem_last_error().decode()
+
+
+def set_globals(rtol=None, atol=None, start_time=None, end_time=None, nelect=None):
+    """Write the module globals calculate_abundances_ reads (the role of the
+    Fortran bind(c) variables of modules.F90:165-166, 383, 529-530)."""
+    L = load()
+    vals = dict(chemistry_module_mp_relative_abundance_tolerance_=rtol,
+                chemistry_module_mp_absolute_abundance_tolerance_=atol,
+                maincode_module_mp_start_time_=start_time, maincode_module_mp_end_time_=end_time,
+                global_module_mp_nelect_=nelect)
+    for name, ct in EXPORTED_GLOBALS:
+        v = vals[name]
+        if v is not None:
+            ct.in_dll(L, name).value = v
+
+
+def get_globals() -> dict:
+    L = load()
+    return {name: ct.in_dll(L, name).value for name, ct in EXPORTED_GLOBALS}
+
+
+def calculate_abundances(abundance: np.ndarray, rate: np.ndarray, density: np.ndarray,
+                         temperature: np.ndarray, precision: Optional[int] = None) -> int:
+    """The reference call ``CALL CALCULATE_ABUNDANCES(ABUNDANCE, RATE, DENSITY,
+    TEMPERATURE, NPART, NSPEC, NREAC)`` (3DPDR.F90:498): abundance
+    [npart, nspec] is overwritten in place; returns the status."""
+    L = load()
+    for a in (abundance, rate, density, temperature):
+        if a.dtype != np.float64 or not a.flags.c_contiguous:
+            raise PdrChemError("arrays must be C-contiguous float64")
+    npart, nspec = abundance.shape
+    nreac = rate.shape[1]
+    if rate.shape[0] != npart or density.shape[0] != npart or temperature.shape[0] != npart:
+        raise PdrChemError("inconsistent point counts")
+    if precision is not None:
+        L.pdrchem_set_precision(precision)
+    dp = C.POINTER(C.c_double)
+    n, s, r = C.c_int(npart), C.c_int(nspec), C.c_int(nreac)
+    return L.calculate_abundances_(abundance.ctypes.data_as(dp), rate.ctypes.data_as(dp),
+                                   density.ctypes.data_as(dp), temperature.ctypes.data_as(dp),
+                                   C.byref(n), C.byref(s), C.byref(r))
+
+
+def network_id(name_or_shape) -> int:
+    if isinstance(name_or_shape, str):
+        nspec, nreac = NETWORKS[name_or_shape]
+    else:
+        nspec, nreac = name_or_shape
+    return load().pdrchem_network_id(nspec, nreac)
+
+
+def kernel_info(net: int, precision: int = MIXED) -> dict:
+    regs, lds, scr, bpc = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+    if load().pdrchem_kernel_info(net, precision, C.byref(regs), C.byref(lds), C.byref(scr), C.byref(bpc)):
+        raise PdrChemError(last_error())
+    return dict(num_regs=regs.value, lds_bytes=lds.value, scratch_bytes=scr.value, blocks_per_cu=bpc.value)
+
+
+# ---------------------------------------------------------------------------
+# device-resident interface over torch tensors (torch provides device memory
+# and streams; the kernels are the library's)
+# ---------------------------------------------------------------------------
+
+def _ptr(t):
+    return C.c_void_p(t.data_ptr()) if t is not None else C.c_void_p(0)
+
+
+def solve_device(net: int, abundance, rate, density, rtol=1e-8, atol=1e-30, t_start=0.0, t_end=1e7,
+                 nelect=None, status=None, stats=None, counter=None, precision=MIXED, stream=None):
+    """Queue a batched solve on torch device tensors (abundance [n, nspec]
+    float64 in/out, rate [n, nreac], density [n]).  Returns (status, stats)."""
+    import torch
+    L = load()
+    n = abundance.shape[0]
+    dev = abundance.device
+    if status is None:
+        status = torch.empty(n, dtype=torch.int32, device=dev)
+    if counter is None:
+        counter = torch.zeros(1, dtype=torch.int32, device=dev)
+    else:
+        counter.zero_()
+    if nelect is None:
+        nelect = abundance.shape[1]
+    for t in (abundance, rate, density):
+        if t.dtype != torch.float64 or not t.is_contiguous():
+            raise PdrChemError("device tensors must be contiguous float64")
+    s = stream if stream is not None else torch.cuda.current_stream(dev)
+    rc = L.pdrchem_solve_device(net, _ptr(abundance), _ptr(rate), _ptr(density), n, rtol, atol, t_start, t_end,
+                                nelect, _ptr(status), _ptr(stats), _ptr(counter), precision,
+                                C.c_void_p(s.cuda_stream))
+    if rc != 0:
+        raise PdrChemError(last_error())
+    return status, stats
+
+
+def rates_device(net: int, T, Td, G, Av, cols=None, phys: Optional[Phys] = None, out=None, neg=None,
+                 stream=None):
+    """Device rate evaluation: T, Td [n]; G, Av [n, nrays]; cols [n, nrays, 5]
+    (H2, HD, CO, C, S) or None.  Returns (rate [n, nreac], neg counter)."""
+    import torch
+    L = load()
+    n = T.shape[0]
+    nrays = G.shape[1] if G.dim() > 1 else 1
+    nreac = [v for v in NETWORKS.values()][net][1]
+    if out is None:
+        out = torch.empty((n, nreac), dtype=torch.float64, device=T.device)
+    if neg is None:
+        neg = torch.zeros(1, dtype=torch.int32, device=T.device)
+    ph = phys or default_phys()
+    s = stream if stream is not None else torch.cuda.current_stream(T.device)
+    rc = L.pdrchem_rates_device(net, C.byref(ph), _ptr(T), _ptr(Td), nrays, _ptr(G), _ptr(Av), _ptr(cols), n,
+                                _ptr(out), _ptr(neg), C.c_void_p(s.cuda_stream))
+    if rc != 0:
+        raise PdrChemError(last_error())
+    return out, neg

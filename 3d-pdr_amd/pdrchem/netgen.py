@@ -140,6 +140,7 @@ class Network:
     reactions: List[Reaction]
     labels: Dict[str, int]    # 1-based species labels (0 = absent) as READ_SPECIES
     nr_indices: Dict[str, int]  # NRGR, NRH2, ... (1-based, 0 = absent)
+    xlambda_spline: int = 0   # see xlambda_mode()
 
     @property
     def nspec(self) -> int:
@@ -350,6 +351,27 @@ def nr_indices(reactions: Sequence[Reaction]) -> Dict[str, int]:
     return out
 
 
+def xlambda_mode(reactions: Sequence[Reaction]) -> int:
+    """Whether XLAMBDA (shield.F90:525-549) interpolates tau(lambda)/tau(V)
+    with the cubic spline (1) or, effectively, piecewise linearly (0).
+
+    XLAMBDA computes its spline second derivatives only while the shared
+    START flag is set (shield.F90:543); COSHIELD clears START the first time
+    it runs (shield.F90:398-401), and COPDRATE calls COSHIELD before SCATTER
+    -> XLAMBDA (shield.F90:104).  So if the first reaction whose rate calls
+    XLAMBDA is a CO photodissociation, the derivative table stays zero for
+    the whole run and the spline degenerates to linear interpolation; if an
+    H2/HD photodissociation (H2PDRATE -> SCATTER, shield.F90:51) comes first,
+    the table is filled.  All three shipped networks list CO before H2, and
+    the amdflang-built reference rate code confirms the linear behaviour."""
+    for r in reactions:
+        if r.rtype in (T_PHOTO_H2, T_PHOTO_HD):
+            return 1
+        if r.rtype == T_PHOTO_CO:
+            return 0
+    return 1
+
+
 def compile_network(name: str, species_path: str, rates_path: str) -> Network:
     species, labels = read_species(species_path)
     reactions = read_rates(rates_path)
@@ -357,7 +379,7 @@ def compile_network(name: str, species_path: str, rates_path: str) -> Network:
         r.rtype = classify(r)
         r.gflag = 1 if r.reactants[0][:1] == "G" else 0
     build_ode_terms(species, reactions)
-    return Network(name, species, reactions, labels, nr_indices(reactions))
+    return Network(name, species, reactions, labels, nr_indices(reactions), xlambda_mode(reactions))
 
 
 # --------------------------------------------------------------------------
@@ -371,6 +393,7 @@ def write_tab(net: Network, path: str) -> None:
     with open(path, "w") as fh:
         fh.write(f"# pdrchem network table v1: {net.name}\n")
         fh.write(f"N {net.nspec} {net.nreac}\n")
+        fh.write(f"F {net.xlambda_spline}\n")
         for s in net.species:
             fh.write(f"S {s.index} {s.charge} {s.abundance:.17g} {s.mass:.17g} {s.name}\n")
         for r in net.reactions:
@@ -393,6 +416,7 @@ def to_json(net: Network) -> dict:
         ],
         "labels": net.labels,
         "nr_indices": net.nr_indices,
+        "xlambda_spline": net.xlambda_spline,
     }
 
 
@@ -404,7 +428,8 @@ def from_json(d: dict) -> Network:
         rd.pop("type_name", None)
         st = {int(k): v for k, v in rd.pop("stoich").items()}
         reactions.append(Reaction(**rd, stoich=st))
-    return Network(d["name"], species, reactions, d["labels"], d["nr_indices"])
+    return Network(d["name"], species, reactions, d["labels"], d["nr_indices"],
+                   d.get("xlambda_spline", 1))
 
 
 NETWORK_FILES = {

@@ -1,0 +1,117 @@
+/*
+ * pdrchem.h -- C ABI of libpdrchem.so, the MI355X-native per-grid-point
+ * chemistry solve for 3D-PDR.
+ *
+ * Drop-in entry point: calculate_abundances_ replaces the reference's
+ * CVODE-based implementation with the same symbol, arguments and in-place
+ * semantics; a Fortran host keeps its implicit-interface CALL.  Everything
+ * else is additive: batched device-pointer entry points for callers that keep
+ * state in HBM, device rate evaluation, and introspection.
+ *
+ * No function here takes or returns a torch type; pointers are plain C
+ * pointers, sizes are int.
+ */
+#ifndef PDRCHEM_H
+#define PDRCHEM_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------------
+ * Reference interface
+ * ------------------------------------------------------------------- */
+
+/* Replaces calculate_abundances_ (calculate_abundances.c:64-65), called from
+ * 3DPDR.F90:498, :623, :1929 as
+ *   CALL CALCULATE_ABUNDANCES(ABUNDANCE, RATE, DENSITY, TEMPERATURE, NPART,
+ *                             NSPEC, NREAC)
+ * abundance[npart*nspec]  in/out, point-major (index n*nspec+i); on return
+ *                         abundances < atol are 0 and slot nelect-1 holds
+ *                         x_e from the last RHS evaluation
+ *                         (calculate_abundances.c:308-317)
+ * rate[npart*nreac]       rate coefficients per point (read only)
+ * density[npart]          n_H per point (read only)
+ * temperature[npart]      gas temperature (read only; unused by the RHS)
+ * Tolerances, start/end time (years) and the electron index are read from
+ * the globals below, as the reference does (calculate_abundances.c:78-91).
+ * The network is chosen at run time from (nspec, nreac).
+ * Returns 0 on success, 1 on a set-up failure (unsupported network, device
+ * error), else the CVODE-style flag of a failed point (-1 too much work,
+ * -3 error-test failures, -4 convergence failures, ...), like the reference. */
+int calculate_abundances_(double *abundance, double *rate, double *density, double *temperature,
+                          int *npart, int *nspec, int *nreac);
+
+/* Globals read by calculate_abundances_.  In the reference they are owned by
+ * Fortran modules with bind(c) names (modules.F90:165-166, 383, 529-530); the
+ * library carries weak definitions with the reference's defaults
+ * (3DPDR.F90:452, 462-463; params.dat end time 1e7 yr) that a Fortran host's
+ * definitions replace at link/load time. */
+extern double chemistry_module_mp_relative_abundance_tolerance_;
+extern double chemistry_module_mp_absolute_abundance_tolerance_;
+extern double maincode_module_mp_start_time_;
+extern double maincode_module_mp_end_time_;
+extern int global_module_mp_nelect_;
+
+/* ---------------------------------------------------------------------
+ * Batched device interface (no counterpart in the reference; used by hosts
+ * that keep state resident in HBM and by the benchmark)
+ * ------------------------------------------------------------------- */
+
+/* network id for (nspec, nreac); -1 if no compiled kernel matches */
+int pdrchem_network_id(int nspec, int nreac);
+
+/* precision of the iteration matrix (the residual, Newton update and error
+ * test are FP64 in both) */
+#define PDRCHEM_FP64 0   /* plain form: J and its scaled inverse in FP64 */
+#define PDRCHEM_MIXED 1  /* J and the scaled inverse in FP32 */
+
+/* Solve npts independent points already in device memory, asynchronously
+ * on `stream` (a hipStream_t; NULL = default stream).  Same per-point
+ * semantics as calculate_abundances_.  t_start/t_end in years; nelect is
+ * 1-based.  d_status[npts] receives each point's flag; d_stats (optional)
+ * receives [npts][8] counters: nst nfe nje nsetups nni ncfn netf lastflag.
+ * d_counter: one zeroed unsigned int of device memory (work queue).
+ * precision: PDRCHEM_FP64 or PDRCHEM_MIXED.
+ * Returns 0 if the launch was queued. */
+int pdrchem_solve_device(int net, double *d_abundance, const double *d_rate, const double *d_density,
+                         int npts, double rtol, double atol, double t_start, double t_end, int nelect,
+                         int *d_status, int *d_stats, unsigned int *d_counter, int precision, void *stream);
+
+/* Precision used by calculate_abundances_ (default PDRCHEM_MIXED; the
+ * environment variable PDRCHEM_PRECISION=fp64 selects PDRCHEM_FP64). */
+void pdrchem_set_precision(int precision);
+
+/* Physical parameters of the rate evaluation (params.dat,
+ * input_parameters.F90:19-58): zeta in units of 1.3e-17 s^-1, v_turb in cm/s. */
+typedef struct {
+  double zeta, v_turb, omega, grain_radius, metallicity, g2d;
+} pdrchem_phys;
+
+/* Batched CALCULATE_REACTION_RATES (calc_reac_rates.F90:16-401) on device
+ * memory: T, Td [npts]; G, Av [npts*nrays]; cols [npts*nrays*5] column
+ * densities of H2, HD, CO, C, S per ray (may be NULL = zero columns);
+ * rate [npts*nreac] out; d_neg: one zeroed int that counts points with a
+ * negative rate (the reference STOPs on those, calc_reac_rates.F90:387-390).
+ * Asynchronous on `stream`. */
+int pdrchem_rates_device(int net, const pdrchem_phys *phys, const double *d_T, const double *d_Td,
+                         int nrays, const double *d_G, const double *d_Av, const double *d_cols,
+                         int npts, double *d_rate, int *d_neg, void *stream);
+
+/* Kernel resource usage of the solver for a network and precision
+ * (registers per lane, LDS bytes per block, scratch bytes per lane, blocks
+ * per CU).  0 = ok. */
+int pdrchem_kernel_info(int net, int precision, int *num_regs, int *lds_bytes, int *scratch_bytes,
+                        int *blocks_per_cu);
+
+/* Upper bound on resident blocks for the persistent solver (0 = fill the
+ * device).  For experiments and tests of the work queue. */
+void pdrchem_set_max_blocks(int max_blocks);
+
+/* Text of the last error of this thread (empty if none). */
+const char *pdrchem_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

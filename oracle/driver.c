@@ -1,0 +1,149 @@
+/*
+ * calculate_abundances_ on the CPU (calculate_abundances.c:64-354):
+ * per-point CVODE-equivalent solve with the reference's output schedule,
+ * retry policy and write-back.  OpenMP dynamic schedule over points as the
+ * reference (calculate_abundances.c:124-130).  Test infrastructure (see
+ * pdr_oracle.h).
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "cvode_bdf.h"
+#include "pdr_oracle.h"
+
+#define SECONDS_IN_YEAR 3.1556926e7
+
+typedef struct {
+  const pdr_net *net;
+  const double *rate;
+  double n_H;
+  double x_e; /* charge sum of the most recent RHS evaluation */
+} chem_ud;
+
+static int chem_f(double t, const double *y, double *ydot, void *ud) {
+  (void)t;
+  chem_ud *c = (chem_ud *)ud;
+  pdr_rhs(c->net, c->rate, c->n_H, y, ydot, &c->x_e);
+  return 0;
+}
+
+static int chem_jac(double t, const double *y, double *J, void *ud) {
+  (void)t;
+  chem_ud *c = (chem_ud *)ud;
+  pdr_jac(c->net, c->rate, c->n_H, y, J);
+  return 0;
+}
+
+static void fill_stats(pdr_cvode_stats *st, const bdf_mem *m, int flag) {
+  if (!st) return;
+  st->nst = m->nst; st->nfe = m->nfe; st->nfeDQ = m->nfeDQ; st->nsetups = m->nsetups;
+  st->nje = m->nje; st->nni = m->nni; st->ncfn = m->ncfn; st->netf = m->netf;
+  st->qlast = m->qu; st->hlast = m->hu; st->flag = flag;
+}
+
+/* one grid point: returns the status contribution of calculate_abundances.c */
+static int solve_point(const pdr_net *net, double *ab, const double *rate, double n_H,
+                       double reltol, double abstol, double start_time, double end_time,
+                       int nelect0, int mode, pdr_cvode_stats *st) {
+  const int neq = net->neq;
+  const long mxstep = 10000000L;
+  chem_ud ud = {net, rate, n_H, ab[nelect0]};
+  bdf_mem *m = bdf_create(neq, chem_f, chem_jac, &ud, reltol, abstol, mxstep, mode);
+  if (!m) return 1;
+  double t0 = start_time * SECONDS_IN_YEAR;
+  double tend = end_time * SECONDS_IN_YEAR;
+  bdf_init(m, t0, ab);
+  double *y = malloc(sizeof(double) * neq);
+  memcpy(y, ab, sizeof(double) * neq);
+  double tout = 1.0e-4 * SECONDS_IN_YEAR;
+  double t = t0;
+  int nfails = 0, status = 0, flag = 0;
+  do {
+    flag = bdf_solve(m, tout, y, &t);
+    if (flag != 0) {
+      nfails++;
+      if (flag == PDR_CV_TOO_MUCH_WORK) {
+        if (nfails >= 5) status = flag;  /* mxstep re-set to the same value */
+      } else if (flag == PDR_CV_CONV_FAILURE) {
+        if (nfails >= 5) status = flag;
+      } else {
+        status = flag;
+        nfails = 5;
+      }
+    }
+    tout = tout * 10;
+    if (tout > tend) tout = tend;
+  } while (t < tout && nfails < 5);
+  for (int i = 0; i < neq; i++) {
+    ab[i] = y[i];
+    if (ab[i] < abstol) ab[i] = 0;
+  }
+  ab[nelect0] = ud.x_e;
+  fill_stats(st, m, flag);
+  free(y);
+  bdf_free(m);
+  return status;
+}
+
+int pdr_calculate_abundances(double *abundance, const double *rate, const double *density,
+                             const double *temperature, int npart, int nspec, int nreac,
+                             const pdr_net *net, double reltol, double abstol,
+                             double start_time, double end_time, int nelect, int mode,
+                             int nthreads, pdr_cvode_stats *stats) {
+  (void)temperature;
+  if (!net || nspec != net->nspec || nreac != net->nreac) return 1;
+  int status = 0;
+#ifdef _OPENMP
+  if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic) num_threads(nthreads)
+#endif
+  for (int n = 0; n < npart; n++) {
+    int s = solve_point(net, abundance + (size_t)n * nspec, rate + (size_t)n * nreac, density[n],
+                        reltol, abstol, start_time, end_time, nelect - 1, mode,
+                        stats ? stats + n : NULL);
+    if (s != 0) {
+#pragma omp critical(pdr_status)
+      status = s;
+    }
+  }
+  return status;
+}
+
+int pdr_integrate_point(const pdr_net *net, const double *rate, double n_H, const double *y0,
+                        double t0, int ntout, const double *tout, double *yout,
+                        const pdr_cvode_opts *o, pdr_cvode_stats *st) {
+  chem_ud ud = {net, rate, n_H, 0.0};
+  bdf_mem *m = bdf_create(net->neq, chem_f, chem_jac, &ud, o->rtol, o->atol, o->mxstep, o->mode);
+  if (!m) return 1;
+  bdf_init(m, t0, y0);
+  int flag = 0;
+  for (int k = 0; k < ntout; k++) {
+    double t;
+    flag = bdf_solve(m, tout[k], yout + (size_t)k * net->neq, &t);
+    if (flag != 0) break;
+  }
+  fill_stats(st, m, flag);
+  bdf_free(m);
+  return flag;
+}
+
+int pdr_integrate_generic(int n, pdr_rhs_fn f, pdr_jac_fn jac, void *ud, const double *y0,
+                          double t0, int ntout, const double *tout, double *yout,
+                          const pdr_cvode_opts *o, pdr_cvode_stats *st) {
+  bdf_mem *m = bdf_create(n, f, jac, ud, o->rtol, o->atol, o->mxstep, o->mode);
+  if (!m) return 1;
+  bdf_init(m, t0, y0);
+  int flag = 0;
+  for (int k = 0; k < ntout; k++) {
+    double t;
+    flag = bdf_solve(m, tout[k], yout + (size_t)k * n, &t);
+    if (flag != 0) break;
+  }
+  fill_stats(st, m, flag);
+  bdf_free(m);
+  return flag;
+}

@@ -1,0 +1,143 @@
+/*
+ * pdr_oracle.h -- CPU restatement of 3D-PDR's per-grid-point chemistry path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP
+ * product path: only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it.  The product library never links it.
+ *
+ * Contents and the reference code each part restates:
+ *   pdr_net_*        network tables written by the network compiler
+ *                    (read_species.F90:91-190, read_rates.F90:53-91)
+ *   pdr_rates        CALCULATE_REACTION_RATES  calc_reac_rates.F90:16-401
+ *                    with H2PDRATE/COPDRATE/CIPDRATE/SIPDRATE, H2SHIELD1,
+ *                    COSHIELD, SCATTER, XLAMBDA, LBAR (shield.F90),
+ *                    H2_FORMATION_RATE (h2_form.F90:8-113) and natural cubic
+ *                    splines (spline.F90, restated from the mathematics)
+ *   pdr_rhs          ydot = f(x), the MakeRates RHS of odes_*.c:26-148
+ *   pdr_jac          exact dense Jacobian (jacobian_*.c plus the x_e chain
+ *                    terms the shipped file omits)
+ *   pdr_cvode_*      CVODE 2.7.0 (SUNDIALS 2.5.0) BDF + Newton + dense LU,
+ *                    restated from its published algorithm (Hindmarsh et al.
+ *                    2005, ACM TOMS 31:363; CVODE user guide, "Mathematical
+ *                    considerations"); the library itself is absent from the
+ *                    reference tree (.MISSING_LARGE_BLOBS:1)
+ *   pdr_calculate_abundances   calculate_abundances_ calculate_abundances.c:64-354
+ */
+#ifndef PDR_ORACLE_H
+#define PDR_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  PT_THERMAL = 0, PT_H2FORM, PT_PAH, PT_PHOTO, PT_PHOTO_H2, PT_PHOTO_HD,
+  PT_PHOTO_CO, PT_PHOTO_CI, PT_PHOTO_SI, PT_CRP, PT_CRPHOT, PT_FREEZE,
+  PT_ELFRZE, PT_CRH, PT_PHOTD, PT_THERM, PT_GRAIN, PT_XR
+};
+
+typedef struct {
+  int nspec, nreac, neq;
+  int xlambda_spline;   /* 1: XLAMBDA uses its cubic spline; 0: linear (see netgen) */
+  int *charge;          /* [nspec] */
+  double *abund0;       /* [nspec] species-file initial abundances */
+  double *mass;         /* [nspec] */
+  int *rtype, *dup, *gflag, *fa, *fb, *nh; /* [nreac] */
+  double *alpha, *beta, *gamma, *rtmin, *rtmax; /* [nreac] */
+  int *st_ptr;          /* [nreac+1] CSR of net stoichiometry */
+  int *st_idx, *st_coef;
+} pdr_net;
+
+int pdr_net_load(const char *path, pdr_net *net);
+void pdr_net_free(pdr_net *net);
+pdr_net *pdr_net_open(const char *path); /* heap handle for ctypes */
+void pdr_net_close(pdr_net *net);
+int pdr_net_nspec(const pdr_net *net);
+int pdr_net_nreac(const pdr_net *net);
+
+/* Global physical parameters (params.dat, input_parameters.F90:19-58) */
+typedef struct {
+  double zeta;          /* cosmic-ray rate in units of 1.3e-17 s^-1 (input_parameters.F90:31) */
+  double v_turb;        /* turbulent velocity, cm/s (3DPDR.F90:47) */
+  double omega;         /* dust albedo */
+  double grain_radius;  /* as read from params.dat */
+  double metallicity;
+  double g2d;           /* gas-to-dust ratio */
+} pdr_phys;
+
+/* Per-point inputs of CALCULATE_REACTION_RATES.  cols[k*5 + c] holds the
+ * column density along ray k of H2, HD, CO, C, S (c = 0..4). */
+int pdr_rates(const pdr_net *net, const pdr_phys *ph, double T, double Td,
+              int nrays, const double *G, const double *Av, const double *cols,
+              double *rate);
+
+/* Batched: arrays are point-major; returns number of points with a negative
+ * rate (the reference STOPs on those, calc_reac_rates.F90:387-390). */
+int pdr_rates_batch(const pdr_net *net, const pdr_phys *ph, int npts,
+                    const double *T, const double *Td, int nrays,
+                    const double *G, const double *Av, const double *cols,
+                    double *rate);
+
+void pdr_rhs(const pdr_net *net, const double *rate, double n_H,
+             const double *x, double *ydot, double *x_e);
+void pdr_jac(const pdr_net *net, const double *rate, double n_H,
+             const double *x, double *J /* row-major neq*neq */);
+
+/* integrator method bits.  0 is the reference configuration: CVODE's
+ * difference-quotient Jacobian and a plain dense LU.  The HIP solver runs
+ * PDR_MODE_GPU: exact Jacobian, iteration matrix scaled by the error weights
+ * (row i by w_i, column j by 1/w_j) and three steps of iterative refinement
+ * with an FP64 residual per linear solve; PDR_MODE_GPU_MIXED also rounds the
+ * saved J and the factorisation to FP32. */
+#define PDR_MODE_EXACT_JAC 1
+#define PDR_MODE_SCALED 2
+#define PDR_MODE_REFINE(k) ((k) << 2) /* k = 0..3 refinement steps */
+#define PDR_MODE_F32 16               /* saved J rounded to FP32; scaled matrix factored in FP32 */
+#define PDR_MODE_GPU (PDR_MODE_EXACT_JAC | PDR_MODE_SCALED | PDR_MODE_REFINE(3))
+#define PDR_MODE_GPU_MIXED (PDR_MODE_GPU | PDR_MODE_F32)
+
+/* integrator options / statistics */
+typedef struct {
+  double rtol, atol;
+  long mxstep;
+  int mode;             /* PDR_MODE_* bits; 0 = the reference configuration */
+  int maxord;           /* 5 */
+} pdr_cvode_opts;
+
+typedef struct {
+  long nst, nfe, nfeDQ, nsetups, nje, nni, ncfn, netf;
+  int qlast;
+  double hlast;
+  int flag;             /* last CVode return flag */
+} pdr_cvode_stats;
+
+/* calculate_abundances_ (calculate_abundances.c:64-354) on the CPU:
+ * abundance[npart*nspec] in/out, rate[npart*nreac], density[npart],
+ * temperature[npart]; start/end time in years; nelect 1-based.
+ * stats (optional) receives per-point counters.  nthreads<=0: OpenMP default. */
+int pdr_calculate_abundances(double *abundance, const double *rate,
+                             const double *density, const double *temperature,
+                             int npart, int nspec, int nreac,
+                             const pdr_net *net, double reltol, double abstol,
+                             double start_time, double end_time, int nelect,
+                             int mode, int nthreads, pdr_cvode_stats *stats);
+
+/* Single integration of one point to a list of output times (seconds);
+ * writes y at each time into yout[ntout*neq].  For integrator unit tests. */
+int pdr_integrate_point(const pdr_net *net, const double *rate, double n_H,
+                        const double *y0, double t0, int ntout,
+                        const double *tout, double *yout,
+                        const pdr_cvode_opts *opts, pdr_cvode_stats *st);
+
+/* Generic ODE integration with the same integrator (for closed-form tests) */
+typedef int (*pdr_rhs_fn)(double t, const double *y, double *ydot, void *ud);
+typedef int (*pdr_jac_fn)(double t, const double *y, double *J, void *ud);
+int pdr_integrate_generic(int n, pdr_rhs_fn f, pdr_jac_fn jac, void *ud,
+                          const double *y0, double t0, int ntout,
+                          const double *tout, double *yout,
+                          const pdr_cvode_opts *opts, pdr_cvode_stats *st);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
