@@ -2,6 +2,7 @@
 // launchers.  See bdf_wave_impl.h for the algorithm and layout.
 #include <hip/hip_runtime.h>
 
+#include "wave_ops.h"
 #include "gen/net_reduced.h"
 #include "pdr_internal.h"
 

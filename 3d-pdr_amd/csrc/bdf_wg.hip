@@ -1,0 +1,127 @@
+// Workgroup-per-system BDF kernels (networks of more than 32 equations),
+// instantiated per network, and their launchers.  See bdf_wg_impl.h for the
+// algorithm and layout.
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+
+#include "wave_ops.h"
+#include "gen/wg_full.h"
+#include "gen/wg_mynetwork.h"
+#include "pdr_internal.h"
+
+namespace bdfg_full {
+namespace NET = pdrwg_full;
+#include "bdf_wg_impl.h"
+}  // namespace bdfg_full
+
+namespace bdfg_mynetwork {
+namespace NET = pdrwg_mynetwork;
+#include "bdf_wg_impl.h"
+}  // namespace bdfg_mynetwork
+
+namespace pdrchem {
+
+namespace {
+
+// Jacobian workspace: one slot of NJ entries per resident workgroup, grown
+// on demand and kept for the life of the process
+struct JacWorkspace {
+  std::mutex mu;
+  void *ptr = nullptr;
+  size_t bytes = 0;
+  void *get(size_t need) {
+    std::lock_guard<std::mutex> lock(mu);
+    if (need > bytes) {
+      if (ptr) (void)hipFree(ptr);
+      ptr = nullptr;
+      bytes = 0;
+      if (hipMalloc(&ptr, need) != hipSuccess) return nullptr;
+      bytes = need;
+    }
+    return ptr;
+  }
+};
+JacWorkspace g_jac;
+
+template <typename Args>
+void fill_args(Args &a, const SolveRequest &r) {
+  a.abundance = r.abundance;
+  a.rate = r.rate;
+  a.density = r.density;
+  a.nsys = r.npts;
+  a.ab_stride = r.nspec;
+  a.rate_stride = r.nreac;
+  a.rtol = r.rtol;
+  a.atol = r.atol;
+  a.t0 = r.t0;
+  a.tend = r.tend;
+  a.tout0 = r.tout0;
+  a.nelect0 = r.nelect0;
+  a.mxstep = r.mxstep;
+  a.status = r.status;
+  a.stats = r.stats;
+  a.counter = r.counter;
+}
+
+template <typename MT, typename Args>
+int launch(void (*kern)(Args, MT *), const SolveRequest &r, int nj, hipStream_t s) {
+  Args a;
+  fill_args(a, r);
+  int dev = 0, ncu = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return -1;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return -1;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 256, 0) != hipSuccess) return -1;
+  if (per_cu < 1) per_cu = 1;
+  long long full = (long long)ncu * per_cu;
+  long long nb = r.npts < full ? r.npts : full;
+  if (r.max_blocks > 0 && nb > r.max_blocks) nb = r.max_blocks;
+  if (nb < 1) nb = 1;
+  MT *jac = (MT *)g_jac.get((size_t)nb * nj * sizeof(MT));
+  if (!jac) return -1;
+  hipLaunchKernelGGL(kern, dim3((unsigned)nb), dim3(256), 0, s, a, jac);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+template <typename Kern>
+int info_of(Kern fn, KernelInfo *info) {
+  hipFuncAttributes attr;
+  if (hipFuncGetAttributes(&attr, (const void *)fn) != hipSuccess) return -1;
+  info->num_regs = attr.numRegs;
+  info->shared_bytes = (int)attr.sharedSizeBytes;
+  info->local_bytes = (int)attr.localSizeBytes;
+  info->max_threads = attr.maxThreadsPerBlock;
+  int per_cu = 0;
+  (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, 0);
+  info->blocks_per_cu = per_cu;
+  return 0;
+}
+
+}  // namespace
+
+int launch_wg_solver(int net, const SolveRequest &r, hipStream_t s) {
+  switch (net) {
+    case NET_FULL:
+      return r.mixed ? launch(bdfg_full::bdf_wg_kernel<float>, r, pdrwg_full::NJ, s)
+                     : launch(bdfg_full::bdf_wg_kernel<double>, r, pdrwg_full::NJ, s);
+    case NET_MYNETWORK:
+      return r.mixed ? launch(bdfg_mynetwork::bdf_wg_kernel<float>, r, pdrwg_mynetwork::NJ, s)
+                     : launch(bdfg_mynetwork::bdf_wg_kernel<double>, r, pdrwg_mynetwork::NJ, s);
+    default:
+      return -2;
+  }
+}
+
+int wg_solver_info(int net, int mixed, KernelInfo *info) {
+  switch (net) {
+    case NET_FULL:
+      return mixed ? info_of(bdfg_full::bdf_wg_kernel<float>, info) : info_of(bdfg_full::bdf_wg_kernel<double>, info);
+    case NET_MYNETWORK:
+      return mixed ? info_of(bdfg_mynetwork::bdf_wg_kernel<float>, info)
+                   : info_of(bdfg_mynetwork::bdf_wg_kernel<double>, info);
+    default:
+      return -2;
+  }
+}
+
+}  // namespace pdrchem

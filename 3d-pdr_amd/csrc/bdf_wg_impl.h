@@ -1,0 +1,643 @@
+// Batched variable-order BDF, one stiff system per workgroup of 256 lanes
+// (four wavefronts), for networks of more than 32 equations (the full
+// network, 214 equations, and MYNETWORK, 57).
+//
+// Included once per network inside a namespace that defines `namespace NET`
+// (a generated csrc/gen/wg_<name>.h).  Replaces, for one grid point per
+// workgroup, the reference's per-point CVODE solve (calculate_abundances.c:
+// 132-326), the generated RHS (odes_full.c:26-700) and CVODE's dense LU.
+// The integrator is the CVODE restatement of bdf_core.h; this file supplies
+// the workgroup as its team.
+//
+// Linear algebra (tables and their derivation: pdrchem/sparse.py and
+// pdrchem/wggen.py).  The Newton matrix is taken in bordered form, with x_e
+// as unknown NEQ, so that it stays as sparse as the network:
+//     [ I - gamma J0   -gamma g ] [dx ]   [r]
+//     [   -q^T            1     ] [dxe] = [0]
+// scaled by the error weights (row i by w_i, column j by 1/w_j).  A static
+// elimination order splits it into a head (pivots eliminated level by level
+// without pivoting, multipliers in place) and a dense tail of NT <= 64
+// unknowns.  The tail's Schur complement Tl - L_TH U_HT is formed with
+// matrix-core (MFMA) products, one 32x32 tile per wavefront, and inverted by
+// Gauss-Jordan with partial pivoting in wavefront 0, whose lanes keep one row
+// of the inverse each in registers.  A solve is: forward substitution over
+// the head levels, the tail rows' gather of head values, the tail inverse
+// applied as one matrix-vector product, the head rows' scatter of tail
+// values, and back substitution over the head levels.  MT is the matrix
+// type (double: plain FP64 form; float: mixed form); the RHS, Newton
+// iteration, error test and every n-vector stay FP64.
+//
+// Memory: n-vectors are held one row per lane (lane i < NB); step-control
+// scalars in the workgroup's LDS record.  LDS holds the factor vector X, the
+// solve vector B, the weights W, chunk partials P and one region that is, by
+// phase: the fluxes F (RHS), the flux partials D (Jacobian evaluation), and
+// the head entries Hv followed by the tail block Tl (factorisation and
+// solves; Tl lies over F, which is dead while the factorisation runs).  The
+// Jacobian of the last evaluation (CVODE keeps it for later setups) lives in
+// a per-workgroup slot of device memory.
+
+#include "wave_ops.h"
+
+using namespace pdrw;
+
+#include "bdf_core.h"
+
+constexpr int BLOCK = NET::BLOCK;
+constexpr int NWAVE = BLOCK / 64;
+static_assert(BLOCK == 256, "the Schur tiles assume four wavefronts");
+static_assert(NET::NB <= BLOCK, "one row per lane");
+static_assert(NET::NT <= 64, "the tail is one wavefront wide");
+
+// LDS region sizes: Hv[NHV] and the zero slot (padded to TBASE entries),
+// then max(F[NSLOT+1] FP64, Tl[NT*NT]); D[2*NSLOT+1] FP64 over all of it
+template <typename MT>
+struct RegionBytes {
+  static constexpr size_t hvp = sizeof(MT) * NET::TBASE;
+  static constexpr size_t f = sizeof(double) * (NET::NSLOT + 1);
+  static constexpr size_t tl = sizeof(MT) * 64 * (64 + 1);  // tail block or inverse, stride 65
+  static constexpr size_t d = sizeof(double) * (2 * NET::NSLOT + 1);
+  static constexpr size_t ftl = f > tl ? f : tl;
+  static constexpr size_t total = (hvp + ftl > d) ? hvp + ftl : d;
+  static_assert(NET::TBASE % 4 == 0, "F and Tl start 16-byte aligned");
+};
+
+template <typename MT>
+struct WgLDS {
+  double X[NET::NB + 1];     // factor vector: x_0..x_{NEQ-1}, x_e, 1
+  double B[NET::NB + 1];     // solve vector; B[NB] = 0
+  double W[NET::NB];         // error weights the matrix was scaled with
+  double P[NET::NV + 1];     // RHS chunk partials (P[NV] = 0), tail gather partials
+  double V[64], V2[64];      // tail inverse exchange
+  double red[2][NWAVE];      // team reductions (double-buffered)
+  MT dinv[NET::NB];          // reciprocals of the head pivots
+  int flag;
+  BdfU u;
+  // Hv[NHV] | zero | Tl[NT*NT] over F[NSLOT+1];  D[2*NSLOT+1] over all of it
+  alignas(16) unsigned char region[RegionBytes<MT>::total];
+};
+
+// per-lane integrator vectors and a 16-column slice of the tail inverse
+template <typename MT>
+struct BdfWG {
+  double zn[QMAX + 1];
+  double ewt, acor, y, ftemp, wsc;
+  MT A4[16];                 // inverse row = lane, columns 16*wave .. +15 (pivot order)
+  int invp;                  // wavefront 0: step at which tail row `lane` was the pivot
+};
+
+template <typename MT>
+struct Team {
+  int tid, lane, wave;
+  bool own_;                 // tid < NEQ
+  double q;                  // charge of row tid
+  double Kp[NET::FR];        // rate * (n_H) of this lane's flux slots
+  uint32_t fab[NET::FR];
+  int rbase, rnl;            // RHS partial run of row tid
+  int red_buf;
+  WgLDS<MT> LDSA *S;
+  MT *Jg;                    // device-memory Jacobian slot [NJ]
+  double rtol, atol;
+
+  __device__ __forceinline__ bool lead() const { return tid == 0; }
+  __device__ __forceinline__ bool own() const { return own_; }
+  __device__ __forceinline__ void sync() { __syncthreads(); }
+  __device__ __forceinline__ double sum(double v) {
+    v = wave_sum(v);
+    if (lane == 0) S->red[red_buf][wave] = v;
+    __syncthreads();
+    const double r = (S->red[red_buf][0] + S->red[red_buf][1]) + (S->red[red_buf][2] + S->red[red_buf][3]);
+    red_buf ^= 1;
+    return r;
+  }
+  __device__ __forceinline__ double vmax(double v) {
+    v = wave_max(v);
+    if (lane == 0) S->red[red_buf][wave] = v;
+    __syncthreads();
+    const double r = fmax(fmax(S->red[red_buf][0], S->red[red_buf][1]), fmax(S->red[red_buf][2], S->red[red_buf][3]));
+    red_buf ^= 1;
+    return r;
+  }
+  __device__ __forceinline__ double rhs(double y, double &xe);
+  __device__ __forceinline__ int setup(BdfU LDSA &u, BdfWG<MT> &b, int convfail);
+  __device__ __forceinline__ double solve(const BdfU LDSA &u, const BdfWG<MT> &b, double r);
+};
+
+// Table base address made opaque at the point of use.  Without it the
+// compiler hoists the per-round addresses of the lane tables (one 64-bit
+// register pair per unrolled round) out of the integrator's loops, where they
+// stay live across the whole solve and push the kernel into spills.
+template <typename T>
+__device__ __forceinline__ const T *opaque(const T *p) {
+  asm volatile("" : "+s"(p));
+  return p;
+}
+
+// the region as LU storage (positions: [0, NHV) head entries, NHV the zero
+// slot, TBASE + r*NT + c the tail block) and as the flux arrays
+template <typename MT>
+__device__ __forceinline__ MT LDSA *regionLU(WgLDS<MT> LDSA *S) {
+  return (MT LDSA *)S->region;
+}
+template <typename MT>
+__device__ __forceinline__ double LDSA *regionF(WgLDS<MT> LDSA *S) {
+  return (double LDSA *)(S->region + RegionBytes<MT>::hvp);
+}
+template <typename MT>
+__device__ __forceinline__ double LDSA *regionD(WgLDS<MT> LDSA *S) {
+  return (double LDSA *)S->region;
+}
+
+// ---------------------------------------------------------------------
+// RHS: fluxes, chunk sums, row sums.  Returns ydot of row tid (0 beyond NEQ)
+// ---------------------------------------------------------------------
+template <typename MT>
+__device__ __forceinline__ double wg_rhs(Team<MT> &c, double y, double &xe_out) {
+  WgLDS<MT> LDSA &S = *c.S;
+  double LDSA *F = regionF(c.S);
+  if (c.own_) S.X[c.tid] = y;
+  const double xe = c.sum(c.own_ ? c.q * y : 0.0);
+  xe_out = xe;
+  if (c.tid == 0) {
+    S.X[NET::NEQ] = xe;
+    F[NET::NSLOT] = 0.0;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < NET::FR; r++) {
+    const uint32_t f = c.fab[r];
+    F[r * BLOCK + c.tid] = c.Kp[r] * S.X[f & 0xff] * S.X[(f >> 8) & 0xff];
+  }
+  __syncthreads();
+  const uint16_t *rt = opaque(NET::rhs_term);
+#pragma unroll
+  for (int p = 0; p < NET::VL; p++) {
+    double acc = 0.0;
+#pragma unroll 6
+    for (int t = 0; t < NET::TM; t++) {
+      const uint32_t w = rt[(p * NET::TM + t) * BLOCK + c.tid];
+      const int coef = (int)(w << 16) >> 28;  // signed 4-bit field at bit 12
+      acc = fma((double)coef, F[w & 0xfff], acc);
+    }
+    S.P[p * BLOCK + c.tid] = acc;
+  }
+  __syncthreads();
+  double yd = 0.0;
+  for (int u = 0; u < c.rnl; u++) yd += S.P[c.rbase + u];
+  return yd;
+}
+
+// ---------------------------------------------------------------------
+// Jacobian of the bordered system at the state in X -> device slot Jg
+// ---------------------------------------------------------------------
+template <typename MT>
+__device__ __forceinline__ void wg_jacobian(Team<MT> &c) {
+  WgLDS<MT> LDSA &S = *c.S;
+  double LDSA *D = regionD(c.S);
+#pragma unroll
+  for (int r = 0; r < NET::FR; r++) {
+    const uint32_t f = c.fab[r];
+    const int s = r * BLOCK + c.tid;
+    D[2 * s] = c.Kp[r] * S.X[(f >> 8) & 0xff];      // d flux / d X[a]
+    D[2 * s + 1] = c.Kp[r] * S.X[f & 0xff];         // d flux / d X[b]
+  }
+  if (c.tid == 0) D[2 * NET::NSLOT] = 0.0;
+  __syncthreads();
+  const uint32_t *jt = opaque(NET::jac_term);
+  double acc = 0.0;
+#pragma unroll 4
+  for (int t = 0; t < NET::JTM; t++) {
+    const uint32_t w = jt[t * BLOCK + c.tid];
+    const int coef = (int)(w << 15) >> 28;          // signed 4-bit field at bit 13
+    acc = fma((double)coef, D[w & 0x1fff], acc);
+    if ((w >> 17) & 1) {
+      c.Jg[w >> 18] = (MT)acc;
+      acc = 0.0;
+    }
+  }
+  __syncthreads();
+}
+
+// ---------------------------------------------------------------------
+// Schur complement Tl <- Tl - L_TH U_HT on the matrix cores
+// ---------------------------------------------------------------------
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ void schur_mfma(const Team<float> &c, float LDSA *LU) {
+  // wave w: output tile rows 32*(w>>1).., columns 32*(w&1)..
+  // v_mfma_f32_32x32x2f32: lane l supplies A[i = l%32][k = l/32] and
+  // B[k = l/32][j = l%32]; result register r of lane l is
+  // D[8*(r/4) + 4*(l/32) + r%4][l%32]
+  const int ti = c.wave >> 1, tj = c.wave & 1;
+  const int li = c.lane & 31, lk = c.lane >> 5;
+  float LDSA *Tl = LU + NET::TBASE;
+  f32x16 acc;
+#pragma unroll
+  for (int r = 0; r < 16; r++) {
+    const int i = 32 * ti + 8 * (r >> 2) + 4 * lk + (r & 3);
+    acc[r] = Tl[i * NET::TS + 32 * tj + li];
+  }
+  const uint16_t *lrow = opaque(NET::lth_pos) + (32 * ti + li) * NET::KH;
+  const uint16_t *ucol = opaque(NET::uht_pos) + 32 * tj + li;
+#pragma unroll 4
+  for (int k0 = 0; k0 < NET::KH; k0 += 2) {
+    const int k = k0 + lk;
+    const float a = -LU[lrow[k]];
+    const float bv = LU[ucol[k * 64]];
+    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a, bv, acc, 0, 0, 0);
+  }
+#pragma unroll
+  for (int r = 0; r < 16; r++) {
+    const int i = 32 * ti + 8 * (r >> 2) + 4 * lk + (r & 3);
+    Tl[i * NET::TS + 32 * tj + li] = acc[r];
+  }
+}
+
+__device__ __forceinline__ void schur_mfma(const Team<double> &c, double LDSA *LU) {
+  // v_mfma_f64_16x16x4f64: lane l supplies A[i = l%16][k = l/16] and
+  // B[k = l/16][j = l%16]; result register r of lane l is D[4*(l/16) + r][l%16].
+  // wave w computes the four 16x16 tiles of row band w.
+  const int li = c.lane & 15, lk = c.lane >> 4;
+  double LDSA *Tl = LU + NET::TBASE;
+#pragma unroll
+  for (int tj = 0; tj < 4; tj++) {
+    const int i0 = 16 * c.wave, j0 = 16 * tj;
+    f64x4 acc;
+#pragma unroll
+    for (int r = 0; r < 4; r++) acc[r] = Tl[(i0 + 4 * lk + r) * NET::TS + j0 + li];
+    const uint16_t *lrow = opaque(NET::lth_pos) + (i0 + li) * NET::KH;
+    const uint16_t *ucol = opaque(NET::uht_pos) + j0 + li;
+#pragma unroll 2
+    for (int k0 = 0; k0 < NET::KH; k0 += 4) {
+      const int k = k0 + lk;
+      const double a = -LU[lrow[k]];
+      const double bv = LU[ucol[k * 64]];
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bv, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++) Tl[(i0 + 4 * lk + r) * NET::TS + j0 + li] = acc[r];
+  }
+}
+
+// ---------------------------------------------------------------------
+// Gauss-Jordan inverse of the tail block in wavefront 0: lane r holds row r
+// (NT <= 64 columns in registers).  Step k: the pivot is the unused row with
+// the largest |a[r][k]| (ties to the lower row), found by a DPP arg-max; the
+// pivot row reaches every lane by v_readlane.  Rows are not moved: A ends as
+// the inverse in pivot order, invp = step at which this row was the pivot.
+// The step loop is not unrolled (it would not fit the instruction cache):
+// the register array is rotated by one column per step so that the pivot
+// column is always A[0], and rotated back into place at the end.
+// ---------------------------------------------------------------------
+template <typename MT>
+__device__ __forceinline__ bool tail_gj(int lane, MT (&A)[64], int &invp) {
+  bool used = lane >= NET::NT;
+  invp = lane;
+#pragma unroll 1
+  for (int k = 0; k < NET::NT; k++) {
+    const MT colk = A[0];
+    MT v = used ? (MT)-1.0 : fabs(colk);
+    int idx = lane;
+    argmax_step<MT, DPP_XOR1>(v, idx);
+    argmax_step<MT, DPP_XOR2>(v, idx);
+    argmax_step<MT, DPP_HALF_MIRROR>(v, idx);
+    argmax_step<MT, DPP_MIRROR>(v, idx);
+    MT vm = rdlane(v, 0);
+    int p = rdlane(idx, 0);
+#pragma unroll
+    for (int row = 1; row < 4; row++) {
+      const MT v1 = rdlane(v, 16 * row);
+      const int i1 = rdlane(idx, 16 * row);
+      const bool t1 = (v1 > vm) || (v1 == vm && i1 < p);
+      vm = t1 ? v1 : vm;
+      p = t1 ? i1 : p;
+    }
+    if (!(vm > (MT)0.0)) return false;
+    const MT d = (MT)1.0 / rdlane(colk, p);
+    const bool isp = (lane == p);
+    if (isp) { used = true; invp = k; }
+    const MT kv = isp ? d : -colk * d;
+#pragma unroll
+    for (int j = 1; j < 64; j++) {
+      const MT pj = rdlane(A[j], p) * d;
+      A[j - 1] = isp ? pj : fma(-colk, pj, A[j]);
+    }
+    A[63] = kv;
+  }
+#pragma unroll 1
+  for (int k = NET::NT; k < 64; k++) {
+    const MT t = A[0];
+#pragma unroll
+    for (int j = 1; j < 64; j++) A[j - 1] = A[j];
+    A[63] = t;
+  }
+  return true;
+}
+
+// ---------------------------------------------------------------------
+// setup: Jacobian refresh decision, M build, head elimination, Schur
+// complement, tail inverse.  Returns 0, or 1 if a pivot vanished.
+// ---------------------------------------------------------------------
+template <typename MT>
+__device__ __forceinline__ int wg_setup(Team<MT> &c, BdfU LDSA &u, BdfWG<MT> &b, int convfail) {
+  WgLDS<MT> LDSA &S = *c.S;
+  MT LDSA *LU = regionLU(c.S);
+  const double gamma = u.gamma;
+  const double dgamma = fabs(gamma / u.gammap - 1.0);
+  const int nst = u.nst;
+  const bool jbad = (nst == 0) || (nst > u.nstlj + MSBJ) ||
+                    ((convfail == CV_FAIL_BAD_J) && (dgamma < DGMAX_J)) || (convfail == CV_FAIL_OTHER);
+  __syncthreads();
+  if (c.tid == 0) {
+    if (jbad) {
+      u.nje++;
+      u.nstlj = nst;
+      u.jcur = 1;
+    } else {
+      u.jcur = 0;
+    }
+  }
+  // factor vector at the predicted state; x_e for the electron weight
+  if (c.own_) S.X[c.tid] = b.zn[0];
+  const double xe = c.sum(c.own_ ? c.q * b.zn[0] : 0.0);
+  if (c.tid == 0) S.X[NET::NEQ] = xe;
+  __syncthreads();
+  if (jbad) wg_jacobian(c);
+  // weights: W M W^-1 (w = error weights; x_e row/column by its own weight)
+  b.wsc = c.own_ ? b.ewt : 1.0;
+  if (c.own_) S.W[c.tid] = b.wsc;
+  if (c.tid == 0) S.W[NET::NEQ] = 1.0 / (c.rtol * fabs(xe) + c.atol);
+  __syncthreads();
+  const uint32_t *mp = opaque(NET::mpos);
+#pragma unroll 2
+  for (int t = 0; t < NET::MTM; t++) {
+    const int p = t * BLOCK + c.tid;
+    const uint32_t d = mp[p];
+    const int row = d & 0xff, col = (d >> 8) & 0xff, src = d >> 16;
+    if (row == 0xff) continue;
+    double m = 0.0;
+    if (row == 0xfe) {
+      m = 0.0;  // the zero slot
+    } else if (row == NET::NEQ) {
+      m = (col == NET::NEQ) ? 1.0 : -(double)NET::charge[col];
+      m *= S.W[row] / S.W[col];
+    } else {
+      if (src != 0xffff) m = -gamma * (double)c.Jg[src];
+      if (row == col) m += 1.0;
+      m *= S.W[row] / S.W[col];
+    }
+    LU[p] = (MT)m;
+  }
+  __syncthreads();
+  // head elimination, level by level
+  for (int lv = 0; lv < NET::NLEV; lv++) {
+    const int offA = NET::elimA_off[lv], nA = NET::elimA_n[lv];
+#pragma unroll 1
+    for (int t = 0; t < nA; t++) {
+      const uint32_t w = opaque(NET::elimA)[(offA + t) * BLOCK + c.tid];
+      const int kind = w >> 28, tgt = w & 0x3fff, dp = (w >> 14) & 0x3fff;
+      if (kind == 1) {
+        LU[tgt] = LU[tgt] / LU[dp];
+      } else if (kind == 2) {
+        S.dinv[tgt] = (MT)1.0 / LU[dp];
+      }
+    }
+    __syncthreads();
+    const int offB = NET::elimB_off[lv], nB = NET::elimB_n[lv];
+    MT acc = (MT)0.0;
+#pragma unroll 2
+    for (int t = 0; t < nB; t++) {
+      const uint64_t w = opaque(NET::elimB)[(size_t)(offB + t) * BLOCK + c.tid];
+      const int tgt = w & 0x3fff, lp = (w >> 14) & 0x3fff, up = (w >> 28) & 0x3fff;
+      acc = fma(LU[lp], LU[up], acc);
+      if ((w >> 42) & 1) {
+        LU[tgt] -= acc;
+        acc = (MT)0.0;
+      }
+    }
+    __syncthreads();
+  }
+  if constexpr (NET::NHEAD > 0) {
+    schur_mfma(c, LU);
+    __syncthreads();
+  }
+  // tail inverse: Gauss-Jordan in wavefront 0 on a register copy, written
+  // back over the block (stride 65), then each wavefront keeps 16 columns
+  MT LDSA *Tl = LU + NET::TBASE;
+  if (c.wave == 0) {
+    MT A[64];
+#pragma unroll
+    for (int j = 0; j < 64; j++) {
+      MT v = (c.lane == j) ? (MT)1.0 : (MT)0.0;
+      if (c.lane < NET::NT && j < NET::NT) v = Tl[c.lane * NET::TS + j];
+      A[j] = v;
+    }
+    const bool ok = tail_gj(c.lane, A, b.invp);
+#pragma unroll
+    for (int j = 0; j < 64; j++) Tl[c.lane * 65 + j] = A[j];
+    if (c.lane == 0) S.flag = ok ? 0 : 1;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < 16; j++) b.A4[j] = Tl[c.lane * 65 + 16 * c.wave + j];
+  const int fl = S.flag;
+  __syncthreads();
+  return fl;
+}
+
+// ---------------------------------------------------------------------
+// solve M dx = r: returns dx of row tid (0 beyond NEQ), with CVODE's
+// 2/(1+gamrat) correction for a changed gamma
+// ---------------------------------------------------------------------
+template <typename MT>
+__device__ __forceinline__ double wg_solve(Team<MT> &c, const BdfU LDSA &u, const BdfWG<MT> &b, double r) {
+  WgLDS<MT> LDSA &S = *c.S;
+  const MT LDSA *LU = regionLU(c.S);
+  const double gamrat = u.gamrat;
+  if (c.tid < NET::NB) S.B[c.tid] = c.own_ ? r * b.wsc : 0.0;
+  if (c.tid == 0) S.B[NET::NB] = 0.0;
+  __syncthreads();
+  // forward substitution over the head
+  for (int lv = 0; lv < NET::NFWD; lv++) {
+    const int off = NET::fwd_off[lv], n = NET::fwd_n[lv];
+    double acc = 0.0;
+#pragma unroll 2
+    for (int t = 0; t < n; t++) {
+      const uint32_t w = opaque(NET::fwd_tab)[(off + t) * BLOCK + c.tid];
+      acc = fma((double)LU[(w >> 8) & 0x3fff], S.B[(w >> 22) & 0xff], acc);
+      if (w >> 30) {
+        S.B[w & 0xff] -= acc;
+        acc = 0.0;
+      }
+    }
+    __syncthreads();
+  }
+  // tail rows gather the head values (wave w: pivots w*KH/4 ..)
+  if constexpr (NET::NHEAD > 0) {
+    constexpr int KQ = NET::KH / 4;
+    const uint16_t *lrow = opaque(NET::lth_pos) + c.lane * NET::KH + c.wave * KQ;
+    const uint8_t *hrow = opaque(NET::head_row) + c.wave * KQ;
+    double acc = 0.0;
+#pragma unroll 4
+    for (int k = 0; k < KQ; k++) acc = fma((double)LU[lrow[k]], S.B[hrow[k]], acc);
+    S.P[c.wave * 64 + c.lane] = acc;
+    __syncthreads();
+  }
+  // tail: x_T = S^-1 b_T; the permuted right-hand side from wavefront 0,
+  // 16-column partial products from every wavefront, combined in wavefront 0
+  if (c.wave == 0) {
+    double bt = 0.0;
+    if (c.lane < NET::NT) {
+      bt = S.B[NET::tail_row[c.lane]];
+      if constexpr (NET::NHEAD > 0) bt -= (S.P[c.lane] + S.P[64 + c.lane]) + (S.P[128 + c.lane] + S.P[192 + c.lane]);
+    }
+    S.V[b.invp] = bt;
+  }
+  __syncthreads();
+  {
+    double acc = 0.0;
+#pragma unroll
+    for (int j = 0; j < 16; j++) acc = fma((double)b.A4[j], S.V[16 * c.wave + j], acc);
+    S.P[c.wave * 64 + c.lane] = acc;
+  }
+  __syncthreads();
+  if (c.wave == 0) {
+    const double acc = (S.P[c.lane] + S.P[64 + c.lane]) + (S.P[128 + c.lane] + S.P[192 + c.lane]);
+    S.V2[b.invp] = acc;
+    wsync();
+    if (c.lane < NET::NT) S.B[NET::tail_row[c.lane]] = S.V2[c.lane];
+  }
+  __syncthreads();
+  // head rows scatter the tail values, then back substitution over the head
+  if constexpr (NET::NHEAD > 0) {
+    if (c.tid < NET::NHEAD) {
+      const uint16_t *ucol = opaque(NET::uht_pos) + c.tid * 64;
+      const uint8_t *trow = opaque(NET::tail_row);
+      double acc = 0.0;
+#pragma unroll 8
+      for (int j = 0; j < NET::NT; j++) acc = fma((double)LU[ucol[j]], S.B[trow[j]], acc);
+      S.B[NET::head_row[c.tid]] -= acc;
+    }
+    __syncthreads();
+  }
+  for (int lv = 0; lv < NET::NBWD; lv++) {
+    const int off = NET::bwd_off[lv], n = NET::bwd_n[lv];
+    double acc = 0.0;
+#pragma unroll 2
+    for (int t = 0; t < n; t++) {
+      const uint32_t w = opaque(NET::bwd_tab)[(off + t) * BLOCK + c.tid];
+      acc = fma((double)LU[(w >> 8) & 0x3fff], S.B[(w >> 22) & 0xff], acc);
+      if (w >> 30) {
+        const int k = w & 0xff;
+        S.B[k] = (S.B[k] - acc) * (double)S.dinv[k];
+        acc = 0.0;
+      }
+    }
+    __syncthreads();
+  }
+  double x = c.own_ ? S.B[c.tid] / b.wsc : 0.0;
+  if (gamrat != 1.0) x *= 2.0 / (1.0 + gamrat);
+  return x;
+}
+
+template <typename MT>
+__device__ __forceinline__ double Team<MT>::rhs(double y, double &xe) { return wg_rhs(*this, y, xe); }
+template <typename MT>
+__device__ __forceinline__ int Team<MT>::setup(BdfU LDSA &u, BdfWG<MT> &b, int convfail) {
+  return wg_setup(*this, u, b, convfail);
+}
+template <typename MT>
+__device__ __forceinline__ double Team<MT>::solve(const BdfU LDSA &u, const BdfWG<MT> &b, double r) {
+  return wg_solve(*this, u, b, r);
+}
+
+// ---------------------------------------------------------------------
+// one grid point: calculate_abundances.c:132-326 semantics
+// ---------------------------------------------------------------------
+template <typename MT>
+__device__ __forceinline__ void wg_solve_point(Team<MT> &c, const PointArgs &a, int sys) {
+  BdfWG<MT> b;
+  WgLDS<MT> LDSA &S = *c.S;
+  BdfU LDSA &u = S.u;
+  const double *rate = a.rate + (size_t)sys * a.rate_stride;
+  const double nH = a.density[sys];
+#pragma unroll
+  for (int r = 0; r < NET::FR; r++) {
+    const int re = NET::slot_reac[r * BLOCK + c.tid];
+    const double k = (re >= 0) ? rate[re] : 0.0;
+    c.Kp[r] = ((c.fab[r] >> 16) & 1) ? k * nH : k;
+  }
+  double *ab = a.abundance + (size_t)sys * a.ab_stride;
+  const double y0 = c.own_ ? ab[c.tid] : 0.0;
+  const double xe0 = ab[a.nelect0];
+  __syncthreads();
+  if (c.tid <= NET::NB) S.X[c.tid] = (c.tid == NET::NEQ + 1) ? 1.0 : 0.0;
+  if (c.tid == 0) {
+    S.P[NET::NV] = 0.0;
+    bdf_init_uniform(u, a.t0, xe0);
+  }
+  __syncthreads();
+  b.zn[0] = y0;
+#pragma unroll
+  for (int j = 1; j <= QMAX; j++) b.zn[j] = 0.0;
+  b.acor = 0.0; b.y = y0; b.ftemp = 0.0; b.wsc = 1.0; b.invp = c.lane;
+#pragma unroll
+  for (int j = 0; j < 16; j++) b.A4[j] = (MT)0;
+  b.ewt = 1.0 / (c.rtol * fabs(y0) + c.atol);
+
+  double yout = y0;
+  int flag = 0;
+  const int status = integrate(c, u, b, a, yout, flag);
+
+  // write-back with clip (calculate_abundances.c:308-317)
+  if (c.own_) {
+    double v = yout;
+    if (v < c.atol) v = 0.0;
+    ab[c.tid] = v;
+  }
+  __syncthreads();
+  if (c.tid == 0) {
+    ab[a.nelect0] = u.xe_last;
+    a.status[sys] = status;
+    if (a.stats) {
+      int *st = a.stats + (size_t)sys * 8;
+      st[0] = u.nst; st[1] = u.nfe; st[2] = u.nje; st[3] = u.nsetups;
+      st[4] = u.nni; st[5] = u.ncfn; st[6] = u.netf; st[7] = flag;
+    }
+  }
+  __syncthreads();
+}
+
+// persistent kernel: every workgroup pulls grid points from a queue; jac is
+// the Jacobian workspace, NJ entries per workgroup
+// FP32 matrices: two workgroups per CU fit the LDS, so the register file is
+// sized for two wavefronts per SIMD
+template <typename MT>
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(MT) == 4 ? 2 : 1)))
+bdf_wg_kernel(PointArgs a, MT *jac) {
+  __shared__ WgLDS<MT> lds;
+  __shared__ unsigned int next;
+  Team<MT> c;
+  c.tid = threadIdx.x;
+  c.lane = c.tid & 63;
+  c.wave = c.tid >> 6;
+  c.own_ = c.tid < NET::NEQ;
+  c.q = (double)NET::charge[c.tid];
+  c.red_buf = 0;
+  c.S = (WgLDS<MT> LDSA *)&lds;
+  c.Jg = jac + (size_t)blockIdx.x * NET::NJ;
+  c.rtol = a.rtol;
+  c.atol = a.atol;
+#pragma unroll
+  for (int r = 0; r < NET::FR; r++) c.fab[r] = NET::slot_fab[r * BLOCK + c.tid];
+  c.rbase = (c.tid < NET::NEQ) ? NET::row_base[c.tid] : 0;
+  c.rnl = (c.tid < NET::NEQ) ? NET::row_nl[c.tid] : 0;
+  for (;;) {
+    if (c.tid == 0) next = atomicAdd(a.counter, 1u);
+    __syncthreads();
+    const unsigned int sys = next;
+    __syncthreads();
+    if (sys >= (unsigned int)a.nsys) break;
+    wg_solve_point(c, a, (int)sys);
+  }
+}

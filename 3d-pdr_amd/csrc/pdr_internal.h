@@ -28,8 +28,12 @@ struct KernelInfo {
   int num_regs, shared_bytes, local_bytes, max_threads, blocks_per_cu;
 };
 
+// reduced network: one system per wavefront (bdf_wave.hip)
 int launch_wave_solver(int net, const SolveRequest &r, hipStream_t s);
 int wave_solver_info(int net, int mixed, KernelInfo *info);
+// full network and MYNETWORK: one system per workgroup (bdf_wg.hip)
+int launch_wg_solver(int net, const SolveRequest &r, hipStream_t s);
+int wg_solver_info(int net, int mixed, KernelInfo *info);
 
 // rate evaluation (calc_reac_rates.F90)
 struct PhysParams {

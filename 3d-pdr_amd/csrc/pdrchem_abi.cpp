@@ -11,6 +11,8 @@
 
 #include "../../include/pdrchem.h"
 #include "gen/net_reduced.h"
+#include "gen/wg_full.h"
+#include "gen/wg_mynetwork.h"
 #include "pdr_internal.h"
 
 using namespace pdrchem;
@@ -57,10 +59,18 @@ struct NetShape {
 };
 const NetShape kNets[NET_COUNT] = {
     {pdrnet_reduced::NSPEC, pdrnet_reduced::NREAC},
-    {215, 2926},
-    {58, 634},
+    {pdrwg_full::NSPEC, pdrwg_full::NREAC},
+    {pdrwg_mynetwork::NSPEC, pdrwg_mynetwork::NREAC},
 };
-const bool kCompiled[NET_COUNT] = {true, false, false};
+const bool kCompiled[NET_COUNT] = {true, true, true};
+
+int launch_solver(int net, const SolveRequest &r, hipStream_t s) {
+  return net == NET_REDUCED ? launch_wave_solver(net, r, s) : launch_wg_solver(net, r, s);
+}
+
+int solver_info(int net, int mixed, KernelInfo *info) {
+  return net == NET_REDUCED ? wave_solver_info(net, mixed, info) : wg_solver_info(net, mixed, info);
+}
 
 // device buffers reused across calculate_abundances_ calls
 struct HostPathBuffers {
@@ -132,7 +142,7 @@ int pdrchem_network_id(int nspec, int nreac) {
 int pdrchem_kernel_info(int net, int precision, int *num_regs, int *lds_bytes, int *scratch_bytes,
                         int *blocks_per_cu) {
   KernelInfo ki{};
-  if (wave_solver_info(net, precision == PDRCHEM_MIXED, &ki) != 0) {
+  if (net < 0 || net >= NET_COUNT || solver_info(net, precision == PDRCHEM_MIXED, &ki) != 0) {
     set_err("no solver kernel for this network");
     return -1;
   }
@@ -178,7 +188,7 @@ int pdrchem_solve_device(int net, double *d_abundance, const double *d_rate, con
   r.counter = d_counter;
   r.max_blocks = g_max_blocks;
   r.mixed = (precision == PDRCHEM_MIXED) ? 1 : 0;
-  int rc = launch_wave_solver(net, r, (hipStream_t)stream);
+  int rc = launch_solver(net, r, (hipStream_t)stream);
   if (rc != 0) {
     set_err("solver launch failed");
     return -1;

@@ -3,12 +3,24 @@
 
 #include "gen/net_reduced.h"
 #include "gen/shield_tables.h"
+#include "gen/wg_full.h"
+#include "gen/wg_mynetwork.h"
 #include "pdr_internal.h"
 
 namespace rates_reduced {
 namespace NET = pdrnet_reduced;
 #include "rates_impl.h"
 }  // namespace rates_reduced
+
+namespace rates_full {
+namespace NET = pdrwg_full;
+#include "rates_impl.h"
+}  // namespace rates_full
+
+namespace rates_mynetwork {
+namespace NET = pdrwg_mynetwork;
+#include "rates_impl.h"
+}  // namespace rates_mynetwork
 
 namespace pdrchem {
 
@@ -19,6 +31,14 @@ int launch_rates(int net, const RateRequest &r, hipStream_t s) {
     case NET_REDUCED:
       total = (long long)r.npts * pdrnet_reduced::NREAC;
       kern = rates_reduced::rates_kernel;
+      break;
+    case NET_FULL:
+      total = (long long)r.npts * pdrwg_full::NREAC;
+      kern = rates_full::rates_kernel;
+      break;
+    case NET_MYNETWORK:
+      total = (long long)r.npts * pdrwg_mynetwork::NREAC;
+      kern = rates_mynetwork::rates_kernel;
       break;
     default:
       return -2;

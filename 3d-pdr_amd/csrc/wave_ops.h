@@ -1,5 +1,7 @@
 // Wavefront-level helpers for gfx950 (wave64): uniform broadcasts and
 // cross-lane reductions used by the one-system-per-wavefront BDF kernel.
+// Reductions run on DPP within each 16-lane row and combine the four rows
+// with v_readlane, so they use no LDS and return wave-uniform results.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -17,38 +19,111 @@ __device__ __forceinline__ double uni(double v) {
   return __longlong_as_double(((uint64_t)hi << 32) | lo);
 }
 
-__device__ __forceinline__ double shfl_xor_d(double v, int m) { return __shfl_xor(v, m, 64); }
+// value of lane `l` (uniform l), as a uniform value
+__device__ __forceinline__ int rdlane(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+__device__ __forceinline__ float rdlane(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ double rdlane(double v, int l) {
+  uint64_t b = __double_as_longlong(v);
+  uint32_t lo = __builtin_amdgcn_readlane((uint32_t)b, l);
+  uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(b >> 32), l);
+  return __longlong_as_double(((uint64_t)hi << 32) | lo);
+}
 
-// sum over all 64 lanes, result uniform
+// DPP lane permutations within a 16-lane row
+constexpr int DPP_XOR1 = 0xB1;         // quad_perm [1,0,3,2]
+constexpr int DPP_XOR2 = 0x4E;         // quad_perm [2,3,0,1]
+constexpr int DPP_HALF_MIRROR = 0x141; // lane i <-> 7-i within each half-row
+constexpr int DPP_MIRROR = 0x140;      // lane i <-> 15-i within the row
+
+template <int CTRL>
+__device__ __forceinline__ int dpp(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp(float v) {
+  return __int_as_float(dpp<CTRL>(__float_as_int(v)));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp(double v) {
+  uint64_t b = __double_as_longlong(v);
+  uint32_t lo = (uint32_t)dpp<CTRL>((int)(uint32_t)b);
+  uint32_t hi = (uint32_t)dpp<CTRL>((int)(uint32_t)(b >> 32));
+  return __longlong_as_double(((uint64_t)hi << 32) | lo);
+}
+
+// the same lane of the other 32-lane half (v_permlane32_swap).  Swapping a
+// register with itself leaves each lane holding {own value, partner value}
+// in the two results (which element is which depends on the lane's half);
+// the partner is the element that is not the lane's own word, and when both
+// words are equal either one is the partner's value.
+__device__ __forceinline__ int other_half(int v) {
+  auto r = __builtin_amdgcn_permlane32_swap((unsigned)v, (unsigned)v, false, false);
+  return (int)((r[0] == (unsigned)v) ? r[1] : r[0]);
+}
+__device__ __forceinline__ float other_half(float v) { return __int_as_float(other_half(__float_as_int(v))); }
+__device__ __forceinline__ double other_half(double v) {
+  uint64_t b = __double_as_longlong(v);
+  uint32_t lo = (uint32_t)other_half((int)(uint32_t)b);
+  uint32_t hi = (uint32_t)other_half((int)(uint32_t)(b >> 32));
+  return __longlong_as_double(((uint64_t)hi << 32) | lo);
+}
+
+// sum over each 16-lane row (every lane of the row receives it)
+template <typename T>
+__device__ __forceinline__ T row_sum(T v) {
+  v += dpp<DPP_XOR1>(v);
+  v += dpp<DPP_XOR2>(v);
+  v += dpp<DPP_HALF_MIRROR>(v);
+  v += dpp<DPP_MIRROR>(v);
+  return v;
+}
+
+// sum over all 64 lanes, result uniform; fixed association order
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int m = 1; m < 64; m <<= 1) v += shfl_xor_d(v, m);
-  return uni(v);
+  v = row_sum(v);
+  return (rdlane(v, 0) + rdlane(v, 16)) + (rdlane(v, 32) + rdlane(v, 48));
 }
 
 // max over all lanes, result uniform
 __device__ __forceinline__ double wave_max(double v) {
-#pragma unroll
-  for (int m = 1; m < 64; m <<= 1) v = fmax(v, shfl_xor_d(v, m));
-  return uni(v);
+  v = fmax(v, dpp<DPP_XOR1>(v));
+  v = fmax(v, dpp<DPP_XOR2>(v));
+  v = fmax(v, dpp<DPP_HALF_MIRROR>(v));
+  v = fmax(v, dpp<DPP_MIRROR>(v));
+  return fmax(fmax(rdlane(v, 0), rdlane(v, 16)), fmax(rdlane(v, 32), rdlane(v, 48)));
 }
 
-// arg-max of v over lanes (ties -> smaller idx); returns uniform value, idx
-__device__ __forceinline__ void wave_argmax(double v, int idx, double &vmax, int &imax) {
-#pragma unroll
-  for (int m = 1; m < 64; m <<= 1) {
-    double ov = shfl_xor_d(v, m);
-    int oi = __shfl_xor(idx, m, 64);
-    bool take = (ov > v) || (ov == v && oi < idx);
-    v = take ? ov : v;
-    idx = take ? oi : idx;
-  }
-  vmax = uni(v);
-  imax = uni(idx);
+template <typename T, int CTRL>
+__device__ __forceinline__ void argmax_step(T &v, int &idx) {
+  T ov = dpp<CTRL>(v);
+  int oi = dpp<CTRL>(idx);
+  bool take = (ov > v) || (ov == v && oi < idx);
+  v = take ? ov : v;
+  idx = take ? oi : idx;
+}
+
+// arg-max of v over the 32 lanes of half `h` (uniform), ties -> smaller idx;
+// returns uniform value and index
+template <typename T>
+__device__ __forceinline__ void half_argmax(T v, int idx, int h, T &vmax, int &imax) {
+  argmax_step<T, DPP_XOR1>(v, idx);
+  argmax_step<T, DPP_XOR2>(v, idx);
+  argmax_step<T, DPP_HALF_MIRROR>(v, idx);
+  argmax_step<T, DPP_MIRROR>(v, idx);
+  const int l0 = 32 * h, l1 = 32 * h + 16;
+  T v0 = rdlane(v, l0), v1 = rdlane(v, l1);
+  int i0 = rdlane(idx, l0), i1 = rdlane(idx, l1);
+  bool t1 = (v1 > v0) || (v1 == v0 && i1 < i0);
+  vmax = t1 ? v1 : v0;
+  imax = t1 ? i1 : i0;
 }
 
 // order LDS traffic of this wavefront (no cross-wave synchronisation: every
-// wavefront of the kernel works on its own system)
+// wavefront of the kernel works on its own system).  LDS instructions of a
+// wavefront complete in order; this only stops the compiler from moving
+// LDS accesses across the point.
 __device__ __forceinline__ void wsync() {
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   __builtin_amdgcn_wave_barrier();

@@ -50,6 +50,49 @@ def _dbl(v: float) -> str:
     return repr(float(v))
 
 
+def emit_rate_params(L: List[str], net: Network) -> None:
+    """Per-reaction rate parameters (calc_reac_rates.F90) for the device rate
+    kernel; appended to the network header inside its namespace."""
+    rs = net.reactions
+    L.append("// rate evaluation parameters (calc_reac_rates.F90)")
+    for nm, get in (("alpha", lambda r: r.alpha), ("beta", lambda r: r.beta),
+                    ("gamma", lambda r: r.gamma), ("rtmin", lambda r: r.rtmin),
+                    ("rtmax", lambda r: r.rtmax)):
+        L.append(f"__device__ const double {nm}[NREAC] = {{")
+        L.append(_fmt_list([_dbl(get(r)) for r in rs], 6))
+        L.append("};")
+    L.append("__device__ const uint8_t rtype[NREAC] = {")
+    L.append(_fmt_list([str(r.rtype) for r in rs], 24))
+    L.append("};")
+    L.append("__device__ const uint8_t dup[NREAC] = {")
+    L.append(_fmt_list([str(min(r.duplicate, 255)) for r in rs], 24))
+    L.append("};")
+    L.append("__device__ const uint8_t gflag[NREAC] = {")
+    L.append(_fmt_list([str(r.gflag) for r in rs], 24))
+    L.append("};")
+    # Duplicate-window selection as a per-member test: the Fortran loop
+    # (calc_reac_rates.F90:113-144) picks the first member with
+    # T <= RTMAX, else the last member.  Member j is therefore selected iff
+    # T > max(RTMAX of earlier members) and (T <= RTMAX_j or j is last).
+    prev_max, last = [], []
+    for idx, r in enumerate(rs):
+        if r.duplicate == 0:
+            prev_max.append("-1.0e300")
+            last.append("1")
+            continue
+        lead = idx - (r.duplicate - 1)
+        pm = max((rs[m].rtmax for m in range(lead, idx)), default=None)
+        prev_max.append("-1.0e300" if pm is None else _dbl(pm))
+        nxt = rs[idx + 1].duplicate if idx + 1 < len(rs) else 0
+        last.append("1" if nxt < r.duplicate else "0")
+    L.append("__device__ const double dup_prev_max[NREAC] = {")
+    L.append(_fmt_list(prev_max, 8))
+    L.append("};")
+    L.append("__device__ const uint8_t dup_last[NREAC] = {")
+    L.append(_fmt_list(last, 24))
+    L.append("};")
+
+
 class WaveTables:
     """Lane tables of the wavefront-per-system layout (N = 32 rows)."""
 
@@ -203,45 +246,7 @@ class WaveTables:
         L.append("__device__ const int8_t charge[N] = {")
         L.append(_fmt_list([str(x) for x in self.charges]))
         L.append("};")
-        # rate parameters
-        rs = net.reactions
-        L.append("// rate evaluation parameters (calc_reac_rates.F90)")
-        for nm, get in (("alpha", lambda r: r.alpha), ("beta", lambda r: r.beta),
-                        ("gamma", lambda r: r.gamma), ("rtmin", lambda r: r.rtmin),
-                        ("rtmax", lambda r: r.rtmax)):
-            L.append(f"__device__ const double {nm}[NREAC] = {{")
-            L.append(_fmt_list([_dbl(get(r)) for r in rs], 6))
-            L.append("};")
-        L.append("__device__ const uint8_t rtype[NREAC] = {")
-        L.append(_fmt_list([str(r.rtype) for r in rs], 24))
-        L.append("};")
-        L.append("__device__ const uint8_t dup[NREAC] = {")
-        L.append(_fmt_list([str(min(r.duplicate, 255)) for r in rs], 24))
-        L.append("};")
-        L.append("__device__ const uint8_t gflag[NREAC] = {")
-        L.append(_fmt_list([str(r.gflag) for r in rs], 24))
-        L.append("};")
-        # Duplicate-window selection as a per-member test: the Fortran loop
-        # (calc_reac_rates.F90:113-144) picks the first member with
-        # T <= RTMAX, else the last member.  Member j is therefore selected iff
-        # T > max(RTMAX of earlier members) and (T <= RTMAX_j or j is last).
-        prev_max, last = [], []
-        for idx, r in enumerate(rs):
-            if r.duplicate == 0:
-                prev_max.append("-1.0e300")
-                last.append("1")
-                continue
-            lead = idx - (r.duplicate - 1)
-            pm = max((rs[m].rtmax for m in range(lead, idx)), default=None)
-            prev_max.append("-1.0e300" if pm is None else _dbl(pm))
-            nxt = rs[idx + 1].duplicate if idx + 1 < len(rs) else 0
-            last.append("1" if nxt < r.duplicate else "0")
-        L.append("__device__ const double dup_prev_max[NREAC] = {")
-        L.append(_fmt_list(prev_max, 8))
-        L.append("};")
-        L.append("__device__ const uint8_t dup_last[NREAC] = {")
-        L.append(_fmt_list(last, 24))
-        L.append("};")
+        emit_rate_params(L, net)
         L.append(f"}}  // namespace {ns}")
         with open(path, "w") as fh:
             fh.write("\n".join(L) + "\n")

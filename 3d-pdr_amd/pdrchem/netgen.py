@@ -401,6 +401,10 @@ def write_tab(net: Network, path: str) -> None:
             fh.write(f"R {r.index} {r.rtype} {r.duplicate} {r.gflag} {r.alpha:.17g} {r.beta:.17g} "
                      f"{r.gamma:.17g} {r.rtmin:.17g} {r.rtmax:.17g} {r.fa} {r.fb} {r.nh} "
                      f"{len(r.stoich)} {st}\n".replace("  \n", " \n"))
+        # elimination order of the bordered iteration matrix (pdrchem.sparse)
+        from .sparse import analyse
+        a = analyse(net)
+        fh.write(f"O {a.nb} {a.nhead} " + " ".join(str(v) for v in a.order) + "\n")
         fh.write("E\n")
 
 

@@ -1,0 +1,451 @@
+"""HIP table generator for the workgroup-per-system BDF kernel (networks with
+more than 32 equations: the full network, 214 equations, and MYNETWORK, 57).
+
+Emits ``csrc/gen/wg_<name>.h``.  One stiff system runs on a workgroup of
+BLOCK = 256 lanes (four wavefronts).  The tables replace, for these
+networks, the reference's MakeRates-generated odes_*.c / jacobian_*.c
+(odes_full.c:26-700) and CVODE's dense LU (calculate_abundances.c:207):
+
+* RHS: flux slot s = round*BLOCK + lane (lane-parallel fluxes); the
+  per-species sums ydot_i = sum_s S_is flux_s are cut into chunks, every
+  chunk owned by one "virtual lane" (VL virtual lanes per lane), so that a
+  heavy row (H2, H) is spread over many lanes; a chunk's partial goes to
+  P[v] and row i adds its contiguous run of partials.
+* Jacobian: exact, of the bordered system (pdrchem.sparse): entries of
+  J0 = df/dx at fixed x_e and of g = df/dx_e, each summed by one lane from
+  the flux partials D[2s] = dflux_s/dX[a_s], D[2s+1] = dflux_s/dX[b_s].
+* Factorisation (pdrchem.sparse): the iteration matrix lives in LDS as
+  Hv[NH] (every stored entry whose row or column is a head pivot) followed by
+  the dense tail block Tl[NT][NT]; head pivots are eliminated level by level
+  without pivoting (phase A: multipliers and pivot reciprocals, phase B:
+  updates grouped by target so that one lane owns every update of an entry
+  within a level); the tail block is inverted by Gauss-Jordan in wavefront 0.
+* Triangular solves of the head by level (forward over L, backward over U),
+  the tail by the explicit inverse.
+"""
+from __future__ import annotations
+
+import heapq
+import math
+import os
+import sys
+from typing import Dict, List, Tuple
+
+import numpy as np
+
+if __package__ in (None, ""):
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    from pdrchem.netgen import load_network, Network  # type: ignore
+    from pdrchem.gpugen import emit_rate_params, _fmt_list  # type: ignore
+    from pdrchem import sparse  # type: ignore
+else:
+    from .netgen import load_network, Network
+    from .gpugen import emit_rate_params, _fmt_list
+    from . import sparse
+
+BLOCK = 256
+POS_BITS = 14          # LU positions (NLU + 1 < 16384)
+COEF_BITS = 4          # signed stoichiometric coefficients (-8..7)
+
+
+def _lpt(loads: List[Tuple[int, object]], nbins: int) -> List[List[object]]:
+    """Longest-processing-time assignment of (weight, item) to nbins lanes;
+    deterministic (ties to the lower lane)."""
+    bins: List[List[object]] = [[] for _ in range(nbins)]
+    heap = [(0, b) for b in range(nbins)]
+    heapq.heapify(heap)
+    for w, item in sorted(loads, key=lambda x: (-x[0], str(x[1]))):
+        load, b = heapq.heappop(heap)
+        bins[b].append(item)
+        heapq.heappush(heap, (load + w, b))
+    return bins
+
+
+def _coef4(c: int) -> int:
+    if not -8 <= c <= 7:
+        raise ValueError(f"stoichiometric coefficient {c} does not fit 4 bits")
+    return c & 0xF
+
+
+class WgTables:
+    def __init__(self, net: Network, block: int = BLOCK):
+        self.net = net
+        self.block = block
+        n = net.neq
+        self.neq = n
+        an = sparse.analyse(net)
+        self.an = an
+        NB = an.nb
+        self.nb = NB
+        if NB > 255:
+            raise ValueError(f"{net.name}: more than 254 equations")
+        XE, ONE = n, n + 1
+        # ------------------------------------------------------------ fluxes
+        self.slot_reac = [r.index for r in net.reactions if r.stoich]
+        nact = len(self.slot_reac)
+        self.fr = max(1, math.ceil(nact / block))
+        self.nslot = self.fr * block
+        reac_slot = {r: s for s, r in enumerate(self.slot_reac)}
+        self.slot_fab = []
+        for s in range(self.nslot):
+            if s < nact:
+                r = net.reactions[self.slot_reac[s]]
+                self.slot_fab.append(r.fa | (r.fb << 8) | (r.nh << 16) | (1 << 17))
+            else:
+                self.slot_fab.append(ONE | (ONE << 8))
+        self.fzero = self.nslot                 # F[NSLOT] = 0
+        # -------------------------------------------------------- RHS chunks
+        rows: List[List[Tuple[int, int]]] = [[] for _ in range(n)]
+        for r in net.reactions:
+            if not r.stoich:
+                continue
+            s = reac_slot[r.index]
+            for k, v in sorted(r.stoich.items()):
+                rows[k].append((s, v))
+        best = None
+        for vl in (1, 2, 3, 4):
+            nv = vl * block
+            nl = [1 if rows[i] else 0 for i in range(n)]
+            if sum(nl) > nv:
+                continue
+            load = lambda i: math.ceil(len(rows[i]) / nl[i]) if nl[i] else 0
+            heap = [(-load(i), i) for i in range(n) if nl[i]]
+            heapq.heapify(heap)
+            used = sum(nl)
+            while used < nv and heap:
+                negl, i = heapq.heappop(heap)
+                if -negl <= 1:
+                    heapq.heappush(heap, (negl, i))
+                    break
+                nl[i] += 1
+                used += 1
+                heapq.heappush(heap, (-load(i), i))
+            tm = max(load(i) for i in range(n))
+            cost = vl * tm + vl * 4  # chunk passes plus per-pass overhead
+            if best is None or cost < best[0]:
+                best = (cost, vl, nl[:], tm)
+        _, self.vl, nl, self.tm = best
+        self.nv = self.vl * block
+        self.row_nl = nl
+        base, b = [], 0
+        for i in range(n):
+            base.append(b)
+            b += nl[i]
+        self.row_base = base
+        self.pm = max(nl)
+        self.pzero = self.nv                   # P[NV] = 0
+        zt = self.fzero | (0 << 12)
+        self.rhs_term = [[[zt] * block for _ in range(self.tm)] for _ in range(self.vl)]
+        for i in range(n):
+            terms = rows[i]
+            if not nl[i]:
+                continue
+            csz = math.ceil(len(terms) / nl[i])
+            for u in range(nl[i]):
+                v = base[i] + u
+                pas, lane = divmod(v, block)
+                for t, (s, c) in enumerate(terms[u * csz:(u + 1) * csz]):
+                    if s >= 4096:
+                        raise ValueError("flux slot does not fit 12 bits")
+                    self.rhs_term[pas][t][lane] = s | (_coef4(c) << 12)
+        # ------------------------------------------------------- Jacobian
+        jterms: Dict[Tuple[int, int], List[Tuple[int, int]]] = {}
+        for r in net.reactions:
+            if not r.stoich:
+                continue
+            s = reac_slot[r.index]
+            for i, coef in r.stoich.items():
+                for which, f in ((0, r.fa), (1, r.fb)):
+                    if f <= XE:
+                        jterms.setdefault((i, f), []).append((2 * s + which, coef))
+        self.jent = sorted(jterms)             # (row, col) with col == XE for g
+        self.nj = len(self.jent)
+        jid = {e: k for k, e in enumerate(self.jent)}
+        self.dzero = 2 * self.nslot            # D[2*NSLOT] = 0
+        bins = _lpt([(len(jterms[e]), e) for e in self.jent], block)
+        lists = []
+        for lane in range(block):
+            lst = []
+            for e in sorted(bins[lane]):
+                tl = jterms[e]
+                for t, (d, c) in enumerate(tl):
+                    last = 1 if t == len(tl) - 1 else 0
+                    lst.append(d | (_coef4(c) << 13) | (last << 17) | (jid[e] << 18))
+            lists.append(lst)
+        self.jtm = max(len(l) for l in lists)
+        if self.dzero >= (1 << 13) or self.nj >= (1 << 14):
+            raise ValueError("Jacobian tables overflow their bit fields")
+        pad = self.dzero
+        self.jac_term = [[(lists[lane][t] if t < len(lists[lane]) else pad) for lane in range(block)]
+                         for t in range(self.jtm)]
+        # --------------------------------------------------- LU positions
+        F = an.filled
+        order = an.order
+        head, tail = an.head, an.tail
+        self.nh_piv = an.nhead
+        self.nt = an.ntail
+        if self.nt > 64:
+            raise ValueError("tail wider than a wavefront")
+        pos_of = np.full((NB, NB), -1, dtype=np.int64)
+        is_head = np.zeros(NB, dtype=bool)
+        is_head[head] = True
+        hv = [(i, j) for i in range(NB) for j in range(NB) if F[i, j] and (is_head[i] or is_head[j])]
+        hv.sort(key=lambda e: (e[0], e[1]))
+        for p, (i, j) in enumerate(hv):
+            pos_of[i, j] = p
+        self.nhv = len(hv)
+        self.luzero = self.nhv                 # LU[NHV] = 0
+        # tail block Tl[NT][NT] after the zero slot, 16-byte aligned: it lies
+        # over the flux array F in LDS, which is FP64
+        self.tbase = 4 * math.ceil((self.nhv + 1) / 4)
+        # row stride NT + 1: a column of the block falls in distinct LDS banks
+        self.tstride = self.nt + 1
+        tidx = {v: r for r, v in enumerate(tail)}
+        for i in tail:
+            for j in tail:
+                pos_of[i, j] = self.tbase + tidx[i] * self.tstride + tidx[j]
+        self.nlu = self.tbase + self.nt * self.tstride
+        if self.nlu >= (1 << POS_BITS):
+            raise ValueError("LU storage exceeds the position field")
+        self.tail = tail
+        self.head = head
+        allpos = hv + [None] + ["pad"] * (self.tbase - self.nhv - 1)
+        for i in tail:
+            allpos += [(i, j) for j in tail] + ["pad"]
+        assert len(allpos) == self.nlu
+        # M entry descriptors in position order: row | col << 8 | src << 16
+        # (src 0xffff: no J term); row 0xfe: the zero slot; row 0xff: padding
+        self.mpos = []
+        for e in allpos:
+            if e is None:
+                self.mpos.append(0xFE)
+                continue
+            if e == "pad":
+                self.mpos.append(0xFF)
+                continue
+            i, j = e
+            src = jid.get((i, j), 0xFFFF) if i < n else 0xFFFF
+            self.mpos.append(i | (j << 8) | (src << 16))
+        self.mtm = math.ceil(self.nlu / block)
+        self.mpos += [0xFF] * (self.mtm * block - self.nlu)
+        # ------------------------------------------ head elimination levels
+        hl = an.head_level
+        self.nlev = int(hl.max()) + 1 if an.nhead else 0
+        later_mask = {}
+        rank = {v: a for a, v in enumerate(order)}
+        self.elimA: List[List[List[int]]] = []   # [level][round][lane]
+        self.elimB: List[List[List[int]]] = []
+        for lev in range(self.nlev):
+            piv = [head[a] for a in range(an.nhead) if hl[a] == lev]
+            itemsA = []
+            upd: Dict[int, List[Tuple[int, int]]] = {}
+            for k in piv:
+                dk = int(pos_of[k, k])
+                itemsA.append((1, ('d', k), (2 << 28) | k | (dk << 14)))
+                rows_l = [i for i in range(NB) if F[i, k] and rank[i] > rank[k]]
+                cols_u = [j for j in range(NB) if F[k, j] and rank[j] > rank[k]]
+                for i in rows_l:
+                    lp = int(pos_of[i, k])
+                    itemsA.append((1, ('l', i, k), (1 << 28) | lp | (dk << 14)))
+                    for j in cols_u:
+                        if not (is_head[i] or is_head[j]):
+                            continue  # tail block: Schur complement product
+                        tp = int(pos_of[i, j])
+                        assert 0 <= tp < self.nhv, (i, j)
+                        upd.setdefault(tp, []).append((lp, int(pos_of[k, j])))
+            binsA = _lpt([(1, it) for it in itemsA], block)
+            ra = max(len(b) for b in binsA)
+            tabA = [[0] * block for _ in range(ra)]
+            for lane in range(block):
+                for t, it in enumerate(sorted(binsA[lane], key=str)):
+                    tabA[t][lane] = it[2]
+            self.elimA.append(tabA)
+            binsB = _lpt([(len(v), tp) for tp, v in upd.items()], block)
+            lanes = []
+            for lane in range(block):
+                lst = []
+                for tp in sorted(binsB[lane]):
+                    us = upd[tp]
+                    for t, (lp, up) in enumerate(us):
+                        last = 1 if t == len(us) - 1 else 0
+                        lst.append(tp | (lp << 14) | (up << 28) | (last << 42))
+                lanes.append(lst)
+            rb = max((len(l) for l in lanes), default=0)
+            padB = self.luzero | (self.luzero << 14) | (self.luzero << 28)
+            self.elimB.append([[(lanes[lane][t] if t < len(lanes[lane]) else padB)
+                                for lane in range(block)] for t in range(rb)])
+        # ------------------------- Schur complement / tail coupling tables
+        # KH head pivots padded to a multiple of 4 (MFMA K steps); the
+        # multipliers L[t][k] of tail rows and U[k][j] of tail columns
+        if an.nhead and self.nt != 64:
+            raise ValueError("a head needs a 64-wide tail (one MFMA tile grid)")
+        self.kh = 4 * math.ceil(an.nhead / 4)
+        self.head_row = list(head) + [NB] * (self.kh - an.nhead)   # B[NB] = 0
+        self.lth = [self.luzero] * (64 * self.kh)                 # [t][k]
+        self.uht = [self.luzero] * (self.kh * 64)                 # [k][j]
+        for a, k in enumerate(head):
+            for t, i in enumerate(tail):
+                if F[i, k]:
+                    self.lth[t * self.kh + a] = int(pos_of[i, k])
+                if F[k, i]:
+                    self.uht[a * 64 + t] = int(pos_of[k, i])
+        # ------------------------------------- head solve levels (head-head)
+        fl, bl = an.fwd_level, an.bwd_level
+        hset = set(head)
+        self.fwd: List[List[List[int]]] = []
+        nfl = int(fl.max()) + 1 if an.nhead else 0
+        for lev in range(nfl):
+            per_row = {}
+            for a in range(an.nhead):
+                if fl[a] != lev:
+                    continue
+                i = head[a]
+                ks = [k for k in head if F[i, k] and rank[k] < rank[i]]
+                if ks:
+                    per_row[i] = [(int(pos_of[i, k]), k) for k in ks]
+            self.fwd.append(self._solve_tab(per_row, dummy=False))
+        self.bwd: List[List[List[int]]] = []
+        nbl = int(bl.max()) + 1 if an.nhead else 0
+        for lev in range(nbl):
+            per_row = {}
+            for a in range(an.nhead):
+                if bl[a] != lev:
+                    continue
+                k = head[a]
+                js = [j for j in head if F[k, j] and rank[j] > rank[k]]
+                per_row[k] = [(int(pos_of[k, j]), j) for j in js] or [(self.luzero, 0)]
+            self.bwd.append(self._solve_tab(per_row, dummy=True))
+        self.diag_pos = [int(pos_of[i, i]) for i in range(NB)]
+        self.charges = net.charges()
+
+    def _solve_tab(self, per_row, dummy):
+        """(row | pos << 8 | col << 22 | last << 30) lists, LPT over lanes."""
+        bins = _lpt([(len(v), i) for i, v in per_row.items()], self.block)
+        lanes = []
+        for lane in range(self.block):
+            lst = []
+            for i in sorted(bins[lane]):
+                ts = per_row[i]
+                for t, (p, k) in enumerate(ts):
+                    last = 1 if t == len(ts) - 1 else 0
+                    lst.append(i | (p << 8) | (k << 22) | (last << 30))
+            lanes.append(lst)
+        r = max((len(l) for l in lanes), default=0)
+        pad = self.luzero << 8
+        return [[(lanes[lane][t] if t < len(lanes[lane]) else pad) for lane in range(self.block)]
+                for t in range(r)]
+
+    # ------------------------------------------------------------------ emit
+    def emit(self, path: str) -> None:
+        net = self.net
+        ns = f"pdrwg_{net.name}"
+        L = [f"// Generated by pdrchem/wggen.py from networks/{net.name}.json; do not edit.",
+             "// Tables of the workgroup-per-system BDF kernel (csrc/bdf_wg_impl.h).",
+             "#pragma once", "#include <stdint.h>", f"namespace {ns} {{"]
+        c = lambda k, v, cm="": L.append(f"constexpr int {k} = {v};" + (f"  // {cm}" if cm else ""))
+        c("NSPEC", net.nspec); c("NREAC", net.nreac); c("NEQ", net.neq)
+        c("NB", self.nb, "bordered size: species + x_e")
+        c("BLOCK", self.block)
+        c("FR", self.fr, "flux rounds per lane"); c("NSLOT", self.nslot)
+        c("NACTIVE", len(self.slot_reac))
+        c("VL", self.vl, "RHS virtual lanes per lane"); c("NV", self.nv)
+        c("TM", self.tm, "RHS terms per chunk"); c("PM", self.pm, "chunks per row (max)")
+        c("NJ", self.nj, "Jacobian entries (J0 and g)"); c("JTM", self.jtm, "Jacobian terms per lane")
+        c("NHV", self.nhv, "head-related LU entries; LU[NHV] = 0")
+        c("TBASE", self.tbase, "LU position of the tail block Tl[NT][NT] (multiple of 4)")
+        c("NT", self.nt, "tail size")
+        c("TS", self.tstride, "row stride of the tail block")
+        c("NLU", self.nlu); c("MTM", self.mtm, "M-build rounds")
+        c("KH", self.kh, "head pivots padded to a multiple of 4")
+        c("NHEAD", self.nh_piv); c("NLEV", self.nlev, "head elimination levels")
+        c("NFWD", len(self.fwd), "forward solve levels of the head")
+        c("NBWD", len(self.bwd), "backward solve levels")
+        c("XLAMBDA_SPLINE", net.xlambda_spline)
+
+        def arr(ctype, name, vals, fmt=str, per=16):
+            L.append(f"__device__ const {ctype} {name}[{max(1, len(vals))}] = {{")
+            L.append(_fmt_list([fmt(v) for v in vals] if vals else ["0"], per))
+            L.append("};")
+
+        hx = lambda v: f"0x{v:x}u"
+        hx64 = lambda v: f"0x{v:x}ull"
+        arr("int16_t", "slot_reac", self.slot_reac + [-1] * (self.nslot - len(self.slot_reac)))
+        arr("uint32_t", "slot_fab", self.slot_fab, hx, 8)
+        flat = [self.rhs_term[p][t][l] for p in range(self.vl) for t in range(self.tm) for l in range(self.block)]
+        L.append("// [pass][t][lane]: flux slot | coef << 12 (4-bit signed)")
+        arr("uint16_t", "rhs_term", flat, hx, 12)
+        rb = [self.row_base[i] if i < self.neq else 0 for i in range(self.block)]
+        rn = [self.row_nl[i] if i < self.neq else 0 for i in range(self.block)]
+        arr("uint16_t", "row_base", rb)
+        arr("uint8_t", "row_nl", rn)
+        L.append("// [t][lane]: D index | coef << 13 | last << 17 | entry << 18")
+        arr("uint32_t", "jac_term", [v for row in self.jac_term for v in row], hx, 8)
+        L.append("// entry -> row | col << 8 (col NEQ: d/dx_e)")
+        arr("uint16_t", "jent", [i | (j << 8) for (i, j) in self.jent], hx, 12)
+        L.append("// [t][lane]: row | col << 8 | J entry << 16 (0xffff: none); row 255 = padding")
+        arr("uint32_t", "mpos", self.mpos, hx, 8)
+        # elimination
+        offA, offB, ra, rbl = [], [], [], []
+        flatA, flatB = [], []
+        for lev in range(self.nlev):
+            offA.append(len(flatA) // self.block); ra.append(len(self.elimA[lev]))
+            for row in self.elimA[lev]:
+                flatA += row
+            offB.append(len(flatB) // self.block); rbl.append(len(self.elimB[lev]))
+            for row in self.elimB[lev]:
+                flatB += row
+        L.append("// head elimination, phase A [round][lane]: kind << 28 | diag pos << 14 | target")
+        L.append("//   kind 1: multiplier at target pos; kind 2: reciprocal of pivot row `target`")
+        arr("int", "elimA_off", offA); arr("int", "elimA_n", ra)
+        arr("uint32_t", "elimA", flatA, hx, 8)
+        L.append("// phase B [round][lane]: target | l pos << 14 | u pos << 28 | last << 42")
+        arr("int", "elimB_off", offB); arr("int", "elimB_n", rbl)
+        arr("uint64_t", "elimB", flatB, hx64, 6)
+        for nm, levs in (("fwd", self.fwd), ("bwd", self.bwd)):
+            off, cnt, flat = [], [], []
+            for lev in levs:
+                off.append(len(flat) // self.block); cnt.append(len(lev))
+                for row in lev:
+                    flat += row
+            L.append(f"// {nm} solve [round][lane]: row | pos << 8 | col << 22 | last << 30")
+            arr("int", f"{nm}_off", off); arr("int", f"{nm}_n", cnt)
+            arr("uint32_t", f"{nm}_tab", flat, hx, 8)
+        arr("uint8_t", "tail_row", list(self.tail) + [0] * (64 - self.nt))
+        arr("uint8_t", "head_row", self.head_row)
+        L.append("// LU positions of L[tail t][head k] ([t][k]) and U[head k][tail j] ([k][j])")
+        arr("uint16_t", "lth_pos", self.lth, str, 16)
+        arr("uint16_t", "uht_pos", self.uht, str, 16)
+        arr("uint16_t", "diag_pos", self.diag_pos)
+        arr("int8_t", "charge", self.charges + [0] * (self.block - self.neq))
+        emit_rate_params(L, net)
+        L.append(f"}}  // namespace {ns}")
+        with open(path, "w") as fh:
+            fh.write("\n".join(L) + "\n")
+
+    def summary(self) -> str:
+        ra = sum(len(x) for x in self.elimA)
+        rb = sum(len(x) for x in self.elimB)
+        return (f"{self.net.name}: NB={self.nb} FR={self.fr} VL={self.vl} TM={self.tm} PM={self.pm} "
+                f"NJ={self.nj} JTM={self.jtm} NLU={self.nlu} (NHV={self.nhv}, NT={self.nt}, KH={self.kh}) "
+                f"MTM={self.mtm} "
+                f"levels={self.nlev} elimA rounds={ra} elimB rounds={rb} "
+                f"fwd rounds={sum(len(x) for x in self.fwd)}/{len(self.fwd)} "
+                f"bwd rounds={sum(len(x) for x in self.bwd)}/{len(self.bwd)}")
+
+
+def generate(names=("mynetwork", "full"), net_dir=None, out_dir=None) -> List[WgTables]:
+    base = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    net_dir = net_dir or os.path.join(base, "networks")
+    out_dir = out_dir or os.path.join(base, "csrc", "gen")
+    os.makedirs(out_dir, exist_ok=True)
+    out = []
+    for n in names:
+        t = WgTables(load_network(n, net_dir))
+        t.emit(os.path.join(out_dir, f"wg_{n}.h"))
+        out.append(t)
+    return out
+
+
+if __name__ == "__main__":
+    for t in generate():
+        print(t.summary())

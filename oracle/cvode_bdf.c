@@ -90,10 +90,12 @@ bdf_mem *bdf_create(int n, pdr_rhs_fn f, pdr_jac_fn jac, void *ud, double rtol, 
   m->n = n; m->f = f; m->jac = jac; m->ud = ud;
   m->rtol = rtol; m->atol = atol; m->mxstep = mxstep; m->mode = mode;
   m->qmax = QMAX;
-  size_t v = (size_t)n;
-  m->buf = calloc(v * (QMAX + 1 + 8) + 2 * v * v, sizeof(double));
-  m->piv = calloc(v, sizeof(int));
+  m->nb = (mode & PDR_MODE_HYBRID) ? n + 1 : n;
+  size_t v = (size_t)m->nb;
+  m->buf = calloc(v * (QMAX + 1 + 10) + 3 * v * v + 4 * v, sizeof(double));
+  m->piv = calloc(2 * v, sizeof(int));
   if (!m->buf || !m->piv) { bdf_free(m); return NULL; }
+  m->tpiv = m->piv + v;
   for (int j = 0; j <= QMAX; j++) m->zn[j] = m->buf + v * j;
   double *p = m->buf + v * (QMAX + 1);
   m->ewt = p; p += v;
@@ -104,8 +106,11 @@ bdf_mem *bdf_create(int n, pdr_rhs_fn f, pdr_jac_fn jac, void *ud, double rtol, 
   m->vtmp = p; p += v;
   m->wsc = p; p += v;
   m->rsd = p; p += v;
+  m->gjw = p; p += 2 * v;
   m->M = p; p += v * v;
-  m->savedJ = p;
+  m->savedJ = p; p += v * v;
+  m->tail = p; p += v * v;
+  m->bwork = p;
   return m;
 }
 
@@ -114,6 +119,12 @@ void bdf_free(bdf_mem *m) {
   free(m->buf);
   free(m->piv);
   free(m);
+}
+
+void bdf_set_border(bdf_mem *m, const double *q, int nhead, const int *order) {
+  m->bq = q;
+  m->nhead = nhead;
+  m->border_order = order;
 }
 
 int bdf_init(bdf_mem *m, double t0, const double *y0) {
@@ -170,6 +181,119 @@ static int lu_factor(int n, double *a, int *piv, int f32) {
   return 0;
 }
 
+/* In-place Gauss-Jordan inverse with partial pivoting, in the order the HIP
+ * kernel uses: at step k the pivot is the row, among those not yet used,
+ * with the largest |a[i][k]| (first such row on ties); rows are not swapped,
+ * piv[k] records the pivot row.  On return a holds the inverse in pivot
+ * order, applied by gj_apply.  f32 rounds every entry to FP32 as formed.
+ * Returns 1 if the matrix is singular. */
+static int gj_invert(int n, double *a, int *piv, int f32) {
+  int used[512];
+  if (n > 512) return 1;
+  for (int i = 0; i < n; i++) {
+    used[i] = 0;
+    if (f32)
+      for (int j = 0; j < n; j++) a[(size_t)i * n + j] = (float)a[(size_t)i * n + j];
+  }
+  for (int k = 0; k < n; k++) {
+    int p = -1;
+    double vm = -1.0;
+    for (int i = 0; i < n; i++)
+      if (!used[i]) {
+        double v = fabs(a[(size_t)i * n + k]);
+        if (v > vm) { vm = v; p = i; }
+      }
+    if (!(vm > 0.0)) return 1;
+    used[p] = 1;
+    piv[k] = p;
+    double d = 1.0 / a[(size_t)p * n + k];
+    if (f32) d = (float)d;
+    double *rp = a + (size_t)p * n;
+    for (int j = 0; j < n; j++) {
+      double v = (j == k) ? d : rp[j] * d;
+      rp[j] = f32 ? (float)v : v;
+    }
+    for (int i = 0; i < n; i++) {
+      if (i == p) continue;
+      double *ri = a + (size_t)i * n;
+      double fi = ri[k];
+      for (int j = 0; j < n; j++) {
+        double v = (j == k) ? -fi * d : ri[j] - fi * rp[j];
+        ri[j] = f32 ? (float)v : v;
+      }
+    }
+  }
+  return 0;
+}
+
+/* LU without pivoting in a fixed elimination order (the order a sparse
+ * factorisation computes once per network): step k eliminates with the
+ * diagonal entry of row/column order[k].  piv[] receives order[].  Returns
+ * k+1 if the k-th pivot is zero. */
+static int g_static_n = 0;
+static int g_static_order[512];
+
+void pdr_set_static_order(int n, const int *order) {
+  g_static_n = (n > 0 && n <= 512) ? n : 0;
+  for (int i = 0; i < g_static_n; i++) g_static_order[i] = order[i];
+}
+
+static int lu_factor_static(int n, double *a, int *piv, int f32) {
+  if (g_static_n != n) return 1;
+  for (int k = 0; k < n; k++) piv[k] = g_static_order[k];
+  if (f32)
+    for (size_t i = 0; i < (size_t)n * n; i++) a[i] = (float)a[i];
+  for (int k = 0; k < n; k++) {
+    const int pk = piv[k];
+    const double d = a[(size_t)pk * n + pk];
+    if (d == 0.0) return k + 1;
+    const double inv = 1.0 / d;
+    for (int kk = k + 1; kk < n; kk++) {
+      const int i = piv[kk];
+      double l = a[(size_t)i * n + pk] * inv;
+      if (f32) l = (float)l;
+      a[(size_t)i * n + pk] = l;
+      if (l == 0.0) continue;
+      for (int jj = k + 1; jj < n; jj++) {
+        const int j = piv[jj];
+        double v = a[(size_t)i * n + j] - l * a[(size_t)pk * n + j];
+        a[(size_t)i * n + j] = f32 ? (float)v : v;
+      }
+    }
+  }
+  return 0;
+}
+
+static void lu_solve_static(int n, const double *a, const int *piv, double *b) {
+  for (int k = 0; k < n; k++) {
+    const int pk = piv[k];
+    for (int kk = k + 1; kk < n; kk++) {
+      const int i = piv[kk];
+      b[i] -= a[(size_t)i * n + pk] * b[pk];
+    }
+  }
+  for (int k = n - 1; k >= 0; k--) {
+    const int pk = piv[k];
+    b[pk] /= a[(size_t)pk * n + pk];
+    for (int kk = 0; kk < k; kk++) {
+      const int i = piv[kk];
+      b[i] -= a[(size_t)i * n + pk] * b[pk];
+    }
+  }
+}
+
+/* x = M^-1 b with the pivot-ordered inverse: x_k = sum_j A[p_k][j] b[p_j] */
+static void gj_apply(int n, const double *a, const int *piv, double *b, double *work) {
+  double *bp = work, *y = work + n;
+  for (int j = 0; j < n; j++) bp[j] = b[piv[j]];
+  for (int i = 0; i < n; i++) {
+    double acc = 0.0;
+    for (int j = 0; j < n; j++) acc += a[(size_t)i * n + j] * bp[j];
+    y[i] = acc;
+  }
+  for (int k = 0; k < n; k++) b[k] = y[piv[k]];
+}
+
 static void lu_solve(int n, const double *a, const int *piv, double *b) {
   for (int k = 0; k < n; k++) {
     int p = piv[k];
@@ -179,6 +303,68 @@ static void lu_solve(int n, const double *a, const int *piv, double *b) {
   for (int k = n - 1; k >= 0; k--) {
     b[k] /= a[(size_t)k * n + k];
     for (int i = 0; i < k; i++) b[i] -= a[(size_t)i * n + k] * b[k];
+  }
+}
+
+/* ---------------- bordered hybrid factorisation ---------------------- */
+/* Head: pivots order[0..nhead-1] eliminated in that order without pivoting
+ * (multipliers stored in place, the pivot slot holds its reciprocal).  Tail:
+ * the Schur complement on order[nhead..nb-1] is copied to `tail` and
+ * inverted by gj_invert.  This is the factorisation of the workgroup kernel. */
+static int hybrid_factor(int nb, double *a, const int *order, int nhead, double *tail, int *tpiv,
+                         int f32) {
+  if (f32)
+    for (size_t i = 0; i < (size_t)nb * nb; i++) a[i] = (float)a[i];
+  for (int h = 0; h < nhead; h++) {
+    const int k = order[h];
+    const double d = a[(size_t)k * nb + k];
+    if (d == 0.0) return 1;
+    double inv = 1.0 / d;
+    if (f32) inv = (float)inv;
+    a[(size_t)k * nb + k] = inv;
+    for (int hh = h + 1; hh < nb; hh++) {
+      const int i = order[hh];
+      double l = a[(size_t)i * nb + k] * inv;
+      if (f32) l = (float)l;
+      a[(size_t)i * nb + k] = l;
+      if (l == 0.0) continue;
+      for (int jj = h + 1; jj < nb; jj++) {
+        const int j = order[jj];
+        double v = a[(size_t)i * nb + j] - l * a[(size_t)k * nb + j];
+        a[(size_t)i * nb + j] = f32 ? (float)v : v;
+      }
+    }
+  }
+  const int t = nb - nhead;
+  for (int r = 0; r < t; r++)
+    for (int c = 0; c < t; c++)
+      tail[(size_t)r * t + c] = a[(size_t)order[nhead + r] * nb + order[nhead + c]];
+  return gj_invert(t, tail, tpiv, f32);
+}
+
+static void hybrid_solve(int nb, const double *a, const int *order, int nhead, const double *tail,
+                         const int *tpiv, double *b, double *work) {
+  for (int h = 0; h < nhead; h++) {
+    const int k = order[h];
+    const double bk = b[k];
+    for (int hh = h + 1; hh < nb; hh++) {
+      const int i = order[hh];
+      b[i] -= a[(size_t)i * nb + k] * bk;
+    }
+  }
+  const int t = nb - nhead;
+  double *bt = work;
+  for (int r = 0; r < t; r++) bt[r] = b[order[nhead + r]];
+  gj_apply(t, tail, tpiv, bt, work + t);
+  for (int r = 0; r < t; r++) b[order[nhead + r]] = bt[r];
+  for (int h = nhead - 1; h >= 0; h--) {
+    const int k = order[h];
+    double s = b[k];
+    for (int jj = h + 1; jj < nb; jj++) {
+      const int j = order[jj];
+      s -= a[(size_t)k * nb + j] * b[j];
+    }
+    b[k] = s * a[(size_t)k * nb + k];
   }
 }
 
@@ -206,8 +392,43 @@ static int dq_jac(bdf_mem *m, const double *y, const double *fy, double *J) {
 
 /* linear-solver setup (dense): decide whether J is re-evaluated, form
  * M = I - gamma*J and factor it */
+static int lsetup_hybrid(bdf_mem *m, int convfail, const double *ypred, int *jcur) {
+  const int n = m->n, nb = m->nb;
+  const size_t nn = (size_t)nb * nb;
+  double dgamma = fabs(m->gamma / m->gammap - 1.0);
+  int jbad = (m->nst == 0) || (m->nst > m->nstlj + MSBJ) ||
+             ((convfail == CV_FAIL_BAD_J) && (dgamma < DGMAX_J)) || (convfail == CV_FAIL_OTHER);
+  if (!jbad) {
+    *jcur = 0;
+  } else {
+    m->nje++;
+    m->nstlj = m->nst;
+    *jcur = 1;
+    if (m->jac(m->tn, ypred, m->savedJ, m->ud) != 0) return -1;
+    if (m->mode & PDR_MODE_F32)
+      for (size_t i = 0; i < nn; i++) m->savedJ[i] = (float)m->savedJ[i];
+  }
+  for (size_t i = 0; i < nn; i++) m->M[i] = -m->gamma * m->savedJ[i];
+  for (int i = 0; i < n; i++) m->M[(size_t)i * nb + i] += 1.0;
+  double xe = 0.0;
+  for (int j = 0; j < n; j++) {
+    m->M[(size_t)n * nb + j] = -m->bq[j];
+    xe += m->bq[j] * ypred[j];
+  }
+  m->M[(size_t)n * nb + n] = 1.0;
+  if (m->mode & PDR_MODE_SCALED) {
+    for (int i = 0; i < n; i++) m->wsc[i] = m->ewt[i];
+    m->wsc[n] = 1.0 / (m->rtol * fabs(xe) + m->atol);
+    for (int i = 0; i < nb; i++)
+      for (int j = 0; j < nb; j++) m->M[(size_t)i * nb + j] *= m->wsc[i] / m->wsc[j];
+  }
+  return hybrid_factor(nb, m->M, m->border_order, m->nhead, m->tail, m->tpiv,
+                       (m->mode & PDR_MODE_F32) != 0) ? 1 : 0;
+}
+
 static int lsetup(bdf_mem *m, int convfail, const double *ypred, const double *fpred, int *jcur) {
   const int n = m->n;
+  if (m->mode & PDR_MODE_HYBRID) return lsetup_hybrid(m, convfail, ypred, jcur);
   double dgamma = fabs(m->gamma / m->gammap - 1.0);
   int jbad = (m->nst == 0) || (m->nst > m->nstlj + MSBJ) ||
              ((convfail == CV_FAIL_BAD_J) && (dgamma < DGMAX_J)) || (convfail == CV_FAIL_OTHER);
@@ -236,6 +457,9 @@ static int lsetup(bdf_mem *m, int convfail, const double *ypred, const double *f
     for (int i = 0; i < n; i++)
       for (int j = 0; j < n; j++) m->M[(size_t)i * n + j] *= m->wsc[i] / m->wsc[j];
   }
+  if (m->mode & PDR_MODE_GJ) return gj_invert(n, m->M, m->piv, (m->mode & PDR_MODE_F32) != 0);
+  if (m->mode & PDR_MODE_STATIC)
+    return lu_factor_static(n, m->M, m->piv, (m->mode & PDR_MODE_F32) != 0) > 0 ? 1 : 0;
   int ier = lu_factor(n, m->M, m->piv, (m->mode & PDR_MODE_F32) != 0);
   return ier > 0 ? 1 : 0;
 }
@@ -243,18 +467,27 @@ static int lsetup(bdf_mem *m, int convfail, const double *ypred, const double *f
 /* solve with the factored (possibly scaled) iteration matrix */
 static void fsolve(bdf_mem *m, double *b) {
   const int n = m->n;
-  if (m->mode & PDR_MODE_SCALED) {
-    for (int i = 0; i < n; i++) b[i] *= m->wsc[i];
-    lu_solve(n, m->M, m->piv, b);
-    for (int i = 0; i < n; i++) b[i] /= m->wsc[i];
-  } else {
-    lu_solve(n, m->M, m->piv, b);
+  if (m->mode & PDR_MODE_HYBRID) {
+    const int sc = (m->mode & PDR_MODE_SCALED) != 0;
+    double *bb = m->bwork;
+    for (int i = 0; i < n; i++) bb[i] = sc ? b[i] * m->wsc[i] : b[i];
+    bb[n] = 0.0;
+    hybrid_solve(m->nb, m->M, m->border_order, m->nhead, m->tail, m->tpiv, bb, bb + m->nb);
+    for (int i = 0; i < n; i++) b[i] = sc ? bb[i] / m->wsc[i] : bb[i];
+    return;
   }
+  if (m->mode & PDR_MODE_SCALED)
+    for (int i = 0; i < n; i++) b[i] *= m->wsc[i];
+  if (m->mode & PDR_MODE_GJ) gj_apply(n, m->M, m->piv, b, m->gjw);
+  else if (m->mode & PDR_MODE_STATIC) lu_solve_static(n, m->M, m->piv, b);
+  else lu_solve(n, m->M, m->piv, b);
+  if (m->mode & PDR_MODE_SCALED)
+    for (int i = 0; i < n; i++) b[i] /= m->wsc[i];
 }
 
 static void lsolve(bdf_mem *m, double *b) {
   const int n = m->n;
-  int nref = (m->mode >> 2) & 3;
+  int nref = (m->mode & PDR_MODE_HYBRID) ? 0 : (m->mode >> 2) & 3;
   double *rhs = m->vtmp;
   if (nref) memcpy(rhs, b, sizeof(double) * n);
   fsolve(m, b);

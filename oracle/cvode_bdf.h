@@ -36,8 +36,16 @@ typedef struct {
   double *M, *savedJ;
   double *wsc;          /* error weights frozen at the last factorisation */
   double *rsd;          /* refinement work vector */
+  double *gjw;          /* 2n work vector of the Gauss-Jordan solve */
   int *piv;
   double *buf;
+  /* PDR_MODE_HYBRID: bordered matrix of size nb = n + 1 (last unknown x_e) */
+  int nb, nhead;
+  const double *bq;         /* [n] charges: x_e = sum_j bq_j x_j */
+  const int *border_order;  /* [nb] elimination order */
+  double *tail;             /* inverse of the Schur complement, (nb-nhead)^2 */
+  int *tpiv;
+  double *bwork;            /* 4*nb */
   /* step state */
   double tn, h, hscale, hprime, hu, eta, etamax, hmin, hmax_inv;
   double tau[7], tq[6], l[6];
@@ -49,6 +57,8 @@ typedef struct {
 bdf_mem *bdf_create(int n, pdr_rhs_fn f, pdr_jac_fn jac, void *ud, double rtol, double atol,
                     long mxstep, int mode);
 void bdf_free(bdf_mem *m);
+/* PDR_MODE_HYBRID only: border charges and the static elimination order */
+void bdf_set_border(bdf_mem *m, const double *q, int nhead, const int *order);
 int bdf_init(bdf_mem *m, double t0, const double *y0);
 int bdf_solve(bdf_mem *m, double tout, double *yout, double *tret);
 int bdf_get_dky(bdf_mem *m, double t, double *dky);

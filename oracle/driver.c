@@ -22,6 +22,7 @@ typedef struct {
   const double *rate;
   double n_H;
   double x_e; /* charge sum of the most recent RHS evaluation */
+  int bordered;
 } chem_ud;
 
 static int chem_f(double t, const double *y, double *ydot, void *ud) {
@@ -34,8 +35,23 @@ static int chem_f(double t, const double *y, double *ydot, void *ud) {
 static int chem_jac(double t, const double *y, double *J, void *ud) {
   (void)t;
   chem_ud *c = (chem_ud *)ud;
-  pdr_jac(c->net, c->rate, c->n_H, y, J);
+  if (c->bordered) pdr_jac_bordered(c->net, c->rate, c->n_H, y, J);
+  else pdr_jac(c->net, c->rate, c->n_H, y, J);
   return 0;
+}
+
+/* PDR_MODE_HYBRID needs the network's elimination order and the charges */
+static bdf_mem *create_for_net(const pdr_net *net, chem_ud *ud, double rtol, double atol,
+                               long mxstep, int mode, double *qbuf) {
+  if ((mode & PDR_MODE_HYBRID) && (!net->border_order || net->nb != net->neq + 1)) return NULL;
+  bdf_mem *m = bdf_create(net->neq, chem_f, chem_jac, ud, rtol, atol, mxstep, mode);
+  if (!m) return NULL;
+  if (mode & PDR_MODE_HYBRID) {
+    ud->bordered = 1;
+    for (int j = 0; j < net->neq; j++) qbuf[j] = net->charge[j];
+    bdf_set_border(m, qbuf, net->nhead, net->border_order);
+  }
+  return m;
 }
 
 static void fill_stats(pdr_cvode_stats *st, const bdf_mem *m, int flag) {
@@ -51,9 +67,10 @@ static int solve_point(const pdr_net *net, double *ab, const double *rate, doubl
                        int nelect0, int mode, pdr_cvode_stats *st) {
   const int neq = net->neq;
   const long mxstep = 10000000L;
-  chem_ud ud = {net, rate, n_H, ab[nelect0]};
-  bdf_mem *m = bdf_create(neq, chem_f, chem_jac, &ud, reltol, abstol, mxstep, mode);
-  if (!m) return 1;
+  chem_ud ud = {net, rate, n_H, ab[nelect0], 0};
+  double *qbuf = malloc(sizeof(double) * (neq + 1));
+  bdf_mem *m = qbuf ? create_for_net(net, &ud, reltol, abstol, mxstep, mode, qbuf) : NULL;
+  if (!m) { free(qbuf); return 1; }
   double t0 = start_time * SECONDS_IN_YEAR;
   double tend = end_time * SECONDS_IN_YEAR;
   bdf_init(m, t0, ab);
@@ -86,6 +103,7 @@ static int solve_point(const pdr_net *net, double *ab, const double *rate, doubl
   fill_stats(st, m, flag);
   free(y);
   bdf_free(m);
+  free(qbuf);
   return status;
 }
 
@@ -116,9 +134,10 @@ int pdr_calculate_abundances(double *abundance, const double *rate, const double
 int pdr_integrate_point(const pdr_net *net, const double *rate, double n_H, const double *y0,
                         double t0, int ntout, const double *tout, double *yout,
                         const pdr_cvode_opts *o, pdr_cvode_stats *st) {
-  chem_ud ud = {net, rate, n_H, 0.0};
-  bdf_mem *m = bdf_create(net->neq, chem_f, chem_jac, &ud, o->rtol, o->atol, o->mxstep, o->mode);
-  if (!m) return 1;
+  chem_ud ud = {net, rate, n_H, 0.0, 0};
+  double *qbuf = malloc(sizeof(double) * (net->neq + 1));
+  bdf_mem *m = qbuf ? create_for_net(net, &ud, o->rtol, o->atol, o->mxstep, o->mode, qbuf) : NULL;
+  if (!m) { free(qbuf); return 1; }
   bdf_init(m, t0, y0);
   int flag = 0;
   for (int k = 0; k < ntout; k++) {
@@ -128,6 +147,7 @@ int pdr_integrate_point(const pdr_net *net, const double *rate, double n_H, cons
   }
   fill_stats(st, m, flag);
   bdf_free(m);
+  free(qbuf);
   return flag;
 }
 

@@ -21,7 +21,7 @@ int pdr_net_load(const char *path, pdr_net *net) {
   net->xlambda_spline = 1;
   FILE *fh = fopen(path, "r");
   if (!fh) return -1;
-  char line[4096];
+  char line[16384];
   int nspec = -1, nreac = -1, ns = 0, nr = 0, nst = 0, cap = 0;
   while (fgets(line, sizeof line, fh)) {
     if (line[0] == '#' || line[0] == '\n') continue;
@@ -81,6 +81,19 @@ int pdr_net_load(const char *path, pdr_net *net) {
       }
       nr++;
       net->st_ptr[nr] = nst;
+    } else if (line[0] == 'O') {
+      int nb, nh, off;
+      char *p = line + 1;
+      if (nspec < 0 || sscanf(p, "%d %d%n", &nb, &nh, &off) != 2 || nb != nspec) goto bad;
+      p += off;
+      net->nb = nb; net->nhead = nh;
+      net->border_order = xcalloc(nb, sizeof(int));
+      for (int j = 0; j < nb; j++) {
+        int v, o2;
+        if (sscanf(p, "%d%n", &v, &o2) != 1 || v < 0 || v >= nb) goto bad;
+        p += o2;
+        net->border_order[j] = v;
+      }
     } else if (line[0] == 'E') {
       break;
     }
@@ -101,6 +114,7 @@ void pdr_net_free(pdr_net *net) {
   free(net->alpha); free(net->beta); free(net->gamma);
   free(net->rtmin); free(net->rtmax);
   free(net->st_ptr); free(net->st_idx); free(net->st_coef);
+  free(net->border_order);
   memset(net, 0, sizeof(*net));
 }
 

@@ -25,9 +25,17 @@ SECONDS_IN_YEAR = 3.1556926e7
 MODE_REFERENCE = 0          # CVODE difference-quotient Jacobian, plain dense LU
 MODE_EXACT_JAC = 1
 MODE_SCALED = 2
-MODE_GPU = 1 | 2 | (3 << 2)  # exact J, weight-scaled matrix, 3 refinement steps
 MODE_F32 = 16
+MODE_GJ = 64
+MODE_GPU = 1 | 2 | MODE_GJ       # exact J, weight-scaled matrix, Gauss-Jordan inverse
 MODE_GPU_MIXED = MODE_GPU | MODE_F32
+MODE_STATIC = 128
+MODE_HYBRID = 256
+MODE_GPU_WG = 1 | 2 | MODE_HYBRID  # bordered matrix, static head + Gauss-Jordan tail
+MODE_GPU_WG_MIXED = MODE_GPU_WG | MODE_F32
+
+def MODE_REFINE(k):
+    return k << 2
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 
@@ -83,6 +91,7 @@ def lib():
                                       _dp, _dp, _dp, _dp]
         L.pdr_rhs.argtypes = [C.c_void_p, _dp, C.c_double, _dp, _dp, C.POINTER(C.c_double)]
         L.pdr_jac.argtypes = [C.c_void_p, _dp, C.c_double, _dp, _dp]
+        L.pdr_jac_bordered.argtypes = [C.c_void_p, _dp, C.c_double, _dp, _dp]
         L.pdr_calculate_abundances.argtypes = [_dp, _dp, _dp, _dp, C.c_int, C.c_int, C.c_int,
                                                C.c_void_p, C.c_double, C.c_double, C.c_double,
                                                C.c_double, C.c_int, C.c_int, C.c_int, C.c_void_p]
@@ -148,6 +157,15 @@ class Oracle:
         lib().pdr_jac(self.h, np.ascontiguousarray(rate, dtype=np.float64), n_H,
                       np.ascontiguousarray(x, dtype=np.float64), J)
         return J.reshape(self.neq, self.neq)
+
+    def jac_bordered(self, rate, n_H, x):
+        """Jacobian of the bordered system, (neq+1)^2: rows < neq hold
+        d ydot / d x at fixed x_e and d ydot / d x_e (last column)."""
+        nb = self.neq + 1
+        J = np.zeros(nb * nb)
+        lib().pdr_jac_bordered(self.h, np.ascontiguousarray(rate, dtype=np.float64), n_H,
+                               np.ascontiguousarray(x, dtype=np.float64), J)
+        return J.reshape(nb, nb)
 
     def calculate_abundances(self, abundance, rate, density, temperature=None, rtol=1e-8,
                              atol=1e-30, start_time=0.0, end_time=1e7, mode=0,

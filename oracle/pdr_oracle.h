@@ -46,6 +46,11 @@ typedef struct {
   double *alpha, *beta, *gamma, *rtmin, *rtmax; /* [nreac] */
   int *st_ptr;          /* [nreac+1] CSR of net stoichiometry */
   int *st_idx, *st_coef;
+  /* bordered iteration matrix (unknowns x_0..x_{neq-1}, x_e): size nb =
+   * neq+1, static elimination order, first nhead pivots eliminated without
+   * pivoting, the rest inverted by Gauss-Jordan (pdrchem/sparse.py) */
+  int nb, nhead;
+  int *border_order;    /* [nb] */
 } pdr_net;
 
 int pdr_net_load(const char *path, pdr_net *net);
@@ -82,19 +87,33 @@ void pdr_rhs(const pdr_net *net, const double *rate, double n_H,
              const double *x, double *ydot, double *x_e);
 void pdr_jac(const pdr_net *net, const double *rate, double n_H,
              const double *x, double *J /* row-major neq*neq */);
+/* Jacobian of the bordered system: Jb row-major (neq+1)^2, rows i < neq hold
+ * d ydot_i / d x_j at fixed x_e (columns j < neq) and d ydot_i / d x_e
+ * (column neq); row neq is zero */
+void pdr_jac_bordered(const pdr_net *net, const double *rate, double n_H,
+                      const double *x, double *Jb);
 
 /* integrator method bits.  0 is the reference configuration: CVODE's
  * difference-quotient Jacobian and a plain dense LU.  The HIP solver runs
  * PDR_MODE_GPU: exact Jacobian, iteration matrix scaled by the error weights
- * (row i by w_i, column j by 1/w_j) and three steps of iterative refinement
- * with an FP64 residual per linear solve; PDR_MODE_GPU_MIXED also rounds the
- * saved J and the factorisation to FP32. */
+ * (row i by w_i, column j by 1/w_j) and inverted explicitly by Gauss-Jordan
+ * with partial pivoting; PDR_MODE_GPU_MIXED also rounds the saved J and the
+ * inverse to FP32.  REFINE(k) adds k steps of iterative refinement with an
+ * FP64 residual (used to compare linear-solve variants). */
 #define PDR_MODE_EXACT_JAC 1
 #define PDR_MODE_SCALED 2
 #define PDR_MODE_REFINE(k) ((k) << 2) /* k = 0..3 refinement steps */
-#define PDR_MODE_F32 16               /* saved J rounded to FP32; scaled matrix factored in FP32 */
-#define PDR_MODE_GPU (PDR_MODE_EXACT_JAC | PDR_MODE_SCALED | PDR_MODE_REFINE(3))
+#define PDR_MODE_F32 16               /* saved J and factorisation rounded to FP32 */
+#define PDR_MODE_GJ 64                /* explicit Gauss-Jordan inverse instead of LU */
+#define PDR_MODE_STATIC 128           /* LU without pivoting in a fixed order */
+#define PDR_MODE_HYBRID 256           /* bordered matrix: static head + Gauss-Jordan tail */
+#define PDR_MODE_GPU_WG (PDR_MODE_EXACT_JAC | PDR_MODE_SCALED | PDR_MODE_HYBRID)
+#define PDR_MODE_GPU_WG_MIXED (PDR_MODE_GPU_WG | PDR_MODE_F32)
+#define PDR_MODE_GPU (PDR_MODE_EXACT_JAC | PDR_MODE_SCALED | PDR_MODE_GJ)
 #define PDR_MODE_GPU_MIXED (PDR_MODE_GPU | PDR_MODE_F32)
+
+/* elimination order used by PDR_MODE_STATIC (n <= 512) */
+void pdr_set_static_order(int n, const int *order);
 
 /* integrator options / statistics */
 typedef struct {

@@ -73,3 +73,27 @@ void pdr_jac(const pdr_net *n, const double *rate, double n_H, const double *x, 
       if (n->charge[j]) J[(size_t)i * neq + j] += gp[i] * n->charge[j];
   }
 }
+
+void pdr_jac_bordered(const pdr_net *n, const double *rate, double n_H, const double *x,
+                      double *Jb) {
+  const int neq = n->neq, nb = neq + 1;
+  double xe = 0.0;
+  for (int i = 0; i < neq; i++)
+    if (n->charge[i]) xe += n->charge[i] * x[i];
+  memset(Jb, 0, sizeof(double) * (size_t)nb * nb);
+  for (int r = 0; r < n->nreac; r++) {
+    int s0 = n->st_ptr[r], s1 = n->st_ptr[r + 1];
+    if (s0 == s1) continue;
+    double k = rate[r];
+    if (n->nh[r]) k *= n_H;
+    int a = n->fa[r], b = n->fb[r];
+    double xa = xval(n, x, xe, a), xb = xval(n, x, xe, b);
+    double da = k * xb, db = k * xa;
+    for (int s = s0; s < s1; s++) {
+      int i = n->st_idx[s];
+      double c = n->st_coef[s];
+      if (a <= neq) Jb[(size_t)i * nb + a] += c * da;  /* column neq is x_e */
+      if (b <= neq) Jb[(size_t)i * nb + b] += c * db;
+    }
+  }
+}

@@ -104,29 +104,78 @@ def test_device_rates_match_oracle(orc):
 
 
 def test_random_points_device_path(orc, net):
-    """Config-2 distribution, device-resident path: device rates -> solve."""
+    """Config-2 distribution, device-resident path: device rates -> solve.
+
+    On some dense, warm points the local error control at rtol = 1e-8 lets the
+    global error of any BDF run (the oracle's included) reach ~1e-4 after
+    ~1e5 steps, so two correct integrators can differ by more than 1e-4
+    there.  Bar: GPU vs FP64 oracle <= 1e-4 on every point where the oracle
+    itself is within 2.5e-5 of the converged solution (oracle at rtol 1e-11,
+    atol 1e-33); on the remaining points the GPU must be no further from the
+    converged solution than max(1e-4, 2x the oracle's own error)."""
+    import torch
+    path = os.path.join(GOLD, "converged_reduced.npz")
+    if not os.path.exists(path):
+        pytest.skip("converged fixture missing")
+    d = np.load(path)
+    dev = torch.device("cuda:0")
+    nid = lib.network_id("reduced")
+    x0, ref, conv = d["x0"], d["oracle"], d["converged"]
+    n = x0.shape[0]
+    rate = torch.tensor(d["rates"], device=dev)
+    oerr = np.array([max_rel(ref[k, :-1], conv[k, :-1]) for k in range(n)])
+    good = oerr <= 2.5e-5
+    for prec in (lib.FP64, lib.MIXED):
+        ab = torch.tensor(x0, device=dev)
+        stats = torch.zeros((n, 8), dtype=torch.int32, device=dev)
+        status, _ = lib.solve_device(nid, ab, rate, torch.tensor(d["n_H"], device=dev), stats=stats,
+                                     precision=prec)
+        torch.cuda.synchronize()
+        assert int(status.abs().sum().item()) == 0
+        got = ab.cpu().numpy()
+        gerr_o = np.array([max_rel(got[k, :-1], ref[k, :-1]) for k in range(n)])
+        gerr_c = np.array([max_rel(got[k, :-1], conv[k, :-1]) for k in range(n)])
+        print(f"precision {prec}: {good.sum()}/{n} well-conditioned points, max err vs oracle there "
+              f"{gerr_o[good].max():.2e}; max err vs converged (all) {gerr_c.max():.2e} (oracle's own "
+              f"{oerr.max():.2e}); nst mean GPU {stats[:, 0].float().mean().item():.0f} oracle "
+              f"{d['oracle_nst'].mean():.0f}")
+        dump = os.environ.get("PDR_DUMP_DIR")
+        if dump:
+            np.savez(os.path.join(dump, f"rand96_gpu_p{prec}.npz"), ab=got, stats=stats.cpu().numpy())
+        assert gerr_o[good].max() <= TOL
+        assert np.all(gerr_c <= np.maximum(TOL, 2.0 * oerr))
+
+
+def test_device_rates_then_solve(orc, net):
+    """Device-resident pipeline on seeded random points: device rates feed the
+    solve without a host round trip; checked against the oracle."""
     import torch
     dev = torch.device("cuda:0")
-    P = workloads.random_points(96)
+    P = workloads.random_points(32, seed=7)
     x0 = workloads.initial_abundances(net, P.n)
     nid = lib.network_id("reduced")
-    T = torch.tensor(P.T, device=dev)
-    rate, neg = lib.rates_device(nid, T, torch.tensor(P.Td, device=dev), torch.tensor(P.G, device=dev),
-                                 torch.tensor(P.Av, device=dev), torch.tensor(P.cols, device=dev))
+    rate, neg = lib.rates_device(nid, torch.tensor(P.T, device=dev), torch.tensor(P.Td, device=dev),
+                                 torch.tensor(P.G, device=dev), torch.tensor(P.Av, device=dev),
+                                 torch.tensor(P.cols, device=dev))
     ab = torch.tensor(x0, device=dev)
-    stats = torch.zeros((P.n, 8), dtype=torch.int32, device=dev)
-    status, _ = lib.solve_device(nid, ab, rate, torch.tensor(P.n_H, device=dev), stats=stats,
-                                 precision=lib.MIXED)
+    status, _ = lib.solve_device(nid, ab, rate, torch.tensor(P.n_H, device=dev))
     torch.cuda.synchronize()
-    assert int(status.abs().sum().item()) == 0
-    got = ab.cpu().numpy()
-    st, ref, ostats = orc.calculate_abundances(x0.copy(), rate.cpu().numpy(), P.n_H, P.T,
-                                               mode=oracle.MODE_GPU_MIXED, nthreads=16)
+    assert int(neg.item()) == 0 and int(status.abs().sum().item()) == 0
+    r_or = orc.rates(P.T, P.Td, P.G, P.Av, P.cols)
+    assert np.allclose(rate.cpu().numpy(), r_or, rtol=1e-12, atol=0)
+    st, ref, _ = orc.calculate_abundances(x0.copy(), r_or, P.n_H, P.T, mode=oracle.MODE_GPU, nthreads=16)
     assert st == 0
-    err = max_rel(got[:, :-1], ref[:, :-1])
-    print(f"random 96: max rel err {err:.3e}; GPU nst mean {stats[:, 0].float().mean().item():.0f} "
-          f"oracle nst mean {ostats['nst'].mean():.0f}")
-    assert err <= TOL
+    got = ab.cpu().numpy()
+    for k in range(P.n):
+        if max_rel(got[k, :-1], ref[k, :-1]) <= TOL:
+            continue
+        # outside the bar: the point must be one where the oracle itself is
+        # inaccurate, and the GPU no worse than 2x the oracle's own error
+        _, conv, _ = orc.calculate_abundances(x0[k:k + 1].copy(), r_or[k:k + 1], P.n_H[k:k + 1], P.T[k:k + 1],
+                                              rtol=1e-11, atol=1e-33, mode=oracle.MODE_GPU, nthreads=1)
+        oe = max_rel(ref[k, :-1], conv[0, :-1])
+        assert oe > 2.5e-5
+        assert max_rel(got[k, :-1], conv[0, :-1]) <= max(TOL, 2 * oe)
 
 
 def test_alone_equals_in_mix(orc, net):

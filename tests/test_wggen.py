@@ -1,0 +1,223 @@
+"""CPU checks of the workgroup-kernel tables (pdrchem/wggen.py, pdrchem/sparse.py).
+
+The HIP kernel (csrc/bdf_wg_impl.h) is table-driven: it executes, lane by
+lane, the lists the generator emits.  These tests execute the same lists in
+numpy, with the kernel's lane/round layout and level order, and compare with
+independent computations:
+
+* RHS (fluxes, chunk sums, row sums) against the oracle's pdr_rhs
+  (odes_full.c:26-700 semantics);
+* Jacobian entries against the oracle's bordered Jacobian, itself pinned by
+  central differences of the RHS in test_oracle.py;
+* the M build, level-ordered head elimination, Schur complement from the
+  L_TH / U_HT position tables, tail inverse and the five-stage solve against
+  numpy.linalg.solve of the dense bordered matrix;
+* the bordered solve against the un-bordered Newton matrix I - gamma*(J0 + g q^T),
+  which is what calculate_abundances' CVODE solve factors.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from pdrchem import sparse, workloads, wggen
+from pdrchem.netgen import load_network
+
+NETS = ["mynetwork", "full"]
+
+
+@pytest.fixture(scope="module", params=NETS)
+def case(request):
+    name = request.param
+    net = load_network(name)
+    tab = wggen.WgTables(net)
+    orc = oracle.Oracle(name)
+    P = workloads.random_points(3, seed=7, columns=True)
+    rate = orc.rates(P.T, P.Td, P.G, P.Av, P.cols)[1]
+    rng = np.random.default_rng(11)
+    x = 10.0 ** rng.uniform(-14, -1, net.neq)
+    return name, net, tab, orc, rate, float(P.n_H[1]), x
+
+
+def _kp(tab, net, rate, nH):
+    kp = np.zeros(tab.nslot)
+    for s, r in enumerate(tab.slot_reac):
+        rr = net.reactions[r]
+        kp[s] = rate[r] * (nH if rr.nh else 1.0)
+    return kp
+
+
+def _xvec(net, x):
+    q = np.asarray(net.charges(), dtype=float)
+    xe = float(np.dot(q, x))
+    return np.concatenate([x, [xe, 1.0]]), xe
+
+
+def _coef(v, bit):
+    c = (v >> bit) & 0xF
+    return c - 16 if c >= 8 else c
+
+
+def test_rhs_tables(case):
+    name, net, tab, orc, rate, nH, x = case
+    X, xe = _xvec(net, x)
+    kp = _kp(tab, net, rate, nH)
+    fab = np.asarray(tab.slot_fab, dtype=np.int64)
+    F = np.zeros(tab.nslot + 1)
+    F[: tab.nslot] = kp * X[fab & 0xFF] * X[(fab >> 8) & 0xFF]
+    P = np.zeros(tab.nv + 1)
+    for p in range(tab.vl):
+        for lane in range(tab.block):
+            acc = 0.0
+            for t in range(tab.tm):
+                w = tab.rhs_term[p][t][lane]
+                acc += _coef(w, 12) * F[w & 0xFFF]
+            P[p * tab.block + lane] = acc
+    yd = np.array([sum(P[tab.row_base[i] + u] for u in range(tab.row_nl[i])) for i in range(net.neq)])
+    ref, xe_ref = orc.rhs(rate, nH, x)
+    assert xe == pytest.approx(xe_ref, rel=1e-14)
+    scale = np.maximum(np.abs(ref), 1e-300)
+    # sums of many terms of both signs: compare against the size of the terms
+    mag = np.zeros(net.neq)
+    for r in net.reactions:
+        if not r.stoich:
+            continue
+        fl = abs(rate[r.index] * (nH if r.nh else 1.0) * X[r.fa] * X[r.fb])
+        for i, c in r.stoich.items():
+            mag[i] += abs(c) * fl
+    assert np.all(np.abs(yd - ref) <= 1e-13 * np.maximum(mag, scale)), name
+
+
+def test_jacobian_tables(case):
+    name, net, tab, orc, rate, nH, x = case
+    X, xe = _xvec(net, x)
+    kp = _kp(tab, net, rate, nH)
+    fab = np.asarray(tab.slot_fab, dtype=np.int64)
+    D = np.zeros(2 * tab.nslot + 1)
+    D[0:2 * tab.nslot:2] = kp * X[(fab >> 8) & 0xFF]
+    D[1:2 * tab.nslot:2] = kp * X[fab & 0xFF]
+    Jg = np.full(tab.nj, np.nan)
+    for lane in range(tab.block):
+        acc = 0.0
+        for t in range(tab.jtm):
+            w = tab.jac_term[t][lane]
+            acc += _coef(w, 13) * D[w & 0x1FFF]
+            if (w >> 17) & 1:
+                Jg[w >> 18] = acc
+                acc = 0.0
+    assert not np.any(np.isnan(Jg)), "every entry is written"
+    Jb = orc.jac_bordered(rate, nH, x)
+    got = np.zeros_like(Jb)
+    for e, (i, j) in enumerate(tab.jent):
+        got[i, j] = Jg[e]
+    mask = np.zeros_like(Jb, dtype=bool)
+    for (i, j) in tab.jent:
+        mask[i, j] = True
+    assert np.all(Jb[~mask] == 0.0), "no oracle entry outside the table pattern"
+    np.testing.assert_allclose(got, Jb, rtol=1e-12, atol=1e-300)
+
+
+def _factor_and_solve(tab, net, Jb, gamma, w, rhs):
+    """The kernel's setup + solve, in the kernel's order, on numpy arrays."""
+    n, NB = net.neq, tab.nb
+    q = np.asarray(net.charges(), dtype=float)
+    jid = {e: k for k, e in enumerate(tab.jent)}
+    Jg = np.array([Jb[i, j] for (i, j) in tab.jent])
+    LU = np.zeros(tab.nlu + 1)
+    for p in range(tab.nlu):
+        d = tab.mpos[p]
+        row, col, src = d & 0xFF, (d >> 8) & 0xFF, d >> 16
+        if row in (0xFF, 0xFE):
+            continue
+        if row == n:
+            m = 1.0 if col == n else -q[col]
+        else:
+            m = (-gamma * Jg[src] if src != 0xFFFF else 0.0) + (1.0 if row == col else 0.0)
+        LU[p] = m * w[row] / w[col]
+    dinv = np.zeros(NB)
+    for lv in range(tab.nlev):
+        for rnd in tab.elimA[lv]:
+            for v in rnd:
+                kind, tgt, dp = v >> 28, v & 0x3FFF, (v >> 14) & 0x3FFF
+                if kind == 1:
+                    LU[tgt] /= LU[dp]
+                elif kind == 2:
+                    dinv[tgt] = 1.0 / LU[dp]
+        acc = np.zeros(tab.block)
+        for rnd in tab.elimB[lv]:
+            for lane, v in enumerate(rnd):
+                tgt, lp, up = v & 0x3FFF, (v >> 14) & 0x3FFF, (v >> 28) & 0x3FFF
+                acc[lane] += LU[lp] * LU[up]
+                if (v >> 42) & 1:
+                    LU[tgt] -= acc[lane]
+                    acc[lane] = 0.0
+    NT, TS, KH = tab.nt, tab.tstride, tab.kh
+    T = np.array([[LU[tab.tbase + r * TS + c] for c in range(NT)] for r in range(NT)])
+    if KH:
+        Lth = np.array(tab.lth, dtype=np.int64).reshape(64, KH)
+        Uht = np.array(tab.uht, dtype=np.int64).reshape(KH, 64)
+        T = T - LU[Lth][:NT] @ LU[Uht][:, :NT]
+    Tinv = np.linalg.inv(T)
+    # solve
+    B = np.zeros(NB + 1)
+    B[:n] = rhs * w[:n]
+    for lv in tab.fwd:
+        acc = np.zeros(tab.block)
+        for rnd in lv:
+            for lane, v in enumerate(rnd):
+                acc[lane] += LU[(v >> 8) & 0x3FFF] * B[(v >> 22) & 0xFF]
+                if v >> 30:
+                    B[v & 0xFF] -= acc[lane]
+                    acc[lane] = 0.0
+    bt = B[tab.tail[:NT]].copy() if NT else np.zeros(0)
+    bt = np.array([B[tab.tail[r]] for r in range(NT)])
+    if KH:
+        hr = np.asarray(tab.head_row)
+        bt -= LU[Lth][:NT] @ B[hr]
+    xt = Tinv @ bt
+    for r in range(NT):
+        B[tab.tail[r]] = xt[r]
+    if KH:
+        for a in range(tab.nh_piv):
+            B[tab.head_row[a]] -= sum(LU[Uht[a, j]] * B[tab.tail[j]] for j in range(NT))
+    for lv in tab.bwd:
+        acc = np.zeros(tab.block)
+        for rnd in lv:
+            for lane, v in enumerate(rnd):
+                acc[lane] += LU[(v >> 8) & 0x3FFF] * B[(v >> 22) & 0xFF]
+                if v >> 30:
+                    k = v & 0xFF
+                    B[k] = (B[k] - acc[lane]) * dinv[k]
+                    acc[lane] = 0.0
+    return B[:n] / w[:n]
+
+
+@pytest.mark.parametrize("gamma", [1e3, 1e9])
+def test_factor_solve_tables(case, gamma):
+    name, net, tab, orc, rate, nH, x = case
+    n = net.neq
+    Jb = orc.jac_bordered(rate, nH, x)
+    rng = np.random.default_rng(5)
+    w = np.concatenate([1.0 / (1e-8 * x + 1e-30), [1.0 / (1e-8 * abs(np.dot(net.charges(), x)) + 1e-30)]])
+    rhs = rng.standard_normal(n) * x
+    got = _factor_and_solve(tab, net, Jb, gamma, w, rhs)
+    # dense reference: the un-bordered Newton matrix with the x_e chain term
+    q = np.asarray(net.charges(), dtype=float)
+    M = np.eye(n) - gamma * (Jb[:n, :n] + np.outer(Jb[:n, n], q))
+    ref = np.linalg.solve(M, rhs)
+    err = np.abs(got - ref) * w[:n]
+    assert np.max(err) < 1e-6 * max(1.0, np.max(np.abs(ref) * w[:n])), (name, gamma, np.max(err))
+
+
+def test_analysis_structure():
+    """Structural facts the kernel relies on."""
+    for name in NETS:
+        net = load_network(name)
+        an = sparse.analyse(net)
+        assert an.nb == net.neq + 1 and an.ntail <= 64
+        assert sorted(an.order) == list(range(an.nb))
+        # head elimination never needs pivoting by position: every head pivot
+        # is a structural nonzero
+        A = sparse.bordered_pattern(net)
+        assert all(A[k, k] for k in an.head)
+        # the electron border stays in the tail (the head never divides by it)
+        assert (an.nb - 1) in an.tail
