@@ -255,7 +255,7 @@ __device__ __forceinline__ void schur_mfma(const Team<float> &c, float LDSA *LU)
 
 __device__ __forceinline__ void schur_mfma(const Team<double> &c, double LDSA *LU) {
   // v_mfma_f64_16x16x4f64: lane l supplies A[i = l%16][k = l/16] and
-  // B[k = l/16][j = l%16]; result register r of lane l is D[4*(l/16) + r][l%16].
+  // B[k = l/16][j = l%16]; result register r of lane l is D[4*r + l/16][l%16].
   // wave w computes the four 16x16 tiles of row band w.
   const int li = c.lane & 15, lk = c.lane >> 4;
   double LDSA *Tl = LU + NET::TBASE;
@@ -264,7 +264,7 @@ __device__ __forceinline__ void schur_mfma(const Team<double> &c, double LDSA *L
     const int i0 = 16 * c.wave, j0 = 16 * tj;
     f64x4 acc;
 #pragma unroll
-    for (int r = 0; r < 4; r++) acc[r] = Tl[(i0 + 4 * lk + r) * NET::TS + j0 + li];
+    for (int r = 0; r < 4; r++) acc[r] = Tl[(i0 + 4 * r + lk) * NET::TS + j0 + li];
     const uint16_t *lrow = opaque(NET::lth_pos) + (i0 + li) * NET::KH;
     const uint16_t *ucol = opaque(NET::uht_pos) + j0 + li;
 #pragma unroll 2
@@ -275,7 +275,7 @@ __device__ __forceinline__ void schur_mfma(const Team<double> &c, double LDSA *L
       acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bv, acc, 0, 0, 0);
     }
 #pragma unroll
-    for (int r = 0; r < 4; r++) Tl[(i0 + 4 * lk + r) * NET::TS + j0 + li] = acc[r];
+    for (int r = 0; r < 4; r++) Tl[(i0 + 4 * r + lk) * NET::TS + j0 + li] = acc[r];
   }
 }
 

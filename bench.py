@@ -1,10 +1,12 @@
 """Benchmark: grid-point chemistry solves/s on one MI355X.
 
-Workload: the reduced network (33 species / 329 reactions) on the seeded
-random-point distribution (log10 n_H ~ U[2,6], log10 T ~ U[1,3], A_V ~
-U[0,20], log10 G0 ~ U[0,5], zeta = 5e-17 s^-1), 1,000,000 points, initial
-abundances H = 1, He = 0.1, C+ = 1.4e-4, O = 3e-4, rest 1e-30, integrated to
-1e7 yr at rtol 1e-8 / atol 1e-30.
+Default workload (the metric's configuration): the reduced network (33
+species / 329 reactions) on the seeded random-point distribution (log10 n_H
+~ U[2,6], log10 T ~ U[1,3], A_V ~ U[0,20], log10 G0 ~ U[0,5], zeta =
+5e-17 s^-1), 1,000,000 points, initial abundances H = 1, He = 0.1, C+ =
+1.4e-4, O = 3e-4, rest 1e-30, integrated to 1e7 yr at rtol 1e-8 / atol 1e-30.
+``--network full`` runs the full network (215 species / 2926 reactions) on
+the same distribution plus log-uniform N(H2), N(CO) in [1e14, 1e22] cm^-2.
 
 One step = device rate evaluation for every point + reset of the abundances
 to the initial state + the batched BDF solve of every point (all inputs
@@ -22,17 +24,35 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "3d-pdr_amd"))
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
-FP64_PEAK_TFLOPS = 78.6     # MI355X FP64 (vector and matrix) peak
+FP64_PEAK_TFLOPS = 78.6     # MI355X FP64 peak (vector and matrix)
+FP32_PEAK_TFLOPS = 157.3    # MI355X FP32 peak (matrix)
 HBM_PEAK_GBS = 8000.0
+PROFILES = os.path.join(ROOT, "profiles")
 
 
 def algorithmic_flops(stats, n, n_rhs_flops, n_jac_flops):
-    """W = nfe*F_f + nje*F_J + nsetups*(n^2 + 2n^3/3) + nni*2n^2 + nst*10n
-    (integrator counters of the run; F_f, F_J static op counts of the
-    network's RHS and exact Jacobian)."""
+    """Work of the reference algorithm (CVODE BDF + Newton + dense LU) for the
+    integrator counters of this run:
+      W64 = nfe*F_f + nje*F_J + nni*2n^2 + nst*10n      (RHS, Jacobian, solves, Nordsieck)
+      WLU = nsetups*(n^2 + 2n^3/3)                      (forming and factoring M)
+    F_f, F_J: static op counts of the network's RHS and exact Jacobian."""
     nst, nfe, nje, nsetups, nni = (stats[:, k].astype(np.float64).sum() for k in (0, 1, 2, 3, 4))
-    return (nfe * n_rhs_flops + nje * n_jac_flops + nsetups * (n * n + 2.0 * n ** 3 / 3.0)
-            + nni * 2.0 * n * n + nst * 10.0 * n)
+    w64 = nfe * n_rhs_flops + nje * n_jac_flops + nni * 2.0 * n * n + nst * 10.0 * n
+    wlu = nsetups * (n * n + 2.0 * n ** 3 / 3.0)
+    return w64, wlu
+
+
+def measured_traffic(kernel_name, npts):
+    """HBM bytes per launch of the solver kernel from the committed rocprof
+    counter summary (bytes per point, copy-kernel calibrated), scaled to npts;
+    None when no summary for this kernel exists."""
+    path = os.path.join(PROFILES, "traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        d = json.load(fh)
+    e = d.get(kernel_name)
+    return None if e is None else float(e["bytes_per_point"]) * npts
 
 
 def network_flops(net):
@@ -44,23 +64,30 @@ def network_flops(net):
     return f_rhs, f_jac
 
 
-def cpu_baseline(net, pts, x0, rates, seconds, threads):
+def oracle_mode(name):
+    import oracle
+    return oracle.MODE_GPU if name == "reduced" else oracle.MODE_GPU_WG
+
+
+def cpu_baseline(name, pts, x0, rates, seconds, threads):
     """Oracle (CPU, same integrator as the GPU path) on a bounded sample of
     the same workload: chunks of points until `seconds` have elapsed."""
     import oracle
-    o = oracle.Oracle("reduced")
-    done, t0, chunk = 0, time.perf_counter(), max(threads * 4, 16)
+    o = oracle.Oracle(name)
+    done, t0, chunk = 0, time.perf_counter(), max(threads * 2, 8)
     while time.perf_counter() - t0 < seconds and done < pts.n:
         sl = slice(done, min(done + chunk, pts.n))
         st, _, _ = o.calculate_abundances(x0[sl].copy(), rates[sl], pts.n_H[sl], pts.T[sl],
-                                          mode=oracle.MODE_GPU, nthreads=threads)
+                                          mode=oracle_mode(name), nthreads=threads)
         done = sl.stop
     el = time.perf_counter() - t0
+    how = ("exact Jacobian, error-weight-scaled Gauss-Jordan inverse" if name == "reduced" else
+           "exact Jacobian, bordered matrix, static-order head elimination and Gauss-Jordan tail")
     return {"value": done / el, "unit": "solves/s", "cores": threads, "kind": "port",
             "sample": f"first {done} points of the same seeded workload in {el:.1f} s; CPU oracle "
-                      f"(C, OpenMP dynamic, -O3) with the same integrator configuration as the GPU path "
-                      f"(exact Jacobian, scaled and refined solve); the reference configuration "
-                      f"(difference-quotient Jacobian, plain LU) stalls on these inputs and is not timed"}
+                      f"(C, OpenMP dynamic, -O3) with the GPU path's integrator configuration ({how}); "
+                      f"the reference configuration (difference-quotient Jacobian, plain dense LU) "
+                      f"stalls on these inputs at atol 1e-30 and is not timed"}
 
 
 def main():
@@ -68,12 +95,15 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--npts", type=int, default=1_000_000)
+    ap.add_argument("--network", choices=["reduced", "full", "mynetwork"], default="reduced")
+    ap.add_argument("--npts", type=int, default=None, help="default 1e6 (reduced), 2e4 (others)")
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--check", type=int, default=64, help="points checked against the oracle")
     a = ap.parse_args()
+    if a.npts is None:
+        a.npts = 1_000_000 if a.network == "reduced" else 20_000
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
               f"(grid points are independent; see DESIGN.md for how the path shards)", file=sys.stderr)
@@ -84,10 +114,11 @@ def main():
     from pdrchem.netgen import load_network
 
     dev = torch.device("cuda:0")
-    net = load_network("reduced")
-    nid = lib.network_id("reduced")
+    name = a.network
+    net = load_network(name)
+    nid = lib.network_id(name)
     prec = lib.MIXED if a.precision == "mixed" else lib.FP64
-    pts = workloads.random_points(a.npts)
+    pts = workloads.random_points(a.npts, columns=(name != "reduced"))
     x0_np = workloads.initial_abundances(net, a.npts)
     T = torch.tensor(pts.T, device=dev)
     Td = torch.tensor(pts.Td, device=dev)
@@ -134,9 +165,18 @@ def main():
     st_np = stats.cpu().numpy()
     nfail = int((status != 0).sum().item())
     f_rhs, f_jac = network_flops(net)
-    W = algorithmic_flops(st_np, net.neq, f_rhs, f_jac)
+    w64, wlu = algorithmic_flops(st_np, net.neq, f_rhs, f_jac)
     kt = float(np.mean(ktimes))
-    achieved = W / kt / 1e12
+    # roofline of the solver kernel: time the work needs at peak, FP32 for the
+    # factorisation in the mixed form, FP64 for everything else
+    t_roof = w64 / (FP64_PEAK_TFLOPS * 1e12) + wlu / ((FP32_PEAK_TFLOPS if prec == lib.MIXED else
+                                                       FP64_PEAK_TFLOPS) * 1e12)
+    achieved = (w64 + wlu) / kt / 1e12
+    peak = (w64 + wlu) / t_roof / 1e12
+    kname = ("bdf_wave_kernel" if name == "reduced" else "bdf_wg_kernel") + f"<{name},{a.precision}>"
+    wl = {"reduced": "reduced network (33 species/329 reactions)",
+          "full": "full network (215 species/2926 reactions), N(H2), N(CO) ~ logU[1e14,1e22]",
+          "mynetwork": "MYNETWORK (58 species/634 reactions)"}[name]
     res = {
         "metric": "grid-point chemistry solves/s at <=1e-4 rel err vs FP64 CVODE; p50/p99 batch ms",
         "value": a.npts / float(np.mean(times)),
@@ -148,30 +188,32 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64" if prec == lib.FP64 else "f64+f32-matrix",
         "data": "synthetic (seeded counter-based RNG)",
-        "config": {"workload": "reduced network (33 species/329 reactions), random points "
-                               "log10 nH~U[2,6], log10 T~U[1,3], Av~U[0,20], log10 G0~U[0,5], to 1e7 yr",
+        "config": {"workload": wl + ", random points log10 nH~U[2,6], log10 T~U[1,3], Av~U[0,20], "
+                               "log10 G0~U[0,5], to 1e7 yr",
                    "points": a.npts, "precision": a.precision, "rtol": 1e-8, "atol": 1e-30},
         "failed_points": nfail,
         "mean_steps_per_point": float(st_np[:, 0].mean()),
         "solver_kernel_ms": kt * 1e3,
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-                     "note": "FP64 peak; work W from the run's integrator counters"},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                     "frac": achieved / peak, "traffic": measured_traffic(kname, a.npts),
+                     "kernel": kname,
+                     "note": "work W of CVODE's algorithm (dense LU) for this run's integrator counters; "
+                             "peak blends FP64 78.6 and (mixed form, factorisation) FP32 157.3 TFLOP/s"},
     }
     if a.check > 0:
         import oracle
-        o = oracle.Oracle("reduced")
+        o = oracle.Oracle(name)
         k = min(a.check, a.npts)
         got = ab[:k].cpu().numpy()
         _, ref, _ = o.calculate_abundances(x0_np[:k].copy(), rate[:k].cpu().numpy(), pts.n_H[:k], pts.T[:k],
-                                           mode=oracle.MODE_GPU, nthreads=min(16, os.cpu_count() or 1))
+                                           mode=oracle_mode(name), nthreads=min(16, os.cpu_count() or 1))
         m = ref[:, :-1] > 1e-20
         res["max_rel_err_vs_oracle"] = float(np.max(np.abs(got[:, :-1][m] - ref[:, :-1][m]) / ref[:, :-1][m]))
         res["checked_points"] = k
     if not a.no_cpu:
         thr = min(16, os.cpu_count() or 1)
         k = min(a.npts, 200000)
-        res["cpu_baseline"] = cpu_baseline(net, pts.subset(np.arange(k)), x0_np[:k], rate[:k].cpu().numpy(),
+        res["cpu_baseline"] = cpu_baseline(name, pts.subset(np.arange(k)), x0_np[:k], rate[:k].cpu().numpy(),
                                            a.cpu_seconds, thr)
     print(json.dumps(res))
 

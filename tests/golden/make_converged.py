@@ -1,10 +1,10 @@
 """Converged reference solutions for GPU parity on the seeded random-point
-workload (reduced network): the CPU oracle at the matched tolerances
-(rtol 1e-8, atol 1e-30, GPU-equivalent integrator configuration) and the
+workloads: the CPU oracle at the matched tolerances (rtol 1e-8, atol 1e-30,
+in the integrator configuration of the GPU kernel for that network) and the
 same oracle at rtol 1e-11 / atol 1e-33 as the converged solution.
 
-Writes tests/golden/converged_reduced.npz.  Run in the build container:
-    python tests/golden/make_converged.py
+Writes tests/golden/converged_<network>.npz.  Run in the build container:
+    python tests/golden/make_converged.py [reduced|full|mynetwork ...]
 """
 import os
 import sys
@@ -19,22 +19,29 @@ import oracle  # noqa: E402
 from pdrchem import workloads  # noqa: E402
 from pdrchem.netgen import load_network  # noqa: E402
 
+# points per network; the full and MYNETWORK sets carry column densities
+CASES = {"reduced": (96, oracle.MODE_GPU, False),
+         "full": (32, oracle.MODE_GPU_WG, True),
+         "mynetwork": (64, oracle.MODE_GPU_WG, True)}
 
-def main(n=96):
-    net = load_network("reduced")
-    o = oracle.Oracle("reduced")
-    P = workloads.random_points(n)
+
+def main(name):
+    n, mode, columns = CASES[name]
+    net = load_network(name)
+    o = oracle.Oracle(name)
+    P = workloads.random_points(n, columns=columns)
     x0 = workloads.initial_abundances(net, P.n)
     r = o.rates(P.T, P.Td, P.G, P.Av, P.cols)
-    st, ref, stats = o.calculate_abundances(x0.copy(), r, P.n_H, P.T, mode=oracle.MODE_GPU, nthreads=8)
+    st, ref, stats = o.calculate_abundances(x0.copy(), r, P.n_H, P.T, mode=mode, nthreads=8)
     assert st == 0
     st, conv, cst = o.calculate_abundances(x0.copy(), r, P.n_H, P.T, rtol=1e-11, atol=1e-33,
-                                           mode=oracle.MODE_GPU, nthreads=8)
+                                           mode=mode, nthreads=8)
     assert st == 0
-    np.savez_compressed(os.path.join(HERE, "converged_reduced.npz"), rates=r, n_H=P.n_H, T=P.T, x0=x0,
+    np.savez_compressed(os.path.join(HERE, f"converged_{name}.npz"), rates=r, n_H=P.n_H, T=P.T, x0=x0,
                         oracle=ref, converged=conv, oracle_nst=stats["nst"], converged_nst=cst["nst"])
-    print("done", stats["nst"].mean(), cst["nst"].mean())
+    print(name, "done", stats["nst"].mean(), cst["nst"].mean())
 
 
 if __name__ == "__main__":
-    main()
+    for nm in (sys.argv[1:] or ["reduced"]):
+        main(nm)

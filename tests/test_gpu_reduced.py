@@ -4,7 +4,7 @@ libpdrchem.so on an MI355X).
 The bar (FP64 CVODE oracle vs the HIP solver, same inputs): max relative
 error <= 1e-4 over species with abundance > 1e-20 per H nucleus.  The oracle
 runs the same integrator restatement in its GPU-equivalent mode (exact
-Jacobian, scaled and refined linear solve); the reference configuration
+Jacobian, error-weight-scaled Gauss-Jordan inverse); the reference configuration
 (difference-quotient Jacobian, plain LU) is compared through the Radau
 fixtures, which agree with it to ~1e-9 where it finishes.
 """
