@@ -35,21 +35,35 @@ constexpr int WAVES = 4;  // wavefronts per block
 #define PDR_WAVES_PER_EU 2
 #endif
 
-// per-wavefront shared memory
+// The saved Jacobian is read only when the iteration matrix is formed; the
+// FP32 form keeps it in LDS (registers are the scarce resource at two
+// wavefronts per SIMD), the FP64 form in registers (16 more KB of LDS per
+// block would halve its occupancy).
+template <typename MT>
+constexpr bool kJinLDS = sizeof(MT) == 4;
+
+// per-wavefront shared memory.  Lane-private arrays that are read once per
+// RHS or per setup (rate constants, the saved Jacobian) live here, indexed
+// [k*64 + lane] (conflict-free), rather than in registers.
+template <typename MT>
 struct WaveLDS {
   double X[64];                    // factor vector: y, x_e, 1
   double F[2 * NET::NSLOT + 1];    // fluxes (RHS) or flux partials (Jacobian); last = 0
   double P[65];                    // RHS lane partials; P[64] = 0
   double V[64];                    // vector exchange
   double V2[64];
-  double C[32];                    // pivot-column exchange
+  double Kp[NET::ROUNDS * 64];     // rate * (n_H) of each lane's flux slots
+  MT J[kJinLDS<MT> ? 16 * 64 : 1]; // saved Jacobian (FP32 form): lane's 16 columns
   BdfU u;
 };
 
-// per-block shared memory: the RHS term table is the same for every wave
+// per-block shared memory: the RHS term table and the slot factors are the
+// same for every wave
+template <typename MT>
 struct BlockLDS {
   uint32_t rterm[NET::TM * 64];
-  WaveLDS w[WAVES];
+  uint32_t fab[NET::ROUNDS * 64];
+  WaveLDS<MT> w[WAVES];
 };
 
 struct Lane {
@@ -59,10 +73,10 @@ struct Lane {
 };
 
 struct Tables {
-  uint32_t fab[NET::ROUNDS];
   int rbase, rnl;
   double q;          // charge of row i
-  const uint32_t LDSA *rterm;  // block LDS copy
+  const uint32_t LDSA *rterm;  // block LDS copies
+  const uint32_t LDSA *fab;
 };
 
 // per-lane integrator vectors and matrices (registers)
@@ -70,7 +84,8 @@ template <typename MT>
 struct Bdf {
   double zn[QMAX + 1];
   double ewt, acor, y, ftemp, wsc;
-  MT J[16], A[16];
+  MT A[16];
+  MT J[kJinLDS<MT> ? 1 : 16];     // saved Jacobian (FP64 form)
   int invp;
 };
 
@@ -79,8 +94,7 @@ template <typename MT>
 struct Ctx {
   Lane ln;
   Tables tb;
-  double Kp[NET::ROUNDS];
-  WaveLDS LDSA *S;
+  WaveLDS<MT> LDSA *S;
   double rtol, atol;
 
   __device__ __forceinline__ bool lead() const { return ln.L == 0; }
@@ -99,7 +113,7 @@ struct Ctx {
 template <typename MT>
 __device__ __forceinline__ double wave_rhs(Ctx<MT> &c, double y, double &xe_out) {
   const Lane &ln = c.ln;
-  WaveLDS LDSA &S = *c.S;
+  WaveLDS<MT> LDSA &S = *c.S;
   if (ln.owner) S.X[ln.i] = y;
   double xe = wave_sum(ln.owner ? c.tb.q * y : 0.0);
   xe_out = xe;
@@ -107,8 +121,8 @@ __device__ __forceinline__ double wave_rhs(Ctx<MT> &c, double y, double &xe_out)
   wsync();
 #pragma unroll
   for (int r = 0; r < NET::ROUNDS; r++) {
-    uint32_t f = c.tb.fab[r];
-    S.F[r * 64 + ln.L] = c.Kp[r] * S.X[f & 0xff] * S.X[(f >> 8) & 0xff];
+    const uint32_t f = c.tb.fab[r * 64 + ln.L];
+    S.F[r * 64 + ln.L] = S.Kp[r * 64 + ln.L] * S.X[f & 0xff] * S.X[(f >> 8) & 0xff];
   }
   wsync();
   double acc = 0.0;
@@ -131,15 +145,16 @@ __device__ __forceinline__ double wave_rhs(Ctx<MT> &c, double y, double &xe_out)
 // hh*16.., including the x_e chain term g_i * charge_j
 // ---------------------------------------------------------------------
 template <typename MT>
-__device__ __forceinline__ void jacobian(Ctx<MT> &c, MT (&Jl)[16]) {
+__device__ __forceinline__ void jacobian(Ctx<MT> &c, Bdf<MT> &b) {
   const Lane &ln = c.ln;
-  WaveLDS LDSA &S = *c.S;
+  WaveLDS<MT> LDSA &S = *c.S;
 #pragma unroll
   for (int r = 0; r < NET::ROUNDS; r++) {
-    uint32_t f = c.tb.fab[r];
-    int s = r * 64 + ln.L;
-    S.F[2 * s] = c.Kp[r] * S.X[(f >> 8) & 0xff];   // d flux / d X[a]
-    S.F[2 * s + 1] = c.Kp[r] * S.X[f & 0xff];      // d flux / d X[b]
+    const int s = r * 64 + ln.L;
+    const uint32_t f = c.tb.fab[s];
+    const double kp = S.Kp[s];
+    S.F[2 * s] = kp * S.X[(f >> 8) & 0xff];   // d flux / d X[a]
+    S.F[2 * s + 1] = kp * S.X[f & 0xff];      // d flux / d X[b]
   }
   wsync();
   double acc[17];
@@ -156,7 +171,11 @@ __device__ __forceinline__ void jacobian(Ctx<MT> &c, MT (&Jl)[16]) {
     acc[j] = a;
   }
 #pragma unroll
-  for (int j = 0; j < 16; j++) Jl[j] = (MT)fma(acc[16], (double)NET::charge[ln.hh * 16 + j], acc[j]);
+  for (int j = 0; j < 16; j++) {
+    const MT v = (MT)fma(acc[16], (double)NET::charge[ln.hh * 16 + j], acc[j]);
+    if constexpr (kJinLDS<MT>) S.J[j * 64 + ln.L] = v;
+    else b.J[j] = v;
+  }
   wsync();
 }
 
@@ -202,7 +221,7 @@ __device__ __forceinline__ bool gj_invert(const Lane &ln, MT (&A)[16], int &invp
 
 // x = M^-1 b with the pivot-ordered in-place inverse
 template <typename MT>
-__device__ __forceinline__ double inv_apply(const Lane &ln, WaveLDS LDSA &S, const MT (&A)[16], int invp, double b) {
+__device__ __forceinline__ double inv_apply(const Lane &ln, WaveLDS<MT> LDSA &S, const MT (&A)[16], int invp, double b) {
   if (ln.hh == 0) S.V[invp] = ln.row_ok ? b : 0.0;
   wsync();
   double acc = 0.0;
@@ -220,7 +239,7 @@ __device__ __forceinline__ double inv_apply(const Lane &ln, WaveLDS LDSA &S, con
 template <typename MT>
 __device__ __forceinline__ int lsetup(Ctx<MT> &c, BdfU LDSA &u, Bdf<MT> &b, int convfail) {
   const Lane &ln = c.ln;
-  WaveLDS LDSA &S = *c.S;
+  WaveLDS<MT> LDSA &S = *c.S;
   double dgamma = fabs(u.gamma / u.gammap - 1.0);
   bool jbad = (u.nst == 0) || (u.nst > u.nstlj + MSBJ) ||
               ((convfail == CV_FAIL_BAD_J) && (dgamma < DGMAX_J)) || (convfail == CV_FAIL_OTHER);
@@ -236,7 +255,7 @@ __device__ __forceinline__ int lsetup(Ctx<MT> &c, BdfU LDSA &u, Bdf<MT> &b, int 
     double xe = wave_sum(ln.owner ? c.tb.q * b.zn[0] : 0.0);
     if (ln.L == 0) S.X[NET::NEQ] = xe;
     wsync();
-    jacobian(c, b.J);
+    jacobian(c, b);
   } else {
     if (ln.L == 0) u.jcur = 0;
   }
@@ -249,7 +268,10 @@ __device__ __forceinline__ int lsetup(Ctx<MT> &c, BdfU LDSA &u, Bdf<MT> &b, int 
   for (int j = 0; j < 16; j++) {
     const int col = ln.hh * 16 + j;
     const double wj = S.V[col];
-    double m = -gamma * (double)b.J[j];
+    double jv;
+    if constexpr (kJinLDS<MT>) jv = (double)S.J[j * 64 + ln.L];
+    else jv = (double)b.J[j];
+    double m = -gamma * jv;
     if (col == ln.i) m += 1.0;
     if (!ln.row_ok) m = (col == ln.i) ? 1.0 : 0.0;
     b.A[j] = (MT)(m * (b.wsc / wj));
@@ -295,9 +317,9 @@ __device__ __forceinline__ void solve_point(Ctx<MT> &c, const PointArgs &a, int 
   for (int r = 0; r < NET::ROUNDS; r++) {
     int s = r * 64 + ln.L;
     int re = NET::slot_reac[s];
-    uint32_t f = c.tb.fab[r];
-    double k = (re >= 0) ? rate[re] : 0.0;
-    c.Kp[r] = ((f >> 16) & 1) ? k * nH : k;
+    const uint32_t f = c.tb.fab[s];
+    const double k = (re >= 0) ? rate[re] : 0.0;
+    c.S->Kp[s] = ((f >> 16) & 1) ? k * nH : k;
   }
   double *ab = a.abundance + (size_t)sys * a.ab_stride;
   const double y0 = ln.row_ok ? ab[ln.i] : 0.0;
@@ -315,7 +337,11 @@ __device__ __forceinline__ void solve_point(Ctx<MT> &c, const PointArgs &a, int 
   for (int j = 1; j <= QMAX; j++) b.zn[j] = 0.0;
   b.acor = 0.0; b.y = y0; b.ftemp = 0.0; b.wsc = 1.0; b.invp = ln.i;
 #pragma unroll
-  for (int j = 0; j < 16; j++) { b.J[j] = (MT)0; b.A[j] = (MT)0; }
+  for (int j = 0; j < 16; j++) b.A[j] = (MT)0;
+  if constexpr (!kJinLDS<MT>) {
+#pragma unroll
+    for (int j = 0; j < 16; j++) b.J[j] = (MT)0;
+  }
   b.ewt = 1.0 / (c.rtol * fabs(y0) + c.atol);
 
   double yout = y0;
@@ -344,8 +370,9 @@ __device__ __forceinline__ void solve_point(Ctx<MT> &c, const PointArgs &a, int 
 // persistent kernel: every wavefront pulls grid points from a queue
 template <typename MT>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PDR_WAVES_PER_EU))) bdf_wave_kernel(PointArgs a) {
-  __shared__ BlockLDS blk;
+  __shared__ BlockLDS<MT> blk;
   for (int k = threadIdx.x; k < NET::TM * 64; k += blockDim.x) blk.rterm[k] = NET::rhs_term[k];
+  for (int k = threadIdx.x; k < NET::ROUNDS * 64; k += blockDim.x) blk.fab[k] = NET::slot_fab[k];
   __syncthreads();
   Ctx<MT> c;
   c.ln.L = lane_id();
@@ -353,15 +380,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PDR_WA
   c.ln.hh = c.ln.L >> 5;
   c.ln.row_ok = c.ln.i < NET::NEQ;
   c.ln.owner = c.ln.row_ok && c.ln.hh == 0;
-  c.S = (WaveLDS LDSA *)&blk.w[threadIdx.x >> 6];
+  c.S = (WaveLDS<MT> LDSA *)&blk.w[threadIdx.x >> 6];
   c.rtol = a.rtol;
   c.atol = a.atol;
-#pragma unroll
-  for (int r = 0; r < NET::ROUNDS; r++) c.tb.fab[r] = NET::slot_fab[r * 64 + c.ln.L];
   c.tb.rbase = NET::row_base[c.ln.L];
   c.tb.rnl = NET::row_nlanes[c.ln.L];
   c.tb.q = (double)NET::charge[c.ln.i];
   c.tb.rterm = (const uint32_t LDSA *)blk.rterm;
+  c.tb.fab = (const uint32_t LDSA *)blk.fab;
   for (;;) {
     unsigned int sys = 0;
     if (c.ln.L == 0) sys = atomicAdd(a.counter, 1u);

@@ -69,7 +69,6 @@ struct WgLDS {
   double P[NET::NV + 1];     // RHS chunk partials (P[NV] = 0), tail gather partials
   double V[64], V2[64];      // tail inverse exchange
   double red[2][NWAVE];      // team reductions (double-buffered)
-  MT dinv[NET::NB];          // reciprocals of the head pivots
   int flag;
   BdfU u;
   // Hv[NHV] | zero | Tl[NT*NT] over F[NSLOT+1];  D[2*NSLOT+1] over all of it
@@ -204,7 +203,7 @@ __device__ __forceinline__ void wg_jacobian(Team<MT> &c) {
   __syncthreads();
   const uint32_t *jt = opaque(NET::jac_term);
   double acc = 0.0;
-#pragma unroll 4
+#pragma unroll 8
   for (int t = 0; t < NET::JTM; t++) {
     const uint32_t w = jt[t * BLOCK + c.tid];
     const int coef = (int)(w << 15) >> 28;          // signed 4-bit field at bit 13
@@ -369,7 +368,7 @@ __device__ __forceinline__ int wg_setup(Team<MT> &c, BdfU LDSA &u, BdfWG<MT> &b,
   if (c.tid == 0) S.W[NET::NEQ] = 1.0 / (c.rtol * fabs(xe) + c.atol);
   __syncthreads();
   const uint32_t *mp = opaque(NET::mpos);
-#pragma unroll 2
+#pragma unroll 4
   for (int t = 0; t < NET::MTM; t++) {
     const int p = t * BLOCK + c.tid;
     const uint32_t d = mp[p];
@@ -378,6 +377,8 @@ __device__ __forceinline__ int wg_setup(Team<MT> &c, BdfU LDSA &u, BdfWG<MT> &b,
     double m = 0.0;
     if (row == 0xfe) {
       m = 0.0;  // the zero slot
+    } else if (row == 0xfd) {
+      m = 1.0;  // the one slot
     } else if (row == NET::NEQ) {
       m = (col == NET::NEQ) ? 1.0 : -(double)NET::charge[col];
       m *= S.W[row] / S.W[col];
@@ -392,20 +393,20 @@ __device__ __forceinline__ int wg_setup(Team<MT> &c, BdfU LDSA &u, BdfWG<MT> &b,
   // head elimination, level by level
   for (int lv = 0; lv < NET::NLEV; lv++) {
     const int offA = NET::elimA_off[lv], nA = NET::elimA_n[lv];
-#pragma unroll 1
+#pragma unroll 4
     for (int t = 0; t < nA; t++) {
       const uint32_t w = opaque(NET::elimA)[(offA + t) * BLOCK + c.tid];
       const int kind = w >> 28, tgt = w & 0x3fff, dp = (w >> 14) & 0x3fff;
       if (kind == 1) {
-        LU[tgt] = LU[tgt] / LU[dp];
+        LU[tgt] = LU[tgt] / LU[dp];       // multiplier
       } else if (kind == 2) {
-        S.dinv[tgt] = (MT)1.0 / LU[dp];
+        LU[tgt] = (MT)1.0 / LU[dp];       // diagonal of the head's U^-1
       }
     }
     __syncthreads();
     const int offB = NET::elimB_off[lv], nB = NET::elimB_n[lv];
     MT acc = (MT)0.0;
-#pragma unroll 2
+#pragma unroll 8
     for (int t = 0; t < nB; t++) {
       const uint64_t w = opaque(NET::elimB)[(size_t)(offB + t) * BLOCK + c.tid];
       const int tgt = w & 0x3fff, lp = (w >> 14) & 0x3fff, up = (w >> 28) & 0x3fff;
@@ -418,6 +419,22 @@ __device__ __forceinline__ int wg_setup(Team<MT> &c, BdfU LDSA &u, BdfWG<MT> &b,
     __syncthreads();
   }
   if constexpr (NET::NHEAD > 0) {
+    // explicit inverses of the head triangles, level by level:
+    // target <- -(sum of src1*src2) * LU[scale]
+    for (int lv = 0; lv < NET::NINV; lv++) {
+      const int off = NET::inv_off[lv], n = NET::inv_n[lv];
+      MT acc = (MT)0.0;
+#pragma unroll 8
+      for (int t = 0; t < n; t++) {
+        const uint64_t w = opaque(NET::inv_tab)[(size_t)(off + t) * BLOCK + c.tid];
+        acc = fma(LU[w & 0x3fff], LU[(w >> 14) & 0x3fff], acc);
+        if ((w >> 56) & 1) {
+          LU[(w >> 28) & 0x3fff] = -acc * LU[(w >> 42) & 0x3fff];
+          acc = (MT)0.0;
+        }
+      }
+      __syncthreads();
+    }
     schur_mfma(c, LU);
     __syncthreads();
   }
@@ -445,6 +462,29 @@ __device__ __forceinline__ int wg_setup(Team<MT> &c, BdfU LDSA &u, BdfWG<MT> &b,
   return fl;
 }
 
+// one chunk of a sparse mat-vec per lane: P[tid] = sum of LU[pos] * B[col]
+// over the lane's entries (pos | col << 14)
+template <typename MT>
+__device__ __forceinline__ void spmv_partials(Team<MT> &c, const uint32_t *tab_, int rounds,
+                                              const MT LDSA *LU) {
+  const uint32_t *tab = opaque(tab_);
+  double acc = 0.0;
+#pragma unroll 8
+  for (int t = 0; t < rounds; t++) {
+    const uint32_t w = tab[t * BLOCK + c.tid];
+    acc = fma((double)LU[w & 0x3fff], c.S->B[w >> 14], acc);
+  }
+  c.S->P[c.tid] = acc;
+}
+
+// sum of a row's chunk partials P[base .. base+cnt), in a fixed order
+template <typename MT>
+__device__ __forceinline__ double row_total(const WgLDS<MT> LDSA &S, int base, int cnt) {
+  double v = 0.0;
+  for (int u = 0; u < cnt; u++) v += S.P[base + u];
+  return v;
+}
+
 // ---------------------------------------------------------------------
 // solve M dx = r: returns dx of row tid (0 beyond NEQ), with CVODE's
 // 2/(1+gamrat) correction for a changed gamma
@@ -457,30 +497,14 @@ __device__ __forceinline__ double wg_solve(Team<MT> &c, const BdfU LDSA &u, cons
   if (c.tid < NET::NB) S.B[c.tid] = c.own_ ? r * b.wsc : 0.0;
   if (c.tid == 0) S.B[NET::NB] = 0.0;
   __syncthreads();
-  // forward substitution over the head
-  for (int lv = 0; lv < NET::NFWD; lv++) {
-    const int off = NET::fwd_off[lv], n = NET::fwd_n[lv];
-    double acc = 0.0;
-#pragma unroll 2
-    for (int t = 0; t < n; t++) {
-      const uint32_t w = opaque(NET::fwd_tab)[(off + t) * BLOCK + c.tid];
-      acc = fma((double)LU[(w >> 8) & 0x3fff], S.B[(w >> 22) & 0xff], acc);
-      if (w >> 30) {
-        S.B[w & 0xff] -= acc;
-        acc = 0.0;
-      }
-    }
-    __syncthreads();
-  }
-  // tail rows gather the head values (wave w: pivots w*KH/4 ..)
   if constexpr (NET::NHEAD > 0) {
-    constexpr int KQ = NET::KH / 4;
-    const uint16_t *lrow = opaque(NET::lth_pos) + c.lane * NET::KH + c.wave * KQ;
-    const uint8_t *hrow = opaque(NET::head_row) + c.wave * KQ;
-    double acc = 0.0;
-#pragma unroll 4
-    for (int k = 0; k < KQ; k++) acc = fma((double)LU[lrow[k]], S.B[hrow[k]], acc);
-    S.P[c.wave * 64 + c.lane] = acc;
+    // S1: y_H = b_H + Linv b_H
+    spmv_partials(c, NET::s1_tab, NET::S1_ROUNDS, LU);
+    __syncthreads();
+    if (c.tid < NET::NHEAD) S.B[NET::head_row[c.tid]] += row_total(S, NET::s1_base[c.tid], NET::s1_cnt[c.tid]);
+    __syncthreads();
+    // S2: the tail rows' gather L_TH y_H (combined in the tail step)
+    spmv_partials(c, NET::s2_tab, NET::S2_ROUNDS, LU);
     __syncthreads();
   }
   // tail: x_T = S^-1 b_T; the permuted right-hand side from wavefront 0,
@@ -489,7 +513,7 @@ __device__ __forceinline__ double wg_solve(Team<MT> &c, const BdfU LDSA &u, cons
     double bt = 0.0;
     if (c.lane < NET::NT) {
       bt = S.B[NET::tail_row[c.lane]];
-      if constexpr (NET::NHEAD > 0) bt -= (S.P[c.lane] + S.P[64 + c.lane]) + (S.P[128 + c.lane] + S.P[192 + c.lane]);
+      if constexpr (NET::NHEAD > 0) bt -= row_total(S, NET::s2_base[c.lane], NET::s2_cnt[c.lane]);
     }
     S.V[b.invp] = bt;
   }
@@ -508,31 +532,16 @@ __device__ __forceinline__ double wg_solve(Team<MT> &c, const BdfU LDSA &u, cons
     if (c.lane < NET::NT) S.B[NET::tail_row[c.lane]] = S.V2[c.lane];
   }
   __syncthreads();
-  // head rows scatter the tail values, then back substitution over the head
   if constexpr (NET::NHEAD > 0) {
-    if (c.tid < NET::NHEAD) {
-      const uint16_t *ucol = opaque(NET::uht_pos) + c.tid * 64;
-      const uint8_t *trow = opaque(NET::tail_row);
-      double acc = 0.0;
-#pragma unroll 8
-      for (int j = 0; j < NET::NT; j++) acc = fma((double)LU[ucol[j]], S.B[trow[j]], acc);
-      S.B[NET::head_row[c.tid]] -= acc;
-    }
+    // S4: z_H = y_H - U_HT x_T
+    spmv_partials(c, NET::s4_tab, NET::S4_ROUNDS, LU);
     __syncthreads();
-  }
-  for (int lv = 0; lv < NET::NBWD; lv++) {
-    const int off = NET::bwd_off[lv], n = NET::bwd_n[lv];
-    double acc = 0.0;
-#pragma unroll 2
-    for (int t = 0; t < n; t++) {
-      const uint32_t w = opaque(NET::bwd_tab)[(off + t) * BLOCK + c.tid];
-      acc = fma((double)LU[(w >> 8) & 0x3fff], S.B[(w >> 22) & 0xff], acc);
-      if (w >> 30) {
-        const int k = w & 0xff;
-        S.B[k] = (S.B[k] - acc) * (double)S.dinv[k];
-        acc = 0.0;
-      }
-    }
+    if (c.tid < NET::NHEAD) S.B[NET::head_row[c.tid]] -= row_total(S, NET::s4_base[c.tid], NET::s4_cnt[c.tid]);
+    __syncthreads();
+    // S5: x_H = Winv z_H
+    spmv_partials(c, NET::s5_tab, NET::S5_ROUNDS, LU);
+    __syncthreads();
+    if (c.tid < NET::NHEAD) S.B[NET::head_row[c.tid]] = row_total(S, NET::s5_base[c.tid], NET::s5_cnt[c.tid]);
     __syncthreads();
   }
   double x = c.own_ ? S.B[c.tid] / b.wsc : 0.0;

@@ -195,9 +195,38 @@ class WgTables:
             pos_of[i, j] = p
         self.nhv = len(hv)
         self.luzero = self.nhv                 # LU[NHV] = 0
-        # tail block Tl[NT][NT] after the zero slot, 16-byte aligned: it lies
-        # over the flux array F in LDS, which is FP64
-        self.tbase = 4 * math.ceil((self.nhv + 1) / 4)
+        self.luone = self.nhv + 1              # LU[NHV+1] = 1
+        # explicit inverses of the head triangles (solves without levels):
+        # Linv = strict lower part of L_HH^-1, Winv = U_HH^-1 (with diagonal)
+        h = an.nhead
+        hpos = {v: a for a, v in enumerate(head)}
+        lrow = [[b for b in range(a) if F[head[a], head[b]]] for a in range(h)]
+        urow = [[b for b in range(a + 1, h) if F[head[a], head[b]]] for a in range(h)]
+        li_pat = [set() for _ in range(h)]
+        for a in range(h):
+            for b in lrow[a]:
+                li_pat[a].add(b)
+                li_pat[a] |= li_pat[b]
+        wi_pat = [set() for _ in range(h)]
+        for a in range(h - 1, -1, -1):
+            wi_pat[a].add(a)
+            for b in urow[a]:
+                wi_pat[a] |= wi_pat[b]
+        self.hi_base = self.nhv + 2
+        li_idx, wi_idx = {}, {}
+        k = self.hi_base
+        for a in range(h):
+            for cc in sorted(li_pat[a]):
+                li_idx[(a, cc)] = k
+                k += 1
+        for a in range(h):
+            for cc in sorted(wi_pat[a]):
+                wi_idx[(a, cc)] = k
+                k += 1
+        self.nhi = k - self.hi_base
+        # tail block Tl[NT][NT] after these, 16-byte aligned: it lies over the
+        # flux array F in LDS, which is FP64
+        self.tbase = 4 * math.ceil(k / 4)
         # row stride NT + 1: a column of the block falls in distinct LDS banks
         self.tstride = self.nt + 1
         tidx = {v: r for r, v in enumerate(tail)}
@@ -209,12 +238,13 @@ class WgTables:
             raise ValueError("LU storage exceeds the position field")
         self.tail = tail
         self.head = head
-        allpos = hv + [None] + ["pad"] * (self.tbase - self.nhv - 1)
+        allpos = hv + [None, "one"] + ["pad"] * (self.tbase - self.nhv - 2)
         for i in tail:
             allpos += [(i, j) for j in tail] + ["pad"]
         assert len(allpos) == self.nlu
         # M entry descriptors in position order: row | col << 8 | src << 16
-        # (src 0xffff: no J term); row 0xfe: the zero slot; row 0xff: padding
+        # (src 0xffff: no J term); row 0xfe: the zero slot, 0xfd: the one slot,
+        # 0xff: padding (and the inverse block, filled later in the setup)
         self.mpos = []
         for e in allpos:
             if e is None:
@@ -222,6 +252,9 @@ class WgTables:
                 continue
             if e == "pad":
                 self.mpos.append(0xFF)
+                continue
+            if e == "one":
+                self.mpos.append(0xFD)
                 continue
             i, j = e
             src = jid.get((i, j), 0xFFFF) if i < n else 0xFFFF
@@ -241,7 +274,7 @@ class WgTables:
             upd: Dict[int, List[Tuple[int, int]]] = {}
             for k in piv:
                 dk = int(pos_of[k, k])
-                itemsA.append((1, ('d', k), (2 << 28) | k | (dk << 14)))
+                itemsA.append((1, ('d', k), (2 << 28) | wi_idx[(hpos[k], hpos[k])] | (dk << 14)))
                 rows_l = [i for i in range(NB) if F[i, k] and rank[i] > rank[k]]
                 cols_u = [j for j in range(NB) if F[k, j] and rank[j] > rank[k]]
                 for i in rows_l:
@@ -289,51 +322,106 @@ class WgTables:
                     self.lth[t * self.kh + a] = int(pos_of[i, k])
                 if F[k, i]:
                     self.uht[a * 64 + t] = int(pos_of[k, i])
-        # ------------------------------------- head solve levels (head-head)
+        # ------------------------ explicit head inverses, computed in the setup
+        # Linv[a][c] = -(L[a][c] + sum_b L[a][b] Linv[b][c])  (rows by forward level)
+        # Winv[a][c] = -Winv[a][a] * sum_b U[a][b] Winv[b][c]  (rows by backward level)
+        # term: src1 | src2 << 14 | target << 28 | scale << 42 | last << 56;
+        # target <- -(sum of src1*src2) * LU[scale]
         fl, bl = an.fwd_level, an.bwd_level
-        hset = set(head)
-        self.fwd: List[List[List[int]]] = []
-        nfl = int(fl.max()) + 1 if an.nhead else 0
+        self.inv_levels: List[List[List[int]]] = []
+        nfl = int(fl.max()) + 1 if h else 0
+        nbl = int(bl.max()) + 1 if h else 0
+        lpos = lambda a, b: int(pos_of[head[a], head[b]])
         for lev in range(nfl):
-            per_row = {}
-            for a in range(an.nhead):
+            per_t = {}
+            for a in range(h):
                 if fl[a] != lev:
                     continue
-                i = head[a]
-                ks = [k for k in head if F[i, k] and rank[k] < rank[i]]
-                if ks:
-                    per_row[i] = [(int(pos_of[i, k]), k) for k in ks]
-            self.fwd.append(self._solve_tab(per_row, dummy=False))
-        self.bwd: List[List[List[int]]] = []
-        nbl = int(bl.max()) + 1 if an.nhead else 0
+                for cc in sorted(li_pat[a]):
+                    terms = []
+                    if cc in lrow[a]:
+                        terms.append((lpos(a, cc), self.luone))
+                    for b in lrow[a]:
+                        if cc in li_pat[b]:
+                            terms.append((lpos(a, b), li_idx[(b, cc)]))
+                    per_t[(li_idx[(a, cc)], self.luone)] = terms
+            self.inv_levels.append(self._term_tab(per_t))
         for lev in range(nbl):
-            per_row = {}
-            for a in range(an.nhead):
+            per_t = {}
+            for a in range(h):
                 if bl[a] != lev:
                     continue
-                k = head[a]
-                js = [j for j in head if F[k, j] and rank[j] > rank[k]]
-                per_row[k] = [(int(pos_of[k, j]), j) for j in js] or [(self.luzero, 0)]
-            self.bwd.append(self._solve_tab(per_row, dummy=True))
+                for cc in sorted(wi_pat[a]):
+                    if cc == a:
+                        continue
+                    terms = [(lpos(a, b), wi_idx[(b, cc)]) for b in urow[a] if cc in wi_pat[b]]
+                    per_t[(wi_idx[(a, cc)], wi_idx[(a, a)])] = terms
+            self.inv_levels.append(self._term_tab(per_t))
+        # ------------------------------- solve: four chunked sparse mat-vecs
+        # S1 y_H = b_H + Linv b_H, S2 tail gather L_TH y_H, S4 U_HT x_T,
+        # S5 x_H = Winv z_H; entry: LU position | B index << 14
+        s1 = [[(li_idx[(a, cc)], head[cc]) for cc in sorted(li_pat[a])] for a in range(h)]
+        s2 = [[(int(pos_of[i, head[b]]), head[b]) for b in range(h) if F[i, head[b]]] for i in tail]
+        s4 = [[(int(pos_of[head[a], j]), j) for j in tail if F[head[a], j]] for a in range(h)]
+        s5 = [[(wi_idx[(a, cc)], head[cc]) for cc in sorted(wi_pat[a])] for a in range(h)]
+        self.spmv = [self._spmv_tab(rows) for rows in (s1, s2, s4, s5)]
+        self.n_inv_terms = sum(sum(1 for row in lev for v in row if (v >> 56) & 1) for lev in self.inv_levels)
         self.diag_pos = [int(pos_of[i, i]) for i in range(NB)]
         self.charges = net.charges()
 
-    def _solve_tab(self, per_row, dummy):
-        """(row | pos << 8 | col << 22 | last << 30) lists, LPT over lanes."""
-        bins = _lpt([(len(v), i) for i, v in per_row.items()], self.block)
+    def _term_tab(self, per_t):
+        """Targets with their (src1, src2) terms -> [round][lane] 64-bit items,
+        every target's terms in one lane (LPT over lanes)."""
+        bins = _lpt([(max(1, len(v)), t) for t, v in per_t.items()], self.block)
         lanes = []
         for lane in range(self.block):
             lst = []
-            for i in sorted(bins[lane]):
-                ts = per_row[i]
-                for t, (p, k) in enumerate(ts):
-                    last = 1 if t == len(ts) - 1 else 0
-                    lst.append(i | (p << 8) | (k << 22) | (last << 30))
+            for key in sorted(bins[lane]):
+                tgt, scale = key
+                terms = per_t[key] or [(self.luzero, self.luzero)]
+                for t, (a, b) in enumerate(terms):
+                    last = 1 if t == len(terms) - 1 else 0
+                    lst.append(a | (b << 14) | (tgt << 28) | (scale << 42) | (last << 56))
             lanes.append(lst)
         r = max((len(l) for l in lanes), default=0)
-        pad = self.luzero << 8
+        pad = self.luzero | (self.luzero << 14) | (self.luzero << 28) | (self.luzero << 42)
         return [[(lanes[lane][t] if t < len(lanes[lane]) else pad) for lane in range(self.block)]
                 for t in range(r)]
+
+    def _spmv_tab(self, rows):
+        """Rows of (LU position, B index) entries -> chunks, one per lane:
+        every row gets at least one chunk (also when empty), extra chunks go
+        to the longest rows.  Returns dict(tab=[round][lane] words,
+        base=[row], cnt=[row]) with partials P[lane] of chunk = lane."""
+        nr = len(rows)
+        if nr == 0:
+            return dict(tab=[], base=[], cnt=[], rounds=0)
+        if nr > self.block:
+            raise ValueError("more rows than lanes")
+        nl = [1] * nr
+        heap = [(-len(rows[i]), i) for i in range(nr)]
+        heapq.heapify(heap)
+        used = nr
+        while used < self.block and heap:
+            negl, i = heapq.heappop(heap)
+            if math.ceil(len(rows[i]) / nl[i]) <= 1:
+                break
+            nl[i] += 1
+            used += 1
+            heapq.heappush(heap, (-math.ceil(len(rows[i]) / nl[i]), i))
+        base, b = [], 0
+        for i in range(nr):
+            base.append(b)
+            b += nl[i]
+        rounds = max(math.ceil(len(rows[i]) / nl[i]) for i in range(nr))
+        pad = self.luzero
+        tab = [[pad] * self.block for _ in range(rounds)]
+        for i in range(nr):
+            csz = math.ceil(len(rows[i]) / nl[i]) if rows[i] else 0
+            for u in range(nl[i]):
+                for t, (p, col) in enumerate(rows[i][u * csz:(u + 1) * csz]):
+                    tab[t][base[i] + u] = p | (col << 14)
+        return dict(tab=tab, base=base, cnt=nl, rounds=rounds)
 
     # ------------------------------------------------------------------ emit
     def emit(self, path: str) -> None:
@@ -358,8 +446,9 @@ class WgTables:
         c("NLU", self.nlu); c("MTM", self.mtm, "M-build rounds")
         c("KH", self.kh, "head pivots padded to a multiple of 4")
         c("NHEAD", self.nh_piv); c("NLEV", self.nlev, "head elimination levels")
-        c("NFWD", len(self.fwd), "forward solve levels of the head")
-        c("NBWD", len(self.bwd), "backward solve levels")
+        c("LUONE", self.luone, "LU[LUONE] = 1")
+        c("NHI", self.nhi, "entries of the head inverses Linv, Winv")
+        c("NINV", len(self.inv_levels), "levels of the head-inverse computation")
         c("XLAMBDA_SPLINE", net.xlambda_spline)
 
         def arr(ctype, name, vals, fmt=str, per=16):
@@ -401,15 +490,20 @@ class WgTables:
         L.append("// phase B [round][lane]: target | l pos << 14 | u pos << 28 | last << 42")
         arr("int", "elimB_off", offB); arr("int", "elimB_n", rbl)
         arr("uint64_t", "elimB", flatB, hx64, 6)
-        for nm, levs in (("fwd", self.fwd), ("bwd", self.bwd)):
-            off, cnt, flat = [], [], []
-            for lev in levs:
-                off.append(len(flat) // self.block); cnt.append(len(lev))
-                for row in lev:
-                    flat += row
-            L.append(f"// {nm} solve [round][lane]: row | pos << 8 | col << 22 | last << 30")
-            arr("int", f"{nm}_off", off); arr("int", f"{nm}_n", cnt)
-            arr("uint32_t", f"{nm}_tab", flat, hx, 8)
+        off, cnt, flat = [], [], []
+        for lev in self.inv_levels:
+            off.append(len(flat) // self.block); cnt.append(len(lev))
+            for row in lev:
+                flat += row
+        L.append("// head inverses [round][lane]: src1 | src2 << 14 | target << 28 | scale << 42 | last << 56")
+        arr("int", "inv_off", off); arr("int", "inv_n", cnt)
+        arr("uint64_t", "inv_tab", flat, hx64, 6)
+        for nm, sp in zip(("s1", "s2", "s4", "s5"), self.spmv):
+            L.append(f"// solve stage {nm} [round][lane]: LU position | B index << 14")
+            c(f"{nm.upper()}_ROUNDS", sp["rounds"])
+            arr("uint32_t", f"{nm}_tab", [v for row in sp["tab"] for v in row], hx, 8)
+            arr("uint16_t", f"{nm}_base", sp["base"])
+            arr("uint8_t", f"{nm}_cnt", sp["cnt"])
         arr("uint8_t", "tail_row", list(self.tail) + [0] * (64 - self.nt))
         arr("uint8_t", "head_row", self.head_row)
         L.append("// LU positions of L[tail t][head k] ([t][k]) and U[head k][tail j] ([k][j])")
@@ -429,8 +523,8 @@ class WgTables:
                 f"NJ={self.nj} JTM={self.jtm} NLU={self.nlu} (NHV={self.nhv}, NT={self.nt}, KH={self.kh}) "
                 f"MTM={self.mtm} "
                 f"levels={self.nlev} elimA rounds={ra} elimB rounds={rb} "
-                f"fwd rounds={sum(len(x) for x in self.fwd)}/{len(self.fwd)} "
-                f"bwd rounds={sum(len(x) for x in self.bwd)}/{len(self.bwd)}")
+                f"NHI={self.nhi} inv rounds={sum(len(x) for x in self.inv_levels)}/{len(self.inv_levels)} "
+                f"spmv rounds={[sp['rounds'] for sp in self.spmv]}")
 
 
 def generate(names=("mynetwork", "full"), net_dir=None, out_dir=None) -> List[WgTables]:

@@ -116,24 +116,34 @@ def test_jacobian_tables(case):
     np.testing.assert_allclose(got, Jb, rtol=1e-12, atol=1e-300)
 
 
+def _spmv(tab, sp, LU, B):
+    """Chunk partials of one solve stage: P[lane] = sum over the lane's chunk."""
+    P = np.zeros(tab.block)
+    for rnd in sp["tab"]:
+        for lane, v in enumerate(rnd):
+            P[lane] += LU[v & 0x3FFF] * B[v >> 14]
+    return [P[sp["base"][r]:sp["base"][r] + sp["cnt"][r]].sum() for r in range(len(sp["base"]))]
+
+
 def _factor_and_solve(tab, net, Jb, gamma, w, rhs):
     """The kernel's setup + solve, in the kernel's order, on numpy arrays."""
     n, NB = net.neq, tab.nb
     q = np.asarray(net.charges(), dtype=float)
-    jid = {e: k for k, e in enumerate(tab.jent)}
     Jg = np.array([Jb[i, j] for (i, j) in tab.jent])
-    LU = np.zeros(tab.nlu + 1)
+    LU = np.zeros(tab.nlu)
     for p in range(tab.nlu):
         d = tab.mpos[p]
         row, col, src = d & 0xFF, (d >> 8) & 0xFF, d >> 16
         if row in (0xFF, 0xFE):
+            continue
+        if row == 0xFD:
+            LU[p] = 1.0
             continue
         if row == n:
             m = 1.0 if col == n else -q[col]
         else:
             m = (-gamma * Jg[src] if src != 0xFFFF else 0.0) + (1.0 if row == col else 0.0)
         LU[p] = m * w[row] / w[col]
-    dinv = np.zeros(NB)
     for lv in range(tab.nlev):
         for rnd in tab.elimA[lv]:
             for v in rnd:
@@ -141,7 +151,7 @@ def _factor_and_solve(tab, net, Jb, gamma, w, rhs):
                 if kind == 1:
                     LU[tgt] /= LU[dp]
                 elif kind == 2:
-                    dinv[tgt] = 1.0 / LU[dp]
+                    LU[tgt] = 1.0 / LU[dp]
         acc = np.zeros(tab.block)
         for rnd in tab.elimB[lv]:
             for lane, v in enumerate(rnd):
@@ -150,6 +160,16 @@ def _factor_and_solve(tab, net, Jb, gamma, w, rhs):
                 if (v >> 42) & 1:
                     LU[tgt] -= acc[lane]
                     acc[lane] = 0.0
+    for lev in tab.inv_levels:
+        acc = np.zeros(tab.block)
+        for rnd in lev:
+            for lane, v in enumerate(rnd):
+                s1, s2 = v & 0x3FFF, (v >> 14) & 0x3FFF
+                tgt, sc = (v >> 28) & 0x3FFF, (v >> 42) & 0x3FFF
+                acc[lane] += LU[s1] * LU[s2]
+                if (v >> 56) & 1:
+                    LU[tgt] = -acc[lane] * LU[sc]
+                    acc[lane] = 0.0
     NT, TS, KH = tab.nt, tab.tstride, tab.kh
     T = np.array([[LU[tab.tbase + r * TS + c] for c in range(NT)] for r in range(NT)])
     if KH:
@@ -157,37 +177,29 @@ def _factor_and_solve(tab, net, Jb, gamma, w, rhs):
         Uht = np.array(tab.uht, dtype=np.int64).reshape(KH, 64)
         T = T - LU[Lth][:NT] @ LU[Uht][:, :NT]
     Tinv = np.linalg.inv(T)
-    # solve
+    # solve: S1 forward, S2 gather, tail, S4 scatter, S5 backward
     B = np.zeros(NB + 1)
     B[:n] = rhs * w[:n]
-    for lv in tab.fwd:
-        acc = np.zeros(tab.block)
-        for rnd in lv:
-            for lane, v in enumerate(rnd):
-                acc[lane] += LU[(v >> 8) & 0x3FFF] * B[(v >> 22) & 0xFF]
-                if v >> 30:
-                    B[v & 0xFF] -= acc[lane]
-                    acc[lane] = 0.0
-    bt = B[tab.tail[:NT]].copy() if NT else np.zeros(0)
-    bt = np.array([B[tab.tail[r]] for r in range(NT)])
-    if KH:
-        hr = np.asarray(tab.head_row)
-        bt -= LU[Lth][:NT] @ B[hr]
+    head, tail = tab.head, tab.tail
+    s1, s2, s4, s5 = tab.spmv
+    if tab.nh_piv:
+        p = _spmv(tab, s1, LU, B)
+        for a, v in enumerate(p):
+            B[head[a]] += v
+        p2 = _spmv(tab, s2, LU, B)
+    else:
+        p2 = [0.0] * NT
+    bt = np.array([B[tail[r]] - p2[r] for r in range(NT)])
     xt = Tinv @ bt
     for r in range(NT):
-        B[tab.tail[r]] = xt[r]
-    if KH:
-        for a in range(tab.nh_piv):
-            B[tab.head_row[a]] -= sum(LU[Uht[a, j]] * B[tab.tail[j]] for j in range(NT))
-    for lv in tab.bwd:
-        acc = np.zeros(tab.block)
-        for rnd in lv:
-            for lane, v in enumerate(rnd):
-                acc[lane] += LU[(v >> 8) & 0x3FFF] * B[(v >> 22) & 0xFF]
-                if v >> 30:
-                    k = v & 0xFF
-                    B[k] = (B[k] - acc[lane]) * dinv[k]
-                    acc[lane] = 0.0
+        B[tail[r]] = xt[r]
+    if tab.nh_piv:
+        p = _spmv(tab, s4, LU, B)
+        for a, v in enumerate(p):
+            B[head[a]] -= v
+        p = _spmv(tab, s5, LU, B)
+        for a, v in enumerate(p):
+            B[head[a]] = v
     return B[:n] / w[:n]
 
 
