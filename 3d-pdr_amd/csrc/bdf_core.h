@@ -596,6 +596,7 @@ struct PointArgs {
   const double *density;   // [nsys]
   int nsys, ab_stride, rate_stride;
   double rtol, atol, t0, tend, tout0;
+  const double *tend_pts;  // [nsys] per-point end time, or null: tend for all
   int nelect0;
   long long mxstep;
   int *status;             // [nsys]
@@ -630,7 +631,8 @@ __device__ __forceinline__ void bdf_init_uniform(BdfU LDSA &u, double t0, double
 // ewt); on return yout is the lane's final value and flag the last CVode
 // return; the result is the point's status contribution.
 template <class T, class S>
-__device__ __forceinline__ int integrate(T &c, BdfU LDSA &u, S &b, const PointArgs &a, double &yout, int &flag) {
+__device__ __forceinline__ int integrate(T &c, BdfU LDSA &u, S &b, const PointArgs &a, double tend, double &yout,
+                                         int &flag) {
   double tout = a.tout0;
   double t = a.t0;
   int nfails = 0, status = 0;
@@ -695,7 +697,7 @@ __device__ __forceinline__ int integrate(T &c, BdfU LDSA &u, S &b, const PointAr
       }
     }
     tout = tout * 10;
-    if (tout > a.tend) tout = a.tend;
+    if (tout > tend) tout = tend;
   } while (t < tout && nfails < 5);
   return status;
 }

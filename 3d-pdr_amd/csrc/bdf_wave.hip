@@ -25,6 +25,7 @@ static void fill_args(Args &a, const SolveRequest &r) {
   a.atol = r.atol;
   a.t0 = r.t0;
   a.tend = r.tend;
+  a.tend_pts = r.tend_pts;
   a.tout0 = r.tout0;
   a.nelect0 = r.nelect0;
   a.mxstep = r.mxstep;

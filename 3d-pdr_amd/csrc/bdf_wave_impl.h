@@ -346,7 +346,8 @@ __device__ __forceinline__ void solve_point(Ctx<MT> &c, const PointArgs &a, int 
 
   double yout = y0;
   int flag = 0;
-  const int status = integrate(c, u, b, a, yout, flag);
+  const double tend = a.tend_pts ? a.tend_pts[sys] : a.tend;
+  const int status = integrate(c, u, b, a, tend, yout, flag);
 
   // write-back with clip (calculate_abundances.c:308-317)
   if (ln.owner) {

@@ -3,6 +3,7 @@
 // algorithm and layout.
 #include <hip/hip_runtime.h>
 
+#include <map>
 #include <mutex>
 
 #include "wave_ops.h"
@@ -24,22 +25,29 @@ namespace pdrchem {
 
 namespace {
 
-// Jacobian workspace: one slot of NJ entries per resident workgroup, grown
-// on demand and kept for the life of the process
+// Jacobian workspace: one slot of NJ entries per resident workgroup, one
+// buffer per stream (launches on one stream run in order; launches on
+// different streams may overlap and must not share slots), grown on demand
+// and kept for the life of the process
 struct JacWorkspace {
+  struct Buf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+  };
   std::mutex mu;
-  void *ptr = nullptr;
-  size_t bytes = 0;
-  void *get(size_t need) {
+  std::map<hipStream_t, Buf> bufs;
+  void *get(hipStream_t s, size_t need) {
     std::lock_guard<std::mutex> lock(mu);
-    if (need > bytes) {
-      if (ptr) (void)hipFree(ptr);
-      ptr = nullptr;
-      bytes = 0;
-      if (hipMalloc(&ptr, need) != hipSuccess) return nullptr;
-      bytes = need;
+    Buf &b = bufs[s];
+    if (need > b.bytes) {
+      // the previous buffer may still be in use by work queued on s
+      if (b.ptr && (hipStreamSynchronize(s) != hipSuccess || hipFree(b.ptr) != hipSuccess)) return nullptr;
+      b.ptr = nullptr;
+      b.bytes = 0;
+      if (hipMalloc(&b.ptr, need) != hipSuccess) return nullptr;
+      b.bytes = need;
     }
-    return ptr;
+    return b.ptr;
   }
 };
 JacWorkspace g_jac;
@@ -56,6 +64,7 @@ void fill_args(Args &a, const SolveRequest &r) {
   a.atol = r.atol;
   a.t0 = r.t0;
   a.tend = r.tend;
+  a.tend_pts = r.tend_pts;
   a.tout0 = r.tout0;
   a.nelect0 = r.nelect0;
   a.mxstep = r.mxstep;
@@ -77,7 +86,7 @@ int launch(void (*kern)(Args, MT *), const SolveRequest &r, int nj, hipStream_t 
   long long nb = r.npts < full ? r.npts : full;
   if (r.max_blocks > 0 && nb > r.max_blocks) nb = r.max_blocks;
   if (nb < 1) nb = 1;
-  MT *jac = (MT *)g_jac.get((size_t)nb * nj * sizeof(MT));
+  MT *jac = (MT *)g_jac.get(s, (size_t)nb * nj * sizeof(MT));
   if (!jac) return -1;
   hipLaunchKernelGGL(kern, dim3((unsigned)nb), dim3(256), 0, s, a, jac);
   return hipGetLastError() == hipSuccess ? 0 : -1;

@@ -68,8 +68,10 @@ struct WgLDS {
   double W[NET::NB];         // error weights the matrix was scaled with
   double P[NET::NV + 1];     // RHS chunk partials (P[NV] = 0), tail gather partials
   double V[64], V2[64];      // tail inverse exchange
+  MT gjcol[2][64];           // Gauss-Jordan: pivot column (double-buffered by step)
+  MT gjd[2];                 //   reciprocal of the pivot
+  int gjp[2];                //   pivot row, -1 if singular
   double red[2][NWAVE];      // team reductions (double-buffered)
-  int flag;
   BdfU u;
   // Hv[NHV] | zero | Tl[NT*NT] over F[NSLOT+1];  D[2*NSLOT+1] over all of it
   alignas(16) unsigned char region[RegionBytes<MT>::total];
@@ -279,56 +281,68 @@ __device__ __forceinline__ void schur_mfma(const Team<double> &c, double LDSA *L
 }
 
 // ---------------------------------------------------------------------
-// Gauss-Jordan inverse of the tail block in wavefront 0: lane r holds row r
-// (NT <= 64 columns in registers).  Step k: the pivot is the unused row with
-// the largest |a[r][k]| (ties to the lower row), found by a DPP arg-max; the
-// pivot row reaches every lane by v_readlane.  Rows are not moved: A ends as
-// the inverse in pivot order, invp = step at which this row was the pivot.
-// The step loop is not unrolled (it would not fit the instruction cache):
-// the register array is rotated by one column per step so that the pivot
-// column is always A[0], and rotated back into place at the end.
+// Gauss-Jordan inverse of the tail block with partial pivoting, spread over
+// the four wavefronts: lane r of every wavefront holds row r, wavefront w
+// columns 16w .. 16w+15 (A4).  Step k: the wavefront that owns column k
+// picks the pivot -- the unused row with the largest |a[r][k]|, ties to the
+// lower row, by DPP arg-max -- and publishes the row index, its reciprocal
+// and the whole column in LDS (double-buffered, so one barrier per step);
+// every wavefront then updates its 16 columns, reading the pivot row's
+// entries with v_readlane.  Rows are not moved: A4 ends as the inverse in
+// pivot order, invp = step at which this row was the pivot.  Returns false
+// if the matrix is singular (uniform over the workgroup).
 // ---------------------------------------------------------------------
 template <typename MT>
-__device__ __forceinline__ bool tail_gj(int lane, MT (&A)[64], int &invp) {
-  bool used = lane >= NET::NT;
-  invp = lane;
+__device__ __forceinline__ bool tail_gj(Team<MT> &c, MT (&A4)[16], int &invp) {
+  WgLDS<MT> LDSA &S = *c.S;
+  bool used = c.lane >= NET::NT;
+  invp = c.lane;
 #pragma unroll 1
-  for (int k = 0; k < NET::NT; k++) {
-    const MT colk = A[0];
-    MT v = used ? (MT)-1.0 : fabs(colk);
-    int idx = lane;
-    argmax_step<MT, DPP_XOR1>(v, idx);
-    argmax_step<MT, DPP_XOR2>(v, idx);
-    argmax_step<MT, DPP_HALF_MIRROR>(v, idx);
-    argmax_step<MT, DPP_MIRROR>(v, idx);
-    MT vm = rdlane(v, 0);
-    int p = rdlane(idx, 0);
+  for (int wk = 0; wk < 4; wk++) {
 #pragma unroll
-    for (int row = 1; row < 4; row++) {
-      const MT v1 = rdlane(v, 16 * row);
-      const int i1 = rdlane(idx, 16 * row);
-      const bool t1 = (v1 > vm) || (v1 == vm && i1 < p);
-      vm = t1 ? v1 : vm;
-      p = t1 ? i1 : p;
+    for (int jk = 0; jk < 16; jk++) {
+      const int k = 16 * wk + jk;
+      if (k >= NET::NT) break;
+      const int buf = k & 1;
+      if (c.wave == wk) {
+        const MT colk = A4[jk];
+        MT v = used ? (MT)-1.0 : fabs(colk);
+        int idx = c.lane;
+        argmax_step<MT, DPP_XOR1>(v, idx);
+        argmax_step<MT, DPP_XOR2>(v, idx);
+        argmax_step<MT, DPP_HALF_MIRROR>(v, idx);
+        argmax_step<MT, DPP_MIRROR>(v, idx);
+        MT vm = rdlane(v, 0);
+        int p = rdlane(idx, 0);
+#pragma unroll
+        for (int row = 1; row < 4; row++) {
+          const MT v1 = rdlane(v, 16 * row);
+          const int i1 = rdlane(idx, 16 * row);
+          const bool t1 = (v1 > vm) || (v1 == vm && i1 < p);
+          vm = t1 ? v1 : vm;
+          p = t1 ? i1 : p;
+        }
+        S.gjcol[buf][c.lane] = colk;
+        if (c.lane == 0) {
+          S.gjp[buf] = (vm > (MT)0.0) ? p : -1;
+          S.gjd[buf] = (vm > (MT)0.0) ? (MT)1.0 / rdlane(colk, p) : (MT)0.0;
+        }
+      }
+      __syncthreads();
+      const int p = S.gjp[buf];
+      if (p < 0) return false;
+      const MT d = S.gjd[buf];
+      const MT fi = S.gjcol[buf][c.lane];
+      const bool isp = (c.lane == p);
+      if (isp) { used = true; invp = k; }
+#pragma unroll
+      for (int j = 0; j < 16; j++) {
+        const MT pj = rdlane(A4[j], p) * d;
+        MT nv = isp ? pj : fma(-fi, pj, A4[j]);
+        if (c.wave == wk && j == jk) nv = isp ? d : -fi * d;
+        A4[j] = nv;
+      }
     }
-    if (!(vm > (MT)0.0)) return false;
-    const MT d = (MT)1.0 / rdlane(colk, p);
-    const bool isp = (lane == p);
-    if (isp) { used = true; invp = k; }
-    const MT kv = isp ? d : -colk * d;
-#pragma unroll
-    for (int j = 1; j < 64; j++) {
-      const MT pj = rdlane(A[j], p) * d;
-      A[j - 1] = isp ? pj : fma(-colk, pj, A[j]);
-    }
-    A[63] = kv;
-  }
-#pragma unroll 1
-  for (int k = NET::NT; k < 64; k++) {
-    const MT t = A[0];
-#pragma unroll
-    for (int j = 1; j < 64; j++) A[j - 1] = A[j];
-    A[63] = t;
   }
   return true;
 }
@@ -438,28 +452,19 @@ __device__ __forceinline__ int wg_setup(Team<MT> &c, BdfU LDSA &u, BdfWG<MT> &b,
     schur_mfma(c, LU);
     __syncthreads();
   }
-  // tail inverse: Gauss-Jordan in wavefront 0 on a register copy, written
-  // back over the block (stride 65), then each wavefront keeps 16 columns
-  MT LDSA *Tl = LU + NET::TBASE;
-  if (c.wave == 0) {
-    MT A[64];
+  // tail inverse: every wavefront loads its 16 columns of the block
+  const MT LDSA *Tl = LU + NET::TBASE;
 #pragma unroll
-    for (int j = 0; j < 64; j++) {
-      MT v = (c.lane == j) ? (MT)1.0 : (MT)0.0;
-      if (c.lane < NET::NT && j < NET::NT) v = Tl[c.lane * NET::TS + j];
-      A[j] = v;
-    }
-    const bool ok = tail_gj(c.lane, A, b.invp);
-#pragma unroll
-    for (int j = 0; j < 64; j++) Tl[c.lane * 65 + j] = A[j];
-    if (c.lane == 0) S.flag = ok ? 0 : 1;
+  for (int j = 0; j < 16; j++) {
+    const int col = 16 * c.wave + j;
+    MT v = (c.lane == col) ? (MT)1.0 : (MT)0.0;
+    if (c.lane < NET::NT && col < NET::NT) v = Tl[c.lane * NET::TS + col];
+    b.A4[j] = v;
   }
   __syncthreads();
-#pragma unroll
-  for (int j = 0; j < 16; j++) b.A4[j] = Tl[c.lane * 65 + 16 * c.wave + j];
-  const int fl = S.flag;
+  const bool ok = tail_gj(c, b.A4, b.invp);
   __syncthreads();
-  return fl;
+  return ok ? 0 : 1;
 }
 
 // one chunk of a sparse mat-vec per lane: P[tid] = sum of LU[pos] * B[col]
@@ -596,7 +601,8 @@ __device__ __forceinline__ void wg_solve_point(Team<MT> &c, const PointArgs &a, 
 
   double yout = y0;
   int flag = 0;
-  const int status = integrate(c, u, b, a, yout, flag);
+  const double tend = a.tend_pts ? a.tend_pts[sys] : a.tend;
+  const int status = integrate(c, u, b, a, tend, yout, flag);
 
   // write-back with clip (calculate_abundances.c:308-317)
   if (c.own_) {

@@ -15,6 +15,7 @@ struct SolveRequest {
   int npts, nspec, nreac;
   double rtol, atol;
   double t0, tend, tout0; // seconds
+  const double *tend_pts; // [npts] per-point end time (seconds) or null
   int nelect0;            // 0-based electron slot
   long long mxstep;
   int *status;            // [npts]

@@ -1,0 +1,362 @@
+// Request loop for serving grid-point solves on one GPU with continuous
+// batching.  No counterpart in the reference (calculate_abundances_ solves
+// one caller-supplied batch and returns); this is the device-resident front
+// end for callers that submit many independent requests over time.
+//
+// A request is npts points of one network with its own end time.  Pending
+// requests wait in one FIFO; whenever a launch slot is free, the oldest
+// pending request's network is batched with every later pending request of
+// the same network that fits (points cap and device-memory budget), packed
+// into pinned staging, copied to a block of the device arena, solved by the
+// persistent kernel with per-point end times, and copied back -- all on the
+// slot's own stream, so batches of different requests and networks overlap
+// on the device.  Requests that arrive while others run are admitted as
+// slots and arena space free up.  Admission is by device-memory footprint:
+// a batch starts only when its inputs and outputs fit the arena.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/pdrchem.h"
+#include "gen/net_reduced.h"
+#include "gen/wg_full.h"
+#include "gen/wg_mynetwork.h"
+#include "pdr_internal.h"
+
+using namespace pdrchem;
+
+namespace {
+
+constexpr double kYear = 3.1556926e7;  // calculate_abundances.c:90
+
+double now_s() {
+  using namespace std::chrono;
+  return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+struct Shape {
+  int nspec, nreac;
+};
+const Shape kShape[NET_COUNT] = {{pdrnet_reduced::NSPEC, pdrnet_reduced::NREAC},
+                                 {pdrwg_full::NSPEC, pdrwg_full::NREAC},
+                                 {pdrwg_mynetwork::NSPEC, pdrwg_mynetwork::NREAC}};
+
+size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
+
+// first-fit allocator over one device allocation
+struct Arena {
+  char *base = nullptr;
+  size_t cap = 0;
+  std::map<size_t, size_t> free_;  // offset -> size, coalesced
+  bool init(size_t bytes) {
+    cap = align256(bytes);
+    if (hipMalloc(&base, cap) != hipSuccess) return false;
+    free_[0] = cap;
+    return true;
+  }
+  long long alloc(size_t n) {
+    n = align256(n);
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+      if (it->second < n) continue;
+      const size_t off = it->first, rest = it->second - n;
+      free_.erase(it);
+      if (rest) free_[off + n] = rest;
+      return (long long)off;
+    }
+    return -1;
+  }
+  void release(size_t off, size_t n) {
+    n = align256(n);
+    auto it = free_.emplace(off, n).first;
+    auto nx = std::next(it);
+    if (nx != free_.end() && it->first + it->second == nx->first) {
+      it->second += nx->second;
+      free_.erase(nx);
+    }
+    if (it != free_.begin()) {
+      auto pv = std::prev(it);
+      if (pv->first + pv->second == it->first) {
+        pv->second += it->second;
+        free_.erase(it);
+      }
+    }
+  }
+};
+
+struct Req {
+  int ticket, net, npts;
+  double *ab;  // caller's arrays (ab in/out), valid until the request completes
+  const double *rate, *dens;
+  double tend_s, t_submit;
+};
+
+struct Done {
+  int ticket, status;
+  double latency_ms;
+};
+
+struct Slot {
+  hipStream_t stream = nullptr;
+  hipEvent_t done = nullptr;
+  bool busy = false;
+  int net = 0;
+  size_t npts = 0, off = 0, bytes = 0;
+  std::vector<Req> reqs;
+  char *pin = nullptr;  // pinned staging of the whole block
+  size_t pin_cap = 0;
+};
+
+}  // namespace
+
+struct pdrchem_server {
+  std::mutex mu;
+  Arena arena;
+  std::vector<Slot> slots;
+  std::deque<Req> pending;
+  std::vector<Done> done;
+  int next_ticket = 0;
+  int precision = PDRCHEM_MIXED;
+  size_t max_batch_pts = 65536;
+  double rtol = 1e-8, atol = 1e-30;
+  std::string err;
+};
+
+namespace {
+
+// device (and pinned) block layout of a batch
+struct Layout {
+  size_t ab, rate, dens, tend, status, counter, total;
+};
+Layout layout(int net, size_t n) {
+  const Shape &s = kShape[net];
+  Layout L;
+  L.ab = 0;
+  L.rate = L.ab + align256(n * s.nspec * 8);
+  L.dens = L.rate + align256(n * s.nreac * 8);
+  L.tend = L.dens + align256(n * 8);
+  L.status = L.tend + align256(n * 8);
+  L.counter = L.status + align256(n * 4);
+  L.total = L.counter + 256;
+  return L;
+}
+
+bool launch_batch(pdrchem_server *S, Slot &sl) {
+  const int net = sl.net;
+  const Shape &s = kShape[net];
+  const size_t n = sl.npts;
+  const Layout L = layout(net, n);
+  if (sl.pin_cap < L.total) {
+    if (sl.pin) (void)hipHostFree(sl.pin);
+    sl.pin = nullptr;
+    sl.pin_cap = 0;
+    if (hipHostMalloc((void **)&sl.pin, L.total, hipHostMallocDefault) != hipSuccess) return false;
+    sl.pin_cap = L.total;
+  }
+  size_t k = 0;
+  for (const Req &r : sl.reqs) {
+    std::memcpy(sl.pin + L.ab + k * s.nspec * 8, r.ab, (size_t)r.npts * s.nspec * 8);
+    std::memcpy(sl.pin + L.rate + k * s.nreac * 8, r.rate, (size_t)r.npts * s.nreac * 8);
+    std::memcpy(sl.pin + L.dens + k * 8, r.dens, (size_t)r.npts * 8);
+    double *te = (double *)(sl.pin + L.tend) + k;
+    for (int i = 0; i < r.npts; i++) te[i] = r.tend_s;
+    k += r.npts;
+  }
+  char *dev = S->arena.base + sl.off;
+  if (hipMemcpyAsync(dev, sl.pin, L.status, hipMemcpyHostToDevice, sl.stream) != hipSuccess) return false;
+  if (hipMemsetAsync(dev + L.counter, 0, sizeof(unsigned int), sl.stream) != hipSuccess) return false;
+  SolveRequest q{};
+  q.abundance = (double *)(dev + L.ab);
+  q.rate = (const double *)(dev + L.rate);
+  q.density = (const double *)(dev + L.dens);
+  q.npts = (int)n;
+  q.nspec = s.nspec;
+  q.nreac = s.nreac;
+  q.rtol = S->rtol;
+  q.atol = S->atol;
+  q.t0 = 0.0;
+  q.tend = sl.reqs.front().tend_s;
+  q.tend_pts = (const double *)(dev + L.tend);
+  q.tout0 = 1.0e-4 * kYear;  // calculate_abundances.c:214
+  q.nelect0 = s.nspec - 1;   // electron last (read_species.F90:164)
+  q.mxstep = 10000000LL;     // calculate_abundances.c:111
+  q.status = (int *)(dev + L.status);
+  q.stats = nullptr;
+  q.counter = (unsigned int *)(dev + L.counter);
+  q.max_blocks = 0;
+  q.mixed = S->precision == PDRCHEM_MIXED ? 1 : 0;
+  const int rc = net == NET_REDUCED ? launch_wave_solver(net, q, sl.stream) : launch_wg_solver(net, q, sl.stream);
+  if (rc != 0) return false;
+  // abundances and statuses back to the staging block
+  if (hipMemcpyAsync(sl.pin + L.ab, dev + L.ab, (size_t)n * s.nspec * 8, hipMemcpyDeviceToHost, sl.stream) !=
+      hipSuccess)
+    return false;
+  if (hipMemcpyAsync(sl.pin + L.status, dev + L.status, n * 4, hipMemcpyDeviceToHost, sl.stream) != hipSuccess)
+    return false;
+  return hipEventRecord(sl.done, sl.stream) == hipSuccess;
+}
+
+// completed batches: results to the callers' arrays, arena block freed
+void retire(pdrchem_server *S) {
+  for (Slot &sl : S->slots) {
+    if (!sl.busy || hipEventQuery(sl.done) != hipSuccess) continue;
+    const Shape &s = kShape[sl.net];
+    const Layout L = layout(sl.net, sl.npts);
+    const double t = now_s();
+    size_t k = 0;
+    for (const Req &r : sl.reqs) {
+      std::memcpy(r.ab, sl.pin + L.ab + k * s.nspec * 8, (size_t)r.npts * s.nspec * 8);
+      const int *st = (const int *)(sl.pin + L.status) + k;
+      int status = 0;
+      for (int i = 0; i < r.npts && !status; i++) status = st[i];
+      S->done.push_back(Done{r.ticket, status, (t - r.t_submit) * 1e3});
+      k += r.npts;
+    }
+    S->arena.release(sl.off, sl.bytes);
+    sl.reqs.clear();
+    sl.busy = false;
+  }
+}
+
+// start batches on free slots while pending requests fit
+bool admit(pdrchem_server *S) {
+  for (Slot &sl : S->slots) {
+    if (sl.busy || S->pending.empty()) continue;
+    const int net = S->pending.front().net;
+    std::vector<Req> take;
+    size_t n = 0;
+    for (auto it = S->pending.begin(); it != S->pending.end();) {
+      if (it->net == net && n + it->npts <= S->max_batch_pts) {
+        n += it->npts;
+        take.push_back(*it);
+        it = S->pending.erase(it);
+      } else {
+        ++it;
+      }
+    }
+    // shrink the batch from the back until it fits the arena
+    long long off = -1;
+    size_t bytes = 0;
+    while (!take.empty()) {
+      bytes = layout(net, n).total;
+      off = S->arena.alloc(bytes);
+      if (off >= 0) break;
+      n -= take.back().npts;
+      // back into the queue at its arrival position (tickets increase)
+      auto pos = S->pending.begin();
+      while (pos != S->pending.end() && pos->ticket < take.back().ticket) ++pos;
+      S->pending.insert(pos, take.back());
+      take.pop_back();
+    }
+    if (take.empty()) return true;  // the oldest request waits for arena space
+    sl.net = net;
+    sl.npts = n;
+    sl.off = (size_t)off;
+    sl.bytes = bytes;
+    sl.reqs = std::move(take);
+    sl.busy = true;
+    if (!launch_batch(S, sl)) {
+      // nothing of this batch runs: its requests complete with the set-up
+      // failure status of calculate_abundances_ (1)
+      (void)hipStreamSynchronize(sl.stream);
+      const double t = now_s();
+      for (const Req &r : sl.reqs) S->done.push_back(Done{r.ticket, 1, (t - r.t_submit) * 1e3});
+      S->arena.release(sl.off, sl.bytes);
+      sl.reqs.clear();
+      sl.busy = false;
+      S->err = "launch of a batch failed";
+      return false;
+    }
+  }
+  return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+pdrchem_server *pdrchem_server_create(size_t device_bytes, int nslots, int precision) {
+  pdrchem_server *S = new pdrchem_server();
+  if (nslots < 1) nslots = 1;
+  if (!S->arena.init(device_bytes ? device_bytes : (size_t)1 << 30)) {
+    delete S;
+    return nullptr;
+  }
+  S->slots.resize(nslots);
+  for (Slot &sl : S->slots) {
+    if (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess) {
+      pdrchem_server_destroy(S);
+      return nullptr;
+    }
+  }
+  S->precision = precision == PDRCHEM_FP64 ? PDRCHEM_FP64 : PDRCHEM_MIXED;
+  S->rtol = chemistry_module_mp_relative_abundance_tolerance_;
+  S->atol = chemistry_module_mp_absolute_abundance_tolerance_;
+  return S;
+}
+
+int pdrchem_server_submit(pdrchem_server *S, int net, int npts, double *abundance, const double *rate,
+                          const double *density, double t_end_years) {
+  if (!S || net < 0 || net >= NET_COUNT || npts <= 0 || !abundance || !rate || !density) return -1;
+  std::lock_guard<std::mutex> lock(S->mu);
+  if ((size_t)npts > S->max_batch_pts || layout(net, npts).total > S->arena.cap) {
+    S->err = "request larger than the serving budget";
+    return -1;
+  }
+  Req r{S->next_ticket++, net, npts, abundance, rate, density, t_end_years * kYear, now_s()};
+  S->pending.push_back(r);
+  if (!admit(S)) return -1;
+  return r.ticket;
+}
+
+int pdrchem_server_progress(pdrchem_server *S, int wait) {
+  if (!S) return -1;
+  std::lock_guard<std::mutex> lock(S->mu);
+  for (;;) {
+    retire(S);
+    if (!admit(S)) return -1;
+    int busy = 0;
+    for (const Slot &sl : S->slots) busy += sl.busy;
+    if (!wait || (busy == 0 && S->pending.empty())) return busy + (int)S->pending.size();
+    for (Slot &sl : S->slots)
+      if (sl.busy) {
+        (void)hipEventSynchronize(sl.done);
+        break;
+      }
+  }
+}
+
+int pdrchem_server_poll(pdrchem_server *S, int max, int *tickets, int *status, double *latency_ms) {
+  if (!S || max < 0) return -1;
+  std::lock_guard<std::mutex> lock(S->mu);
+  const int n = (int)std::min<size_t>((size_t)max, S->done.size());
+  for (int i = 0; i < n; i++) {
+    if (tickets) tickets[i] = S->done[i].ticket;
+    if (status) status[i] = S->done[i].status;
+    if (latency_ms) latency_ms[i] = S->done[i].latency_ms;
+  }
+  S->done.erase(S->done.begin(), S->done.begin() + n);
+  return n;
+}
+
+void pdrchem_server_destroy(pdrchem_server *S) {
+  if (!S) return;
+  for (Slot &sl : S->slots) {
+    if (sl.stream) (void)hipStreamSynchronize(sl.stream);
+    if (sl.done) (void)hipEventDestroy(sl.done);
+    if (sl.stream) (void)hipStreamDestroy(sl.stream);
+    if (sl.pin) (void)hipHostFree(sl.pin);
+  }
+  if (S->arena.base) (void)hipFree(S->arena.base);
+  delete S;
+}
+
+const char *pdrchem_server_last_error(pdrchem_server *S) { return S ? S->err.c_str() : "null server"; }
+
+}  // extern "C"

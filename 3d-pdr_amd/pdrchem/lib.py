@@ -24,6 +24,8 @@ NETWORKS = {"reduced": (33, 329), "full": (215, 2926), "mynetwork": (58, 634)}
 EXPORTED_FUNCTIONS = [
     "calculate_abundances_", "pdrchem_network_id", "pdrchem_solve_device", "pdrchem_set_precision",
     "pdrchem_rates_device", "pdrchem_kernel_info", "pdrchem_set_max_blocks", "pdrchem_last_error",
+    "pdrchem_server_create", "pdrchem_server_submit", "pdrchem_server_progress", "pdrchem_server_poll",
+    "pdrchem_server_last_error", "pdrchem_server_destroy",
 ]
 EXPORTED_GLOBALS = [
     ("chemistry_module_mp_relative_abundance_tolerance_", C.c_double),
@@ -78,6 +80,15 @@ def load(path: Optional[str] = None):
     L.pdrchem_set_precision.argtypes = [C.c_int]
     L.pdrchem_set_precision.restype = None
     L.pdrchem_last_error.restype = C.c_char_p
+    L.pdrchem_server_create.argtypes = [C.c_size_t, C.c_int, C.c_int]
+    L.pdrchem_server_create.restype = vp
+    L.pdrchem_server_submit.argtypes = [vp, C.c_int, C.c_int, vp, vp, vp, C.c_double]
+    L.pdrchem_server_progress.argtypes = [vp, C.c_int]
+    L.pdrchem_server_poll.argtypes = [vp, C.c_int, ip, ip, dp]
+    L.pdrchem_server_last_error.argtypes = [vp]
+    L.pdrchem_server_last_error.restype = C.c_char_p
+    L.pdrchem_server_destroy.argtypes = [vp]
+    L.pdrchem_server_destroy.restype = None
     _lib = L
     return L
 
@@ -199,3 +210,60 @@ def rates_device(net: int, T, Td, G, Av, cols=None, phys: Optional[Phys] = None,
     if rc != 0:
         raise PdrChemError(last_error())
     return out, neg
+
+
+# ---------------------------------------------------------------------------
+# request loop (continuous batching) over host arrays
+# ---------------------------------------------------------------------------
+
+class Server:
+    """Thin owner of a pdrchem_server: submit requests of host arrays, drive
+    progress, collect (ticket, status, latency_ms).  The arrays of a request
+    are kept referenced until it completes (the library writes the
+    abundances back into them)."""
+
+    def __init__(self, device_bytes: int = 1 << 30, nslots: int = 4, precision: int = MIXED):
+        self.L = load()
+        self.h = self.L.pdrchem_server_create(device_bytes, nslots, precision)
+        if not self.h:
+            raise PdrChemError("pdrchem_server_create failed")
+        self.live = {}
+
+    def submit(self, net: int, abundance: np.ndarray, rate: np.ndarray, density: np.ndarray, t_end_years: float) -> int:
+        for a in (abundance, rate, density):
+            if a.dtype != np.float64 or not a.flags.c_contiguous:
+                raise PdrChemError("arrays must be C-contiguous float64")
+        n = abundance.shape[0]
+        t = self.L.pdrchem_server_submit(self.h, net, n, abundance.ctypes.data, rate.ctypes.data,
+                                         density.ctypes.data, float(t_end_years))
+        if t < 0:
+            raise PdrChemError(self.L.pdrchem_server_last_error(self.h).decode())
+        self.live[t] = (abundance, rate, density)
+        return t
+
+    def progress(self, wait: bool = False) -> int:
+        r = self.L.pdrchem_server_progress(self.h, 1 if wait else 0)
+        if r < 0:
+            raise PdrChemError(self.L.pdrchem_server_last_error(self.h).decode())
+        return r
+
+    def poll(self, max_n: int = 4096):
+        tk = (C.c_int * max_n)()
+        st = (C.c_int * max_n)()
+        lat = (C.c_double * max_n)()
+        n = self.L.pdrchem_server_poll(self.h, max_n, tk, st, lat)
+        out = [(tk[i], st[i], lat[i]) for i in range(n)]
+        for t, _, _ in out:
+            self.live.pop(t, None)
+        return out
+
+    def close(self):
+        if self.h:
+            self.L.pdrchem_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -93,3 +93,37 @@ def initial_abundances(net: Network, n: int, rest: float = 1.0e-30) -> np.ndarra
 def zeta_normalised(zeta_s: float = 5.0e-17) -> float:
     """Cosmic-ray rate in the units the rate code uses (input_parameters.F90:31)."""
     return zeta_s / 1.3e-17
+
+
+@dataclasses.dataclass
+class Request:
+    arrival: float        # seconds after the start of the stream
+    network: str
+    npts: int
+    t_end: float          # years
+    first: int            # index of the request's first point in its network's point stream
+
+
+def request_stream(duration: float = 60.0, rate: float = 200.0, seed: int = 20251204,
+                   networks=("reduced", "full")) -> list:
+    """Serving workload: Poisson arrivals at `rate` per second for `duration`
+    seconds; points per request floor(exp(U[0, ln 4096])) (1..4096, mean
+    ~492); network chosen with equal probability; end time log-uniform in
+    [1e3, 1e7] yr.  Each network's requests take consecutive points of that
+    network's seeded random-point stream (random_points(..., start=first))."""
+    kmax = int(duration * rate * 1.5) + 64
+    idx = np.arange(kmax, dtype=np.uint64)
+    gaps = -np.log(1.0 - uniform(seed, 10, idx)) / rate
+    arrival = np.cumsum(gaps)
+    which = (uniform(seed, 11, idx) * len(networks)).astype(np.int64)
+    npts = np.clip(np.floor(np.exp(uniform(seed, 12, idx) * np.log(4096.0))).astype(np.int64), 1, 4096)
+    t_end = 10.0 ** (3.0 + 4.0 * uniform(seed, 13, idx))
+    out = []
+    firsts = {nm: 0 for nm in networks}
+    for k in range(kmax):
+        if arrival[k] >= duration:
+            break
+        nm = networks[which[k]]
+        out.append(Request(float(arrival[k]), nm, int(npts[k]), float(t_end[k]), firsts[nm]))
+        firsts[nm] += int(npts[k])
+    return out

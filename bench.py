@@ -101,6 +101,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--check", type=int, default=64, help="points checked against the oracle")
+    ap.add_argument("--serve", action="store_true",
+                    help="serving workload: Poisson stream of requests through the request loop")
+    ap.add_argument("--serve-rate", type=float, default=6.0, help="requests per second")
+    ap.add_argument("--serve-duration", type=float, default=30.0, help="seconds of arrivals")
     a = ap.parse_args()
     if a.npts is None:
         a.npts = 1_000_000 if a.network == "reduced" else 20_000
@@ -113,6 +117,9 @@ def main():
     from pdrchem import lib, workloads
     from pdrchem.netgen import load_network
 
+    if a.serve:
+        serve(a)
+        return
     dev = torch.device("cuda:0")
     name = a.network
     net = load_network(name)
@@ -215,6 +222,67 @@ def main():
         k = min(a.npts, 200000)
         res["cpu_baseline"] = cpu_baseline(name, pts.subset(np.arange(k)), x0_np[:k], rate[:k].cpu().numpy(),
                                            a.cpu_seconds, thr)
+    print(json.dumps(res))
+
+
+def serve(a):
+    """Serving workload: requests of 1..4096 points (log-uniform) of the
+    reduced or the full network (equal odds), end times log-uniform in
+    [1e3, 1e7] yr, Poisson arrivals, submitted to the request loop as they
+    arrive; latency is submit-to-result, throughput is points per second
+    from the first arrival to the last result.  Inputs (device-evaluated
+    rates) are prepared before the stream starts."""
+    import torch
+    from pdrchem import lib, workloads
+    from pdrchem.netgen import load_network
+    dev = torch.device("cuda:0")
+    prec = lib.MIXED if a.precision == "mixed" else lib.FP64
+    R = workloads.request_stream(duration=a.serve_duration, rate=a.serve_rate)
+    data = {}
+    for name in ("reduced", "full"):
+        tot = sum(r.npts for r in R if r.network == name)
+        net = load_network(name)
+        nid = lib.network_id(name)
+        P = workloads.random_points(max(tot, 1), columns=(name != "reduced"))
+        rate, neg = lib.rates_device(nid, *[torch.tensor(v, device=dev) for v in (P.T, P.Td, P.G, P.Av, P.cols)])
+        torch.cuda.synchronize()
+        data[name] = dict(nid=nid, rate=rate.cpu().numpy(), nH=P.n_H.copy(),
+                          ab=workloads.initial_abundances(net, max(tot, 1)))
+    lib.set_globals(rtol=1e-8, atol=1e-30)
+    srv = lib.Server(device_bytes=4 << 30, nslots=4, precision=prec)
+    lat, status = {}, {}
+    t0 = time.perf_counter()
+    i = 0
+    while i < len(R) or len(lat) < len(R):
+        now = time.perf_counter() - t0
+        while i < len(R) and R[i].arrival <= now:
+            r, d = R[i], data[R[i].network]
+            sl = slice(r.first, r.first + r.npts)
+            srv.submit(d["nid"], d["ab"][sl], d["rate"][sl], d["nH"][sl], r.t_end)
+            i += 1
+        srv.progress()
+        for tk, st, ms in srv.poll():
+            lat[tk], status[tk] = ms, st
+        if i < len(R):
+            time.sleep(max(0.0, min(0.002, R[i].arrival - (time.perf_counter() - t0))))
+        else:
+            time.sleep(0.001)
+    wall = time.perf_counter() - t0
+    srv.close()
+    L = np.array([lat[k] for k in sorted(lat)])
+    npts = sum(r.npts for r in R)
+    res = {
+        "metric": "grid-point chemistry solves/s at <=1e-4 rel err vs FP64 CVODE; p50/p99 batch ms",
+        "value": npts / wall, "unit": "solves/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+        "ms_per_step": wall * 1e3, "p50_batch_ms": float(np.percentile(L, 50)),
+        "p99_batch_ms": float(np.percentile(L, 99)), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64" if prec == lib.FP64 else "f64+f32-matrix",
+        "data": "synthetic (seeded counter-based RNG)",
+        "config": {"workload": f"serving: Poisson {a.serve_rate} req/s for {a.serve_duration} s, 1-4096 points "
+                               f"per request (log-uniform), reduced/full 50/50, t_end logU[1e3,1e7] yr",
+                   "requests": len(R), "points": npts, "precision": a.precision, "slots": 4},
+        "failed_requests": int(sum(1 for v in status.values() if v != 0)),
+    }
     print(json.dumps(res))
 
 

@@ -14,6 +14,8 @@
 #ifndef PDRCHEM_H
 #define PDRCHEM_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -110,6 +112,41 @@ void pdrchem_set_max_blocks(int max_blocks);
 
 /* Text of the last error of this thread (empty if none). */
 const char *pdrchem_last_error(void);
+
+/* ---------------------------------------------------------------------
+ * Request loop with continuous batching (no counterpart in the reference:
+ * for callers that submit many independent solves over time, e.g. an outer
+ * iteration that re-solves shrinking, uneven sets of points)
+ * ------------------------------------------------------------------- */
+
+typedef struct pdrchem_server pdrchem_server;
+
+/* device_bytes: arena for in-flight batches (0 = 1 GiB; admission is by this
+ * footprint); nslots: batches in flight at once, each on its own stream;
+ * precision as for pdrchem_solve_device.  Tolerances are read from the
+ * globals above at creation.  NULL on failure. */
+pdrchem_server *pdrchem_server_create(size_t device_bytes, int nslots, int precision);
+
+/* Queue npts points of network `net` to be integrated from t = 0 to
+ * t_end_years with calculate_abundances_ semantics.  abundance[npts*nspec]
+ * (host, in/out), rate[npts*nreac] and density[npts] must stay valid until
+ * the request is reported complete.  Returns the request's ticket (>= 0) or
+ * -1 (bad arguments, request larger than the arena). */
+int pdrchem_server_submit(pdrchem_server *server, int net, int npts, double *abundance, const double *rate,
+                          const double *density, double t_end_years);
+
+/* Retire finished batches (results copied into the callers' arrays) and
+ * start pending ones; wait != 0 blocks until nothing is pending or in
+ * flight.  Returns the number of requests pending or in flight, -1 on error. */
+int pdrchem_server_progress(pdrchem_server *server, int wait);
+
+/* Up to max completed requests since the last poll: ticket, status (0, or
+ * the CVODE-style flag of its first failed point, or 1 if its batch could
+ * not be launched) and submit-to-completion latency in ms.  Returns the count. */
+int pdrchem_server_poll(pdrchem_server *server, int max, int *tickets, int *status, double *latency_ms);
+
+const char *pdrchem_server_last_error(pdrchem_server *server);
+void pdrchem_server_destroy(pdrchem_server *server);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,94 @@
+"""Request loop with continuous batching (pdrchem_server_* in libpdrchem.so).
+
+CPU: the seeded request stream of the serving workload (Poisson arrivals,
+log-uniform sizes and end times) and the library's exports.
+GPU: requests of two networks and many end times submitted while others run
+are batched together on several streams; every request's result must be
+bit-identical to the same points solved alone with the same end time (each
+point is integrated by its own wavefront / workgroup with a fixed order of
+operations, whatever else shares the launch), and the bookkeeping (tickets,
+statuses, latencies) must account for every request.
+"""
+import numpy as np
+import pytest
+
+from pdrchem import workloads
+
+
+def test_request_stream_properties():
+    R = workloads.request_stream(duration=30.0, rate=200.0)
+    assert R == workloads.request_stream(duration=30.0, rate=200.0)  # seeded
+    arr = np.array([r.arrival for r in R])
+    assert np.all(np.diff(arr) >= 0) and arr[-1] < 30.0
+    assert abs(len(R) / 30.0 - 200.0) < 200.0 * 0.1              # Poisson count, 6000 expected
+    n = np.array([r.npts for r in R])
+    assert n.min() >= 1 and n.max() <= 4096
+    assert 400 < n.mean() < 600                                   # E = 4095/ln 4096 ~ 492
+    te = np.array([r.t_end for r in R])
+    assert te.min() >= 1e3 and te.max() <= 1e7
+    frac_full = np.mean([r.network == "full" for r in R])
+    assert 0.45 < frac_full < 0.55
+    # each network's requests take consecutive, disjoint point ranges
+    for nm in ("reduced", "full"):
+        sub = [r for r in R if r.network == nm]
+        assert all(b.first == a.first + a.npts for a, b in zip(sub, sub[1:]))
+
+
+def _requests(orc_by_net, nets_sizes, seed):
+    from pdrchem.netgen import load_network
+    rng = np.random.default_rng(seed)
+    out = []
+    for k, (name, n) in enumerate(nets_sizes):
+        net = load_network(name)
+        P = workloads.random_points(n, seed=seed + k, columns=(name != "reduced"))
+        x0 = workloads.initial_abundances(net, n)
+        r = orc_by_net[name].rates(P.T, P.Td, P.G, P.Av, P.cols)
+        t_end = float(10.0 ** rng.uniform(3, 7))
+        out.append((name, x0, np.ascontiguousarray(r), P.n_H.copy(), t_end))
+    return out
+
+
+@pytest.mark.gpu
+def test_server_mix_equals_alone():
+    import torch
+    import oracle
+    from pdrchem import lib
+    orc = {nm: oracle.Oracle(nm) for nm in ("reduced", "full")}
+    sizes = [("reduced", 37), ("full", 3), ("reduced", 200), ("reduced", 1), ("full", 5),
+             ("reduced", 64), ("full", 2), ("reduced", 9)]
+    reqs = _requests(orc, sizes, seed=42)
+    lib.set_globals(rtol=1e-8, atol=1e-30)
+    srv = lib.Server(device_bytes=256 << 20, nslots=3, precision=lib.MIXED)
+    abs_mix, tickets = [], []
+    for name, x0, r, nH, te in reqs:
+        ab = x0.copy()
+        abs_mix.append(ab)
+        tickets.append(srv.submit(lib.network_id(name), ab, r, nH, te))
+        srv.progress()
+    srv.progress(wait=True)
+    done = srv.poll()
+    srv.close()
+    assert sorted(t for t, _, _ in done) == sorted(tickets)
+    assert all(st == 0 for _, st, _ in done)
+    assert all(lat > 0 for _, _, lat in done)
+    # the same requests, each alone through the device entry point
+    dev = torch.device("cuda:0")
+    for (name, x0, r, nH, te), ab_mix in zip(reqs, abs_mix):
+        ab = torch.tensor(x0, device=dev)
+        status, _ = lib.solve_device(lib.network_id(name), ab, torch.tensor(r, device=dev),
+                                     torch.tensor(nH, device=dev), t_end=te, precision=lib.MIXED)
+        torch.cuda.synchronize()
+        assert int(status.abs().sum().item()) == 0
+        alone = ab.cpu().numpy()
+        assert np.array_equal(alone, ab_mix), (name, te, float(np.max(np.abs(alone - ab_mix))))
+
+
+@pytest.mark.gpu
+def test_server_rejects_oversized():
+    from pdrchem import lib
+    srv = lib.Server(device_bytes=1 << 20, nslots=1)
+    n = 4096
+    ab = np.zeros((n, 215))
+    with pytest.raises(lib.PdrChemError):
+        srv.submit(lib.network_id("full"), ab, np.zeros((n, 2926)), np.ones(n), 1e3)
+    srv.close()
