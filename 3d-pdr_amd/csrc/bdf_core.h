@@ -64,6 +64,29 @@ __device__ __forceinline__ double selz(const double (&a)[NA], int k) {
   return v;
 }
 
+// x^(1/n) for x >= 0, n >= 1, to about 1e-7 relative.  CVODE takes these
+// roots only to turn error-norm ratios into step-size ratios, which it then
+// clamps and applies with safety factors, so FP32 accuracy is ample.  The
+// binary exponent is split off exactly (x = m 2^e, e = kn + r), leaving a
+// root of m 2^r in [2^(-n), 2^n) for the hardware log2 / exp2; the library
+// pow(double) costs about 30x the instructions.
+__device__ __forceinline__ double root_n(double x, int n) {
+  if (x == 0.0) return 0.0;
+  if (!(x > 0.0) || isinf(x)) return (x < 0.0 && n > 1) ? __builtin_nan("") : x;  // as pow
+  int e;
+  const double m = frexp(x, &e);
+  const int k = e / n, r = e - k * n;
+  const float f = ((float)r + __builtin_amdgcn_logf((float)m)) * __builtin_amdgcn_rcpf((float)n);
+  return ldexp((double)__builtin_amdgcn_exp2f(f), k);
+}
+
+// x^n for a small integer n >= 0
+__device__ __forceinline__ double ipow(double x, int n) {
+  double p = 1.0;
+  for (int i = 0; i < n; i++) p *= x;
+  return p;
+}
+
 template <class T>
 __device__ __forceinline__ double wrms(T &c, double v, double w) {
   double p = c.own() ? v * w : 0.0;
@@ -354,7 +377,7 @@ __device__ __forceinline__ int do_error_test(T &c, BdfU LDSA &u, S &b, int &nfla
   if (c.lead()) u.netf++;
   if (nef == MXNEF) { c.sync(); return -3; }  // CV_ERR_FAILURE
   if (nef <= MXNEF1) {
-    double eta = 1.0 / (pow(BIAS2 * dsm, 1.0 / L) + ADDON);
+    double eta = 1.0 / (root_n(BIAS2 * dsm, L) + ADDON);
     eta = fmax(ETAMIN, eta);
     if (nef >= SMALL_NEF) eta = fmin(eta, ETAMXF);
     if (c.lead()) { u.etamax = 1.0; u.eta = eta; }
@@ -438,7 +461,7 @@ __device__ __forceinline__ void prepare_next_step(T &c, BdfU LDSA &u, S &b, doub
     hprime = h;
     eta = 1.0;
   } else {
-    double etaq = 1.0 / (pow(BIAS2 * dsm, 1.0 / L) + ADDON);
+    double etaq = 1.0 / (root_n(BIAS2 * dsm, L) + ADDON);
     if (qwait != 0) {
       eta = etaq;
       set_eta_vals(eta, hprime, h, etamax);
@@ -447,13 +470,13 @@ __device__ __forceinline__ void prepare_next_step(T &c, BdfU LDSA &u, S &b, doub
       double etaqm1 = 0.0, etaqp1 = 0.0;
       if (q > 1) {
         double ddn = wrms(c, selz(b.zn, q), b.ewt) * u.tq[1];
-        etaqm1 = 1.0 / (pow(BIAS1 * ddn, 1.0 / q) + ADDON);
+        etaqm1 = 1.0 / (root_n(BIAS1 * ddn, q) + ADDON);
       }
       const double saved_tq5 = u.saved_tq5;
       if (q != QMAX && saved_tq5 != 0.0) {
-        double cquot = (u.tq[5] / saved_tq5) * pow(h / u.tau[2], (double)L);
+        double cquot = (u.tq[5] / saved_tq5) * ipow(h / u.tau[2], L);
         double dup = wrms(c, b.acor - cquot * b.zn[QMAX], b.ewt) * u.tq[3];
-        etaqp1 = 1.0 / (pow(BIAS3 * dup, 1.0 / (L + 1)) + ADDON);
+        etaqp1 = 1.0 / (root_n(BIAS3 * dup, L + 1) + ADDON);
       }
       double etam = fmax(etaqm1, fmax(etaq, etaqp1));
       if (etam < THRESH) {

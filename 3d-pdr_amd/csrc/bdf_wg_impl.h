@@ -66,6 +66,7 @@ struct WgLDS {
   double X[NET::NB + 1];     // factor vector: x_0..x_{NEQ-1}, x_e, 1
   double B[NET::NB + 1];     // solve vector; B[NB] = 0
   double W[NET::NB];         // error weights the matrix was scaled with
+  double Winv[NET::NB];      //   and their reciprocals
   double P[NET::NV + 1];     // RHS chunk partials (P[NV] = 0), tail gather partials
   double V[64], V2[64];      // tail inverse exchange
   MT gjcol[2][64];           // Gauss-Jordan: pivot column (double-buffered by step)
@@ -131,6 +132,25 @@ template <typename T>
 __device__ __forceinline__ const T *opaque(const T *p) {
   asm volatile("" : "+s"(p));
   return p;
+}
+
+// Runs f over a lane's n table items (item t at tab[(off + t)*BLOCK + tid])
+// in order, loading U items at a time ahead of their use: the loads of a
+// group are independent and overlap, so a lane waits for the table once per
+// U items instead of once per item.  (A plain loop whose body ends in a
+// conditional store is not unrolled by the compiler and pays the full
+// load latency every item.)
+template <int U, typename W, typename F>
+__device__ __forceinline__ void for_items(const W *tab, int off, int n, int tid, F &&f) {
+  int t = 0;
+  for (; t + U <= n; t += U) {
+    W w[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) w[u] = tab[(size_t)(off + t + u) * BLOCK + tid];
+#pragma unroll
+    for (int u = 0; u < U; u++) f(w[u]);
+  }
+  for (; t < n; t++) f(tab[(size_t)(off + t) * BLOCK + tid]);
 }
 
 // the region as LU storage (positions: [0, NHV) head entries, NHV the zero
@@ -203,18 +223,15 @@ __device__ __forceinline__ void wg_jacobian(Team<MT> &c) {
   }
   if (c.tid == 0) D[2 * NET::NSLOT] = 0.0;
   __syncthreads();
-  const uint32_t *jt = opaque(NET::jac_term);
   double acc = 0.0;
-#pragma unroll 8
-  for (int t = 0; t < NET::JTM; t++) {
-    const uint32_t w = jt[t * BLOCK + c.tid];
+  for_items<8>(opaque(NET::jac_term), 0, NET::JTM, c.tid, [&](uint32_t w) {
     const int coef = (int)(w << 15) >> 28;          // signed 4-bit field at bit 13
     acc = fma((double)coef, D[w & 0x1fff], acc);
     if ((w >> 17) & 1) {
       c.Jg[w >> 18] = (MT)acc;
       acc = 0.0;
     }
-  }
+  });
   __syncthreads();
 }
 
@@ -378,75 +395,88 @@ __device__ __forceinline__ int wg_setup(Team<MT> &c, BdfU LDSA &u, BdfWG<MT> &b,
   if (jbad) wg_jacobian(c);
   // weights: W M W^-1 (w = error weights; x_e row/column by its own weight)
   b.wsc = c.own_ ? b.ewt : 1.0;
-  if (c.own_) S.W[c.tid] = b.wsc;
-  if (c.tid == 0) S.W[NET::NEQ] = 1.0 / (c.rtol * fabs(xe) + c.atol);
+  if (c.own_) {
+    S.W[c.tid] = b.wsc;
+    S.Winv[c.tid] = 1.0 / b.wsc;
+  }
+  if (c.tid == 0) {
+    const double we = 1.0 / (c.rtol * fabs(xe) + c.atol);
+    S.W[NET::NEQ] = we;
+    S.Winv[NET::NEQ] = 1.0 / we;
+  }
   __syncthreads();
+  // M = I - gamma J, scaled, in LU storage order; the table entries and the
+  // saved Jacobian values of four positions are fetched before they are used
   const uint32_t *mp = opaque(NET::mpos);
-#pragma unroll 4
-  for (int t = 0; t < NET::MTM; t++) {
-    const int p = t * BLOCK + c.tid;
-    const uint32_t d = mp[p];
-    const int row = d & 0xff, col = (d >> 8) & 0xff, src = d >> 16;
-    if (row == 0xff) continue;
-    double m = 0.0;
-    if (row == 0xfe) {
-      m = 0.0;  // the zero slot
-    } else if (row == 0xfd) {
-      m = 1.0;  // the one slot
-    } else if (row == NET::NEQ) {
+  auto m_entry = [&](uint32_t d, double jv) -> double {
+    const int row = d & 0xff, col = (d >> 8) & 0xff;
+    if (row == 0xfe) return 0.0;  // the zero slot
+    if (row == 0xfd) return 1.0;  // the one slot
+    double m;
+    if (row == NET::NEQ) {
       m = (col == NET::NEQ) ? 1.0 : -(double)NET::charge[col];
-      m *= S.W[row] / S.W[col];
     } else {
-      if (src != 0xffff) m = -gamma * (double)c.Jg[src];
+      m = -gamma * jv;
       if (row == col) m += 1.0;
-      m *= S.W[row] / S.W[col];
     }
-    LU[p] = (MT)m;
+    return m * S.W[row] * S.Winv[col];
+  };
+  int t0 = 0;
+  for (; t0 + 4 <= NET::MTM; t0 += 4) {
+    uint32_t d[4];
+    double jv[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) d[u] = mp[(t0 + u) * BLOCK + c.tid];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int src = d[u] >> 16;
+      jv[u] = ((d[u] & 0xff) < 0xfd && src != 0xffff) ? (double)c.Jg[src] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+      if ((d[u] & 0xff) != 0xff) LU[(t0 + u) * BLOCK + c.tid] = (MT)m_entry(d[u], jv[u]);
+  }
+  for (; t0 < NET::MTM; t0++) {
+    const uint32_t d = mp[t0 * BLOCK + c.tid];
+    const int src = d >> 16;
+    const double jv = ((d & 0xff) < 0xfd && src != 0xffff) ? (double)c.Jg[src] : 0.0;
+    if ((d & 0xff) != 0xff) LU[t0 * BLOCK + c.tid] = (MT)m_entry(d, jv);
   }
   __syncthreads();
   // head elimination, level by level
   for (int lv = 0; lv < NET::NLEV; lv++) {
-    const int offA = NET::elimA_off[lv], nA = NET::elimA_n[lv];
-#pragma unroll 4
-    for (int t = 0; t < nA; t++) {
-      const uint32_t w = opaque(NET::elimA)[(offA + t) * BLOCK + c.tid];
+    for_items<4>(opaque(NET::elimA), NET::elimA_off[lv], NET::elimA_n[lv], c.tid, [&](uint32_t w) {
       const int kind = w >> 28, tgt = w & 0x3fff, dp = (w >> 14) & 0x3fff;
       if (kind == 1) {
         LU[tgt] = LU[tgt] / LU[dp];       // multiplier
       } else if (kind == 2) {
         LU[tgt] = (MT)1.0 / LU[dp];       // diagonal of the head's U^-1
       }
-    }
+    });
     __syncthreads();
-    const int offB = NET::elimB_off[lv], nB = NET::elimB_n[lv];
     MT acc = (MT)0.0;
-#pragma unroll 8
-    for (int t = 0; t < nB; t++) {
-      const uint64_t w = opaque(NET::elimB)[(size_t)(offB + t) * BLOCK + c.tid];
+    for_items<8>(opaque(NET::elimB), NET::elimB_off[lv], NET::elimB_n[lv], c.tid, [&](uint64_t w) {
       const int tgt = w & 0x3fff, lp = (w >> 14) & 0x3fff, up = (w >> 28) & 0x3fff;
       acc = fma(LU[lp], LU[up], acc);
       if ((w >> 42) & 1) {
         LU[tgt] -= acc;
         acc = (MT)0.0;
       }
-    }
+    });
     __syncthreads();
   }
   if constexpr (NET::NHEAD > 0) {
     // explicit inverses of the head triangles, level by level:
     // target <- -(sum of src1*src2) * LU[scale]
     for (int lv = 0; lv < NET::NINV; lv++) {
-      const int off = NET::inv_off[lv], n = NET::inv_n[lv];
       MT acc = (MT)0.0;
-#pragma unroll 8
-      for (int t = 0; t < n; t++) {
-        const uint64_t w = opaque(NET::inv_tab)[(size_t)(off + t) * BLOCK + c.tid];
+      for_items<8>(opaque(NET::inv_tab), NET::inv_off[lv], NET::inv_n[lv], c.tid, [&](uint64_t w) {
         acc = fma(LU[w & 0x3fff], LU[(w >> 14) & 0x3fff], acc);
         if ((w >> 56) & 1) {
           LU[(w >> 28) & 0x3fff] = -acc * LU[(w >> 42) & 0x3fff];
           acc = (MT)0.0;
         }
-      }
+      });
       __syncthreads();
     }
     schur_mfma(c, LU);
