@@ -47,6 +47,7 @@ constexpr bool kJinLDS = sizeof(MT) == 4;
 // [k*64 + lane] (conflict-free), rather than in registers.
 template <typename MT>
 struct WaveLDS {
+  alignas(16) MT PR[32];           // pivot row of the Gauss-Jordan step
   double X[64];                    // factor vector: y, x_e, 1
   double F[2 * NET::NSLOT + 1];    // fluxes (RHS) or flux partials (Jacobian); last = 0
   double P[65];                    // RHS lane partials; P[64] = 0
@@ -189,31 +190,47 @@ __device__ __forceinline__ void jacobian(Ctx<MT> &c, Bdf<MT> &b) {
 // pivot.  Returns false if singular.
 // ---------------------------------------------------------------------
 template <typename MT>
-__device__ __forceinline__ bool gj_invert(const Lane &ln, MT (&A)[16], int &invp) {
+__device__ __forceinline__ bool gj_invert(const Lane &ln, WaveLDS<MT> LDSA &S, MT (&A)[16], int &invp) {
   bool used = !ln.row_ok;
   invp = ln.i;
+  MT LDSA *prow = S.PR + 16 * ln.hh;  // this half's columns of the pivot row
 #pragma unroll
   for (int k = 0; k < NET::NEQ; k++) {
     const int hk = k >> 4, jk = k & 15;
     const MT colk = A[jk];
-    MT v = (ln.hh == hk && !used) ? fabs(colk) : (MT)-1.0;
-    MT vmax;
-    int p;
-    half_argmax(v, ln.i, hk, vmax, p);
-    if (!(vmax > (MT)0.0)) return false;
-    const MT d = (MT)1.0 / rdlane(colk, p + 32 * hk);
+    const uint32_t key = (ln.hh == hk && !used) ? pivot_key<5>(fabs(colk), ln.i) : 0u;
+    bool ok;
+    const int p = pivot_row<5, MT>(half_max(key, hk), ok);
+    if (!ok) return false;
+    const bool isp = (ln.i == p);
+    // the pivot row goes through LDS: lanes p and p + 32 write their 16
+    // columns, every lane reads back those of its half (one broadcast
+    // address per column)
+    if (isp) {
+#pragma unroll
+      for (int j = 0; j < 16; j++) prow[j] = A[j];
+    }
+    wsync();
+    MT pr[16];
+#pragma unroll
+    for (int j = 0; j < 16; j++) pr[j] = prow[j];
+    const MT piv = S.PR[k];
+    wsync();
+    const MT d = (MT)1.0 / piv;
     const MT oc = other_half(colk);
     const MT fi = (ln.hh == hk) ? colk : oc;
-    const bool isp = (ln.i == p);
-    if (isp) { used = true; invp = k; }
+    used = used || isp;
+    invp = isp ? k : invp;
 #pragma unroll
     for (int j = 0; j < 16; j++) {
-      const MT lo = rdlane(A[j], p), hi = rdlane(A[j], p + 32);
-      const MT pj = (ln.hh ? hi : lo) * d;
-      const bool isk = (ln.hh == hk) && (j == jk);
-      MT nv = isp ? pj : fma(-fi, pj, A[j]);
-      MT kv = isp ? d : -fi * d;
-      A[j] = isk ? kv : nv;
+      const MT pj = pr[j] * d;
+      const MT nv = isp ? pj : fma(-fi, pj, A[j]);
+      if (j == jk) {
+        const MT kv = isp ? d : -fi * d;
+        A[j] = (ln.hh == hk) ? kv : nv;
+      } else {
+        A[j] = nv;
+      }
     }
   }
   return true;
@@ -262,22 +279,22 @@ __device__ __forceinline__ int lsetup(Ctx<MT> &c, BdfU LDSA &u, Bdf<MT> &b, int 
   const double gamma = u.gamma;
   // M = I - gamma J, scaled: W M W^-1 (w = error weights)
   b.wsc = ln.row_ok ? b.ewt : 1.0;
-  if (ln.hh == 0) S.V[ln.i] = b.wsc;
+  if (ln.hh == 0) S.V[ln.i] = 1.0 / b.wsc;
   wsync();
 #pragma unroll
   for (int j = 0; j < 16; j++) {
     const int col = ln.hh * 16 + j;
-    const double wj = S.V[col];
+    const double wjinv = S.V[col];
     double jv;
     if constexpr (kJinLDS<MT>) jv = (double)S.J[j * 64 + ln.L];
     else jv = (double)b.J[j];
     double m = -gamma * jv;
     if (col == ln.i) m += 1.0;
     if (!ln.row_ok) m = (col == ln.i) ? 1.0 : 0.0;
-    b.A[j] = (MT)(m * (b.wsc / wj));
+    b.A[j] = (MT)(m * b.wsc * wjinv);
   }
   wsync();
-  bool ok = gj_invert(ln, b.A, b.invp);
+  bool ok = gj_invert(ln, S, b.A, b.invp);
   return ok ? 0 : 1;
 }
 

@@ -302,8 +302,9 @@ __device__ __forceinline__ void schur_mfma(const Team<double> &c, double LDSA *L
 // the four wavefronts: lane r of every wavefront holds row r, wavefront w
 // columns 16w .. 16w+15 (A4).  Step k: the wavefront that owns column k
 // picks the pivot -- the unused row with the largest |a[r][k]|, ties to the
-// lower row, by DPP arg-max -- and publishes the row index, its reciprocal
-// and the whole column in LDS (double-buffered, so one barrier per step);
+// lower row, by a DPP max over packed value/row keys -- and publishes the
+// row index, its reciprocal and the whole column in LDS (double-buffered,
+// so one barrier per step);
 // every wavefront then updates its 16 columns, reading the pivot row's
 // entries with v_readlane.  Rows are not moved: A4 ends as the inverse in
 // pivot order, invp = step at which this row was the pivot.  Returns false
@@ -323,26 +324,12 @@ __device__ __forceinline__ bool tail_gj(Team<MT> &c, MT (&A4)[16], int &invp) {
       const int buf = k & 1;
       if (c.wave == wk) {
         const MT colk = A4[jk];
-        MT v = used ? (MT)-1.0 : fabs(colk);
-        int idx = c.lane;
-        argmax_step<MT, DPP_XOR1>(v, idx);
-        argmax_step<MT, DPP_XOR2>(v, idx);
-        argmax_step<MT, DPP_HALF_MIRROR>(v, idx);
-        argmax_step<MT, DPP_MIRROR>(v, idx);
-        MT vm = rdlane(v, 0);
-        int p = rdlane(idx, 0);
-#pragma unroll
-        for (int row = 1; row < 4; row++) {
-          const MT v1 = rdlane(v, 16 * row);
-          const int i1 = rdlane(idx, 16 * row);
-          const bool t1 = (v1 > vm) || (v1 == vm && i1 < p);
-          vm = t1 ? v1 : vm;
-          p = t1 ? i1 : p;
-        }
+        bool ok;
+        const int p = pivot_row<6, MT>(wave_max_u(used ? 0u : pivot_key<6>(fabs(colk), c.lane)), ok);
         S.gjcol[buf][c.lane] = colk;
         if (c.lane == 0) {
-          S.gjp[buf] = (vm > (MT)0.0) ? p : -1;
-          S.gjd[buf] = (vm > (MT)0.0) ? (MT)1.0 / rdlane(colk, p) : (MT)0.0;
+          S.gjp[buf] = ok ? p : -1;
+          S.gjd[buf] = ok ? (MT)1.0 / rdlane(colk, p) : (MT)0.0;
         }
       }
       __syncthreads();

@@ -95,29 +95,53 @@ __device__ __forceinline__ double wave_max(double v) {
   return fmax(fmax(rdlane(v, 0), rdlane(v, 16)), fmax(rdlane(v, 32), rdlane(v, 48)));
 }
 
-template <typename T, int CTRL>
-__device__ __forceinline__ void argmax_step(T &v, int &idx) {
-  T ov = dpp<CTRL>(v);
-  int oi = dpp<CTRL>(idx);
-  bool take = (ov > v) || (ov == v && oi < idx);
-  v = take ? ov : v;
-  idx = take ? oi : idx;
+// Pivot search by one unsigned max.  The key of a candidate |a| >= 0 in
+// row idx (< 2^IB) is the value's leading word (FP32: the whole word; FP64:
+// the high word) with its low IB bits replaced by 2^IB - 1 - idx.  For
+// non-negative values the key orders like the value, so the largest key
+// names the largest candidate, ties going to the smaller row; values closer
+// than about 2^(IB-23) relative (FP32) or 2^(IB-20) (FP64) count as ties,
+// which is as good a pivot either way.  Non-candidates pass key 0.
+__device__ __forceinline__ uint32_t lead_word(float a) { return __float_as_uint(a); }
+__device__ __forceinline__ uint32_t lead_word(double a) { return (uint32_t)(__double_as_longlong(a) >> 32); }
+
+template <int IB, typename T>
+__device__ __forceinline__ uint32_t pivot_key(T a, int idx) {
+  constexpr uint32_t M = (1u << IB) - 1;
+  return (lead_word(a) & ~M) | (M - (uint32_t)idx);
 }
 
-// arg-max of v over the 32 lanes of half `h` (uniform), ties -> smaller idx;
-// returns uniform value and index
-template <typename T>
-__device__ __forceinline__ void half_argmax(T v, int idx, int h, T &vmax, int &imax) {
-  argmax_step<T, DPP_XOR1>(v, idx);
-  argmax_step<T, DPP_XOR2>(v, idx);
-  argmax_step<T, DPP_HALF_MIRROR>(v, idx);
-  argmax_step<T, DPP_MIRROR>(v, idx);
-  const int l0 = 32 * h, l1 = 32 * h + 16;
-  T v0 = rdlane(v, l0), v1 = rdlane(v, l1);
-  int i0 = rdlane(idx, l0), i1 = rdlane(idx, l1);
-  bool t1 = (v1 > v0) || (v1 == v0 && i1 < i0);
-  vmax = t1 ? v1 : v0;
-  imax = t1 ? i1 : i0;
+// row of the winning key, and whether its value is a usable pivot (neither
+// zero nor inf/NaN)
+template <int IB, typename T>
+__device__ __forceinline__ int pivot_row(uint32_t kmax, bool &ok) {
+  constexpr uint32_t M = (1u << IB) - 1;
+  constexpr uint32_t LIM = sizeof(T) == 4 ? 0x7f800000u : 0x7ff00000u;
+  ok = (kmax > M) && (kmax < LIM);
+  return (int)(M - (kmax & M));
+}
+
+// max over each 16-lane row (every lane of the row receives it)
+__device__ __forceinline__ uint32_t row_max(uint32_t k) {
+  k = max(k, (uint32_t)dpp<DPP_XOR1>((int)k));
+  k = max(k, (uint32_t)dpp<DPP_XOR2>((int)k));
+  k = max(k, (uint32_t)dpp<DPP_HALF_MIRROR>((int)k));
+  k = max(k, (uint32_t)dpp<DPP_MIRROR>((int)k));
+  return k;
+}
+__device__ __forceinline__ uint32_t rdlane_u(uint32_t v, int l) {
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, l);
+}
+
+// largest key over the 32 lanes of half h (uniform h), result uniform
+__device__ __forceinline__ uint32_t half_max(uint32_t k, int h) {
+  k = row_max(k);
+  return max(rdlane_u(k, 32 * h), rdlane_u(k, 32 * h + 16));
+}
+// largest key over all 64 lanes, result uniform
+__device__ __forceinline__ uint32_t wave_max_u(uint32_t k) {
+  k = row_max(k);
+  return max(max(rdlane_u(k, 0), rdlane_u(k, 16)), max(rdlane_u(k, 32), rdlane_u(k, 48)));
 }
 
 // order LDS traffic of this wavefront (no cross-wave synchronisation: every
