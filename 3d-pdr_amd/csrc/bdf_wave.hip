@@ -34,29 +34,32 @@ static void fill_args(Args &a, const SolveRequest &r) {
   a.counter = r.counter;
 }
 
-template <typename Kern, typename Args>
-static int launch_persistent(Kern kern, const Args &a, int npts, hipStream_t s, int max_blocks) {
+template <typename MT>
+static int launch_persistent(void (*kern)(bdfw_reduced::PointArgs, bdfw_reduced::WaveWS<MT> *),
+                             const bdfw_reduced::PointArgs &a, int npts, hipStream_t s, int max_blocks) {
+  constexpr int WAVES = bdfw_reduced::WAVES;
   int dev = 0, ncu = 0, per_cu = 0;
   if (hipGetDevice(&dev) != hipSuccess) return -1;
   if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return -1;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 256, 0) != hipSuccess) return -1;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64 * WAVES, 0) != hipSuccess) return -1;
   if (per_cu < 1) per_cu = 1;
-  long long want = ((long long)npts + 3) / 4;  // 4 wavefronts per block
+  long long want = ((long long)npts + WAVES - 1) / WAVES;
   long long full = (long long)ncu * per_cu;
   long long nb = want < full ? want : full;
   if (max_blocks > 0 && nb > max_blocks) nb = max_blocks;
   if (nb < 1) nb = 1;
-  hipLaunchKernelGGL(kern, dim3((unsigned)nb), dim3(256), 0, s, a);
+  auto *ws = (bdfw_reduced::WaveWS<MT> *)stream_workspace(s, (size_t)nb * WAVES * sizeof(bdfw_reduced::WaveWS<MT>));
+  if (!ws) return -1;
+  hipLaunchKernelGGL(kern, dim3((unsigned)nb), dim3(64 * WAVES), 0, s, a, ws);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
-
 int launch_wave_solver(int net, const SolveRequest &r, hipStream_t s) {
   switch (net) {
     case NET_REDUCED: {
       bdfw_reduced::PointArgs a;
       fill_args(a, r);
-      if (r.mixed) return launch_persistent(bdfw_reduced::bdf_wave_kernel<float>, a, r.npts, s, r.max_blocks);
-      return launch_persistent(bdfw_reduced::bdf_wave_kernel<double>, a, r.npts, s, r.max_blocks);
+      if (r.mixed) return launch_persistent<float>(bdfw_reduced::bdf_wave_kernel<float>, a, r.npts, s, r.max_blocks);
+      return launch_persistent<double>(bdfw_reduced::bdf_wave_kernel<double>, a, r.npts, s, r.max_blocks);
     }
     default:
       return -2;

@@ -30,31 +30,49 @@ using namespace pdrw;
 #include "bdf_core.h"
 
 constexpr int WAVES = 4;  // wavefronts per block
-// wavefronts per SIMD the register allocation is sized for
-#ifndef PDR_WAVES_PER_EU
-#define PDR_WAVES_PER_EU 2
+
+// Occupancy: the kernel is latency-bound (every step is a chain of
+// dependent cross-lane reductions and LDS round trips), so throughput grows
+// with the wavefronts per SIMD that hide each other's latency, as long as
+// the register budget that buys them does not push the integrator's state
+// into scratch inside the Newton loop.  LDS is kept small enough not to be
+// the tighter limit.
+#ifndef PDR_WPE32
+#define PDR_WPE32 3
 #endif
-
-// The saved Jacobian is read only when the iteration matrix is formed; the
-// FP32 form keeps it in LDS (registers are the scarce resource at two
-// wavefronts per SIMD), the FP64 form in registers (16 more KB of LDS per
-// block would halve its occupancy).
+#ifndef PDR_WPE64
+#define PDR_WPE64 2
+#endif
 template <typename MT>
-constexpr bool kJinLDS = sizeof(MT) == 4;
+constexpr int kWavesPerEU = sizeof(MT) == 4 ? PDR_WPE32 : PDR_WPE64;
 
-// per-wavefront shared memory.  Lane-private arrays that are read once per
-// RHS or per setup (rate constants, the saved Jacobian) live here, indexed
-// [k*64 + lane] (conflict-free), rather than in registers.
+// Per-wavefront workspace in device memory for the saved Jacobian, read
+// only when the iteration matrix is formed without a new Jacobian.  Indexed
+// [k*64 + lane] and only ever read back by the lane that wrote it, so it
+// needs no synchronisation.
+template <typename MT>
+struct WaveWS {
+  MT J[16 * 64];                // the lane's 16 columns
+};
+
+// LDS words (doubles) of the flux region: FP64 fluxes F[0..NSLOT) and the
+// zero F[NSLOT] for the RHS; partials D[0..2*NSLOT) in the matrix type and
+// the zero D[2*NSLOT] for the Jacobian, over the same bytes
+template <typename MT>
+constexpr int kFWords = (NET::NSLOT + 1) > (int)(((2 * NET::NSLOT + 1) * sizeof(MT) + 7) / 8)
+                            ? (NET::NSLOT + 1)
+                            : (int)(((2 * NET::NSLOT + 1) * sizeof(MT) + 7) / 8);
+
+// per-wavefront shared memory
 template <typename MT>
 struct WaveLDS {
   alignas(16) MT PR[32];           // pivot row of the Gauss-Jordan step
   double X[64];                    // factor vector: y, x_e, 1
-  double F[2 * NET::NSLOT + 1];    // fluxes (RHS) or flux partials (Jacobian); last = 0
+  double F[kFWords<MT>];           // fluxes (RHS) or flux partials (Jacobian)
   double P[65];                    // RHS lane partials; P[64] = 0
   double V[64];                    // vector exchange
   double V2[64];
   double Kp[NET::ROUNDS * 64];     // rate * (n_H) of each lane's flux slots
-  MT J[kJinLDS<MT> ? 16 * 64 : 1]; // saved Jacobian (FP32 form): lane's 16 columns
   BdfU u;
 };
 
@@ -86,7 +104,6 @@ struct Bdf {
   double zn[QMAX + 1];
   double ewt, acor, y, ftemp, wsc;
   MT A[16];
-  MT J[kJinLDS<MT> ? 1 : 16];     // saved Jacobian (FP64 form)
   int invp;
 };
 
@@ -96,6 +113,7 @@ struct Ctx {
   Lane ln;
   Tables tb;
   WaveLDS<MT> LDSA *S;
+  WaveWS<MT> *W;
   double rtol, atol;
 
   __device__ __forceinline__ bool lead() const { return ln.L == 0; }
@@ -146,16 +164,17 @@ __device__ __forceinline__ double wave_rhs(Ctx<MT> &c, double y, double &xe_out)
 // hh*16.., including the x_e chain term g_i * charge_j
 // ---------------------------------------------------------------------
 template <typename MT>
-__device__ __forceinline__ void jacobian(Ctx<MT> &c, Bdf<MT> &b) {
+__device__ __forceinline__ void jacobian(Ctx<MT> &c, MT (&jv)[16]) {
   const Lane &ln = c.ln;
   WaveLDS<MT> LDSA &S = *c.S;
+  MT LDSA *D = (MT LDSA *)S.F;  // partials in the matrix type
 #pragma unroll
   for (int r = 0; r < NET::ROUNDS; r++) {
     const int s = r * 64 + ln.L;
     const uint32_t f = c.tb.fab[s];
     const double kp = S.Kp[s];
-    S.F[2 * s] = kp * S.X[(f >> 8) & 0xff];   // d flux / d X[a]
-    S.F[2 * s + 1] = kp * S.X[f & 0xff];      // d flux / d X[b]
+    D[2 * s] = (MT)(kp * S.X[(f >> 8) & 0xff]);   // d flux / d X[a]
+    D[2 * s + 1] = (MT)(kp * S.X[f & 0xff]);      // d flux / d X[b]
   }
   wsync();
   double acc[17];
@@ -166,18 +185,19 @@ __device__ __forceinline__ void jacobian(Ctx<MT> &c, Bdf<MT> &b) {
 #pragma unroll 8
     for (int t = 0; t < NET::JT[j]; t++) {
       uint32_t w = NET::jac_term[(off + t) * 64 + ln.L];
-      a = fma((double)(int16_t)(w >> 16), S.F[w & 0xffff], a);
+      a = fma((double)(int16_t)(w >> 16), (double)D[w & 0xffff], a);
     }
     off += NET::JT[j];
     acc[j] = a;
   }
 #pragma unroll
   for (int j = 0; j < 16; j++) {
-    const MT v = (MT)fma(acc[16], (double)NET::charge[ln.hh * 16 + j], acc[j]);
-    if constexpr (kJinLDS<MT>) S.J[j * 64 + ln.L] = v;
-    else b.J[j] = v;
+    jv[j] = (MT)fma(acc[16], (double)NET::charge[ln.hh * 16 + j], acc[j]);
+    c.W->J[j * 64 + ln.L] = jv[j];
   }
   wsync();
+  // in the FP64 form the partials covered the RHS's zero flux F[NSLOT]
+  if (ln.L == 0) S.F[NET::NSLOT] = 0.0;
 }
 
 // ---------------------------------------------------------------------
@@ -257,6 +277,11 @@ template <typename MT>
 __device__ __forceinline__ int lsetup(Ctx<MT> &c, BdfU LDSA &u, Bdf<MT> &b, int convfail) {
   const Lane &ln = c.ln;
   WaveLDS<MT> LDSA &S = *c.S;
+  // b.A (the old inverse, no longer needed) receives J: the saved one,
+  // fetched first so that its latency overlaps the refresh decision, or a
+  // new one
+#pragma unroll
+  for (int j = 0; j < 16; j++) b.A[j] = c.W->J[j * 64 + ln.L];
   double dgamma = fabs(u.gamma / u.gammap - 1.0);
   bool jbad = (u.nst == 0) || (u.nst > u.nstlj + MSBJ) ||
               ((convfail == CV_FAIL_BAD_J) && (dgamma < DGMAX_J)) || (convfail == CV_FAIL_OTHER);
@@ -272,7 +297,7 @@ __device__ __forceinline__ int lsetup(Ctx<MT> &c, BdfU LDSA &u, Bdf<MT> &b, int 
     double xe = wave_sum(ln.owner ? c.tb.q * b.zn[0] : 0.0);
     if (ln.L == 0) S.X[NET::NEQ] = xe;
     wsync();
-    jacobian(c, b);
+    jacobian(c, b.A);
   } else {
     if (ln.L == 0) u.jcur = 0;
   }
@@ -285,10 +310,7 @@ __device__ __forceinline__ int lsetup(Ctx<MT> &c, BdfU LDSA &u, Bdf<MT> &b, int 
   for (int j = 0; j < 16; j++) {
     const int col = ln.hh * 16 + j;
     const double wjinv = S.V[col];
-    double jv;
-    if constexpr (kJinLDS<MT>) jv = (double)S.J[j * 64 + ln.L];
-    else jv = (double)b.J[j];
-    double m = -gamma * jv;
+    double m = -gamma * (double)b.A[j];
     if (col == ln.i) m += 1.0;
     if (!ln.row_ok) m = (col == ln.i) ? 1.0 : 0.0;
     b.A[j] = (MT)(m * b.wsc * wjinv);
@@ -344,7 +366,8 @@ __device__ __forceinline__ void solve_point(Ctx<MT> &c, const PointArgs &a, int 
   wsync();
   c.S->X[ln.L] = (ln.L == NET::NEQ + 1) ? 1.0 : 0.0;
   if (ln.L == 0) {
-    c.S->F[2 * NET::NSLOT] = 0.0;
+    c.S->F[NET::NSLOT] = 0.0;
+    ((MT LDSA *)c.S->F)[2 * NET::NSLOT] = (MT)0;
     c.S->P[64] = 0.0;
     bdf_init_uniform(u, a.t0, xe0);
   }
@@ -355,10 +378,6 @@ __device__ __forceinline__ void solve_point(Ctx<MT> &c, const PointArgs &a, int 
   b.acor = 0.0; b.y = y0; b.ftemp = 0.0; b.wsc = 1.0; b.invp = ln.i;
 #pragma unroll
   for (int j = 0; j < 16; j++) b.A[j] = (MT)0;
-  if constexpr (!kJinLDS<MT>) {
-#pragma unroll
-    for (int j = 0; j < 16; j++) b.J[j] = (MT)0;
-  }
   b.ewt = 1.0 / (c.rtol * fabs(y0) + c.atol);
 
   double yout = y0;
@@ -387,7 +406,8 @@ __device__ __forceinline__ void solve_point(Ctx<MT> &c, const PointArgs &a, int 
 
 // persistent kernel: every wavefront pulls grid points from a queue
 template <typename MT>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PDR_WAVES_PER_EU))) bdf_wave_kernel(PointArgs a) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWavesPerEU<MT>)))
+bdf_wave_kernel(PointArgs a, WaveWS<MT> *ws) {
   __shared__ BlockLDS<MT> blk;
   for (int k = threadIdx.x; k < NET::TM * 64; k += blockDim.x) blk.rterm[k] = NET::rhs_term[k];
   for (int k = threadIdx.x; k < NET::ROUNDS * 64; k += blockDim.x) blk.fab[k] = NET::slot_fab[k];
@@ -399,6 +419,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PDR_WA
   c.ln.row_ok = c.ln.i < NET::NEQ;
   c.ln.owner = c.ln.row_ok && c.ln.hh == 0;
   c.S = (WaveLDS<MT> LDSA *)&blk.w[threadIdx.x >> 6];
+  c.W = ws + (size_t)blockIdx.x * WAVES + (threadIdx.x >> 6);
   c.rtol = a.rtol;
   c.atol = a.atol;
   c.tb.rbase = NET::row_base[c.ln.L];

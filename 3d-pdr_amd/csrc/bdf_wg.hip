@@ -25,11 +25,11 @@ namespace pdrchem {
 
 namespace {
 
-// Jacobian workspace: one slot of NJ entries per resident workgroup, one
-// buffer per stream (launches on one stream run in order; launches on
-// different streams may overlap and must not share slots), grown on demand
-// and kept for the life of the process
-struct JacWorkspace {
+// Solver workspace (saved Jacobians, per-wavefront rate constants): one
+// buffer per stream, grown on demand and kept for the life of the process.
+// Launches on one stream run in order and may reuse it; launches on
+// different streams may overlap and must not share it.
+struct StreamWorkspace {
   struct Buf {
     void *ptr = nullptr;
     size_t bytes = 0;
@@ -50,7 +50,13 @@ struct JacWorkspace {
     return b.ptr;
   }
 };
-JacWorkspace g_jac;
+StreamWorkspace g_ws;
+
+}  // namespace
+
+void *stream_workspace(hipStream_t s, size_t bytes) { return g_ws.get(s, bytes); }
+
+namespace {
 
 template <typename Args>
 void fill_args(Args &a, const SolveRequest &r) {
@@ -86,7 +92,7 @@ int launch(void (*kern)(Args, MT *), const SolveRequest &r, int nj, hipStream_t 
   long long nb = r.npts < full ? r.npts : full;
   if (r.max_blocks > 0 && nb > r.max_blocks) nb = r.max_blocks;
   if (nb < 1) nb = 1;
-  MT *jac = (MT *)g_jac.get(s, (size_t)nb * nj * sizeof(MT));
+  MT *jac = (MT *)stream_workspace(s, (size_t)nb * nj * sizeof(MT));
   if (!jac) return -1;
   hipLaunchKernelGGL(kern, dim3((unsigned)nb), dim3(256), 0, s, a, jac);
   return hipGetLastError() == hipSuccess ? 0 : -1;

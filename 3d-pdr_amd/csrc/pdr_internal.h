@@ -29,6 +29,10 @@ struct KernelInfo {
   int num_regs, shared_bytes, local_bytes, max_threads, blocks_per_cu;
 };
 
+// device workspace of at least `bytes` for solver launches on stream s
+// (bdf_wg.hip); nullptr on allocation failure
+void *stream_workspace(hipStream_t s, size_t bytes);
+
 // reduced network: one system per wavefront (bdf_wave.hip)
 int launch_wave_solver(int net, const SolveRequest &r, hipStream_t s);
 int wave_solver_info(int net, int mixed, KernelInfo *info);

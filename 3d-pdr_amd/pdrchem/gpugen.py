@@ -107,8 +107,10 @@ class WaveTables:
         nact = len(self.slot_reac)
         self.rounds = max(1, math.ceil(nact / WAVE))
         self.nslot = self.rounds * WAVE
-        # F[2*nslot] is zero in both the flux (RHS) and partials (Jacobian) use
-        self.zero_slot = 2 * self.nslot
+        # the RHS reads fluxes F[0..nslot) (FP64) and the zero F[nslot]; the
+        # Jacobian reads partials D[0..2*nslot) (matrix type) and the zero
+        # D[2*nslot], in the same LDS region
+        self.zero_slot = self.nslot
         reac_slot = {r: s for s, r in enumerate(self.slot_reac)}
         self.slot_fab = []
         for s in range(self.nslot):
