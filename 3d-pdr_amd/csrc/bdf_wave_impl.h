@@ -177,18 +177,32 @@ __device__ __forceinline__ void jacobian(Ctx<MT> &c, MT (&jv)[16]) {
     D[2 * s + 1] = (MT)(kp * S.X[f & 0xff]);      // d flux / d X[b]
   }
   wsync();
+  // the term table lives in global memory (L2): its rounds are fetched in
+  // groups of G ahead of use so that a lane waits for the table once per
+  // group, not once per round; the column of each round is a compile-time
+  // constant after unrolling
+  constexpr int G = 16;
   double acc[17];
-  int off = 0;
 #pragma unroll
-  for (int j = 0; j < 17; j++) {
-    double a = 0.0;
-#pragma unroll 8
-    for (int t = 0; t < NET::JT[j]; t++) {
-      uint32_t w = NET::jac_term[(off + t) * 64 + ln.L];
-      a = fma((double)(int16_t)(w >> 16), (double)D[w & 0xffff], a);
+  for (int j = 0; j < 17; j++) acc[j] = 0.0;
+  const uint32_t *jt = NET::jac_term + ln.L;
+#pragma unroll
+  for (int g = 0; g < NET::JT_TOTAL; g += G) {
+    uint32_t w[G];
+#pragma unroll
+    for (int u = 0; u < G; u++)
+      if (g + u < NET::JT_TOTAL) w[u] = jt[(g + u) * 64];
+    MT dv[G];
+#pragma unroll
+    for (int u = 0; u < G; u++)
+      if (g + u < NET::JT_TOTAL) dv[u] = D[w[u] & 0xffff];
+#pragma unroll
+    for (int u = 0; u < G; u++) {
+      if (g + u < NET::JT_TOTAL) {
+        const int j = NET::jac_col[g + u];
+        acc[j] = fma((double)(int16_t)(w[u] >> 16), (double)dv[u], acc[j]);
+      }
     }
-    off += NET::JT[j];
-    acc[j] = a;
   }
 #pragma unroll
   for (int j = 0; j < 16; j++) {

@@ -241,6 +241,11 @@ class WaveTables:
         for row in self.jac_term:
             for (d, c) in row:
                 flat.append(f"0x{(d & 0xffff) | ((c & 0xffff) << 16):x}u")
+        cols = [j for j in range(len(self.jt)) for _ in range(self.jt[j])]
+        L.append("// column (16 = d/dx_e) of each round of jac_term")
+        L.append(f"constexpr int8_t jac_col[(JT_TOTAL > 0 ? JT_TOTAL : 1)] = {{")
+        L.append(_fmt_list([str(x) for x in cols] if cols else ["0"]))
+        L.append("};")
         L.append("// [column j (16 = d/dx_e)][t][lane]: D index | (int16 coefficient) << 16")
         L.append(f"__device__ const uint32_t jac_term[(JT_TOTAL > 0 ? JT_TOTAL : 1) * 64] = {{")
         L.append(_fmt_list(flat if flat else ["0u"] * 64, 8))
