@@ -66,13 +66,19 @@ constexpr int kFWords = (NET::NSLOT + 1) > (int)(((2 * NET::NSLOT + 1) * sizeof(
 // per-wavefront shared memory
 template <typename MT>
 struct WaveLDS {
-  alignas(16) MT PR[32];           // pivot row of the Gauss-Jordan step
   double X[64];                    // factor vector: y, x_e, 1
   double F[kFWords<MT>];           // fluxes (RHS) or flux partials (Jacobian)
-  double P[65];                    // RHS lane partials; P[64] = 0
-  double V[64];                    // vector exchange
-  double V2[64];
   double Kp[NET::ROUNDS * 64];     // rate * (n_H) of each lane's flux slots
+  // used by one phase at a time: the RHS (P), the M build and the solve
+  // (V, V2), the Gauss-Jordan steps (PR)
+  union {
+    struct {
+      double V[64];                // vector exchange
+      double V2[64];
+    };
+    double P[64];                  // RHS lane partials
+    alignas(16) MT PR[32];         // pivot row of the Gauss-Jordan step
+  };
   BdfU u;
 };
 
@@ -154,7 +160,7 @@ __device__ __forceinline__ double wave_rhs(Ctx<MT> &c, double y, double &xe_out)
   wsync();
   double yd = 0.0;
 #pragma unroll
-  for (int u = 0; u < NET::PM; u++) yd += (u < c.tb.rnl) ? S.P[min(c.tb.rbase + u, 64)] : 0.0;
+  for (int u = 0; u < NET::PM; u++) yd += (u < c.tb.rnl) ? S.P[min(c.tb.rbase + u, 63)] : 0.0;
   wsync();
   return yd;
 }
@@ -382,7 +388,6 @@ __device__ __forceinline__ void solve_point(Ctx<MT> &c, const PointArgs &a, int 
   if (ln.L == 0) {
     c.S->F[NET::NSLOT] = 0.0;
     ((MT LDSA *)c.S->F)[2 * NET::NSLOT] = (MT)0;
-    c.S->P[64] = 0.0;
     bdf_init_uniform(u, a.t0, xe0);
   }
   wsync();
