@@ -153,6 +153,37 @@ __device__ __forceinline__ void for_items(const W *tab, int off, int n, int tid,
   for (; t < n; t++) f(tab[(size_t)(off + t) * BLOCK + tid]);
 }
 
+// As for_items, and the LDS operands of a group's items are fetched by
+// gather(w) for the whole group before consume(w, g) runs on them in order,
+// so that a lane waits for LDS once per group as well.  Valid only where no
+// item's consume writes what another item of the same call gathers: the
+// elimination, inverse and solve tables are level-scheduled so that within
+// one call no target is also a source (which concurrent lanes need anyway).
+template <int U, typename W, typename G, typename C>
+__device__ __forceinline__ void for_items_gc(const W *tab, int off, int n, int tid, G &&gather, C &&consume) {
+  using V = decltype(gather(W()));
+  int t = 0;
+  for (; t + U <= n; t += U) {
+    W w[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) w[u] = tab[(size_t)(off + t + u) * BLOCK + tid];
+    V g[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) g[u] = gather(w[u]);
+#pragma unroll
+    for (int u = 0; u < U; u++) consume(w[u], g[u]);
+  }
+  for (; t < n; t++) {
+    const W w = tab[(size_t)(off + t) * BLOCK + tid];
+    consume(w, gather(w));
+  }
+}
+
+template <typename MT>
+struct Pair {
+  MT a, b;
+};
+
 // the region as LU storage (positions: [0, NHV) head entries, NHV the zero
 // slot, TBASE + r*NT + c the tail block) and as the flux arrays
 template <typename MT>
@@ -193,12 +224,11 @@ __device__ __forceinline__ double wg_rhs(Team<MT> &c, double y, double &xe_out) 
 #pragma unroll
   for (int p = 0; p < NET::VL; p++) {
     double acc = 0.0;
-#pragma unroll 6
-    for (int t = 0; t < NET::TM; t++) {
-      const uint32_t w = rt[(p * NET::TM + t) * BLOCK + c.tid];
-      const int coef = (int)(w << 16) >> 28;  // signed 4-bit field at bit 12
-      acc = fma((double)coef, F[w & 0xfff], acc);
-    }
+    for_items_gc<6>(rt, p * NET::TM, NET::TM, c.tid, [&](uint16_t w) { return F[w & 0xfff]; },
+                    [&](uint16_t w, double f) {
+                      const int coef = (int)((uint32_t)w << 16) >> 28;  // signed 4-bit field at bit 12
+                      acc = fma((double)coef, f, acc);
+                    });
     S.P[p * BLOCK + c.tid] = acc;
   }
   __syncthreads();
@@ -224,14 +254,15 @@ __device__ __forceinline__ void wg_jacobian(Team<MT> &c) {
   if (c.tid == 0) D[2 * NET::NSLOT] = 0.0;
   __syncthreads();
   double acc = 0.0;
-  for_items<8>(opaque(NET::jac_term), 0, NET::JTM, c.tid, [&](uint32_t w) {
-    const int coef = (int)(w << 15) >> 28;          // signed 4-bit field at bit 13
-    acc = fma((double)coef, D[w & 0x1fff], acc);
-    if ((w >> 17) & 1) {
-      c.Jg[w >> 18] = (MT)acc;
-      acc = 0.0;
-    }
-  });
+  for_items_gc<8>(opaque(NET::jac_term), 0, NET::JTM, c.tid, [&](uint32_t w) { return D[w & 0x1fff]; },
+                  [&](uint32_t w, double d) {
+                    const int coef = (int)(w << 15) >> 28;  // signed 4-bit field at bit 13
+                    acc = fma((double)coef, d, acc);
+                    if ((w >> 17) & 1) {
+                      c.Jg[w >> 18] = (MT)acc;
+                      acc = 0.0;
+                    }
+                  });
   __syncthreads();
 }
 
@@ -392,64 +423,70 @@ __device__ __forceinline__ int wg_setup(Team<MT> &c, BdfU LDSA &u, BdfWG<MT> &b,
     S.Winv[NET::NEQ] = 1.0 / we;
   }
   __syncthreads();
-  // M = I - gamma J, scaled, in LU storage order; the table entries and the
-  // saved Jacobian values of four positions are fetched before they are used
+  // M = I - gamma J, scaled, in LU storage order.  Position entry d: row,
+  // column, source of J in the saved Jacobian; the bordered row holds the
+  // charges.  The table entries and the J values or charges of a group of
+  // positions are fetched before they are used.
   const uint32_t *mp = opaque(NET::mpos);
+  auto m_value = [&](uint32_t d) -> double {
+    const int row = d & 0xff, col = (d >> 8) & 0xff, src = d >> 16;
+    if (row == NET::NEQ) return (col == NET::NEQ) ? 0.0 : (double)NET::charge[col];
+    return (row < 0xfd && src != 0xffff) ? (double)c.Jg[src] : 0.0;
+  };
   auto m_entry = [&](uint32_t d, double jv) -> double {
     const int row = d & 0xff, col = (d >> 8) & 0xff;
     if (row == 0xfe) return 0.0;  // the zero slot
     if (row == 0xfd) return 1.0;  // the one slot
     double m;
     if (row == NET::NEQ) {
-      m = (col == NET::NEQ) ? 1.0 : -(double)NET::charge[col];
+      m = (col == NET::NEQ) ? 1.0 : -jv;
     } else {
       m = -gamma * jv;
       if (row == col) m += 1.0;
     }
     return m * S.W[row] * S.Winv[col];
   };
+  constexpr int U = 8;
   int t0 = 0;
-  for (; t0 + 4 <= NET::MTM; t0 += 4) {
-    uint32_t d[4];
-    double jv[4];
+  for (; t0 + U <= NET::MTM; t0 += U) {
+    uint32_t d[U];
+    double jv[U];
 #pragma unroll
-    for (int u = 0; u < 4; u++) d[u] = mp[(t0 + u) * BLOCK + c.tid];
+    for (int u = 0; u < U; u++) d[u] = mp[(t0 + u) * BLOCK + c.tid];
 #pragma unroll
-    for (int u = 0; u < 4; u++) {
-      const int src = d[u] >> 16;
-      jv[u] = ((d[u] & 0xff) < 0xfd && src != 0xffff) ? (double)c.Jg[src] : 0.0;
-    }
+    for (int u = 0; u < U; u++) jv[u] = m_value(d[u]);
 #pragma unroll
-    for (int u = 0; u < 4; u++)
+    for (int u = 0; u < U; u++)
       if ((d[u] & 0xff) != 0xff) LU[(t0 + u) * BLOCK + c.tid] = (MT)m_entry(d[u], jv[u]);
   }
   for (; t0 < NET::MTM; t0++) {
     const uint32_t d = mp[t0 * BLOCK + c.tid];
-    const int src = d >> 16;
-    const double jv = ((d & 0xff) < 0xfd && src != 0xffff) ? (double)c.Jg[src] : 0.0;
-    if ((d & 0xff) != 0xff) LU[t0 * BLOCK + c.tid] = (MT)m_entry(d, jv);
+    if ((d & 0xff) != 0xff) LU[t0 * BLOCK + c.tid] = (MT)m_entry(d, m_value(d));
   }
   __syncthreads();
   // head elimination, level by level
   for (int lv = 0; lv < NET::NLEV; lv++) {
-    for_items<4>(opaque(NET::elimA), NET::elimA_off[lv], NET::elimA_n[lv], c.tid, [&](uint32_t w) {
-      const int kind = w >> 28, tgt = w & 0x3fff, dp = (w >> 14) & 0x3fff;
-      if (kind == 1) {
-        LU[tgt] = LU[tgt] / LU[dp];       // multiplier
-      } else if (kind == 2) {
-        LU[tgt] = (MT)1.0 / LU[dp];       // diagonal of the head's U^-1
-      }
-    });
+    for_items_gc<4>(opaque(NET::elimA), NET::elimA_off[lv], NET::elimA_n[lv], c.tid,
+                    [&](uint32_t w) { return Pair<MT>{LU[w & 0x3fff], LU[(w >> 14) & 0x3fff]}; },
+                    [&](uint32_t w, Pair<MT> g) {
+                      const int kind = w >> 28, tgt = w & 0x3fff;
+                      if (kind == 1) {
+                        LU[tgt] = g.a / g.b;            // multiplier
+                      } else if (kind == 2) {
+                        LU[tgt] = (MT)1.0 / g.b;        // diagonal of the head's U^-1
+                      }
+                    });
     __syncthreads();
     MT acc = (MT)0.0;
-    for_items<8>(opaque(NET::elimB), NET::elimB_off[lv], NET::elimB_n[lv], c.tid, [&](uint64_t w) {
-      const int tgt = w & 0x3fff, lp = (w >> 14) & 0x3fff, up = (w >> 28) & 0x3fff;
-      acc = fma(LU[lp], LU[up], acc);
-      if ((w >> 42) & 1) {
-        LU[tgt] -= acc;
-        acc = (MT)0.0;
-      }
-    });
+    for_items_gc<8>(opaque(NET::elimB), NET::elimB_off[lv], NET::elimB_n[lv], c.tid,
+                    [&](uint64_t w) { return Pair<MT>{LU[(w >> 14) & 0x3fff], LU[(w >> 28) & 0x3fff]}; },
+                    [&](uint64_t w, Pair<MT> g) {
+                      acc = fma(g.a, g.b, acc);
+                      if ((w >> 42) & 1) {
+                        LU[w & 0x3fff] -= acc;
+                        acc = (MT)0.0;
+                      }
+                    });
     __syncthreads();
   }
   if constexpr (NET::NHEAD > 0) {
@@ -457,13 +494,15 @@ __device__ __forceinline__ int wg_setup(Team<MT> &c, BdfU LDSA &u, BdfWG<MT> &b,
     // target <- -(sum of src1*src2) * LU[scale]
     for (int lv = 0; lv < NET::NINV; lv++) {
       MT acc = (MT)0.0;
-      for_items<8>(opaque(NET::inv_tab), NET::inv_off[lv], NET::inv_n[lv], c.tid, [&](uint64_t w) {
-        acc = fma(LU[w & 0x3fff], LU[(w >> 14) & 0x3fff], acc);
-        if ((w >> 56) & 1) {
-          LU[(w >> 28) & 0x3fff] = -acc * LU[(w >> 42) & 0x3fff];
-          acc = (MT)0.0;
-        }
-      });
+      for_items_gc<8>(opaque(NET::inv_tab), NET::inv_off[lv], NET::inv_n[lv], c.tid,
+                      [&](uint64_t w) { return Pair<MT>{LU[w & 0x3fff], LU[(w >> 14) & 0x3fff]}; },
+                      [&](uint64_t w, Pair<MT> g) {
+                        acc = fma(g.a, g.b, acc);
+                        if ((w >> 56) & 1) {
+                          LU[(w >> 28) & 0x3fff] = -acc * LU[(w >> 42) & 0x3fff];
+                          acc = (MT)0.0;
+                        }
+                      });
       __syncthreads();
     }
     schur_mfma(c, LU);
@@ -489,13 +528,9 @@ __device__ __forceinline__ int wg_setup(Team<MT> &c, BdfU LDSA &u, BdfWG<MT> &b,
 template <typename MT>
 __device__ __forceinline__ void spmv_partials(Team<MT> &c, const uint32_t *tab_, int rounds,
                                               const MT LDSA *LU) {
-  const uint32_t *tab = opaque(tab_);
   double acc = 0.0;
-#pragma unroll 8
-  for (int t = 0; t < rounds; t++) {
-    const uint32_t w = tab[t * BLOCK + c.tid];
-    acc = fma((double)LU[w & 0x3fff], c.S->B[w >> 14], acc);
-  }
+  for_items_gc<8>(opaque(tab_), 0, rounds, c.tid, [&](uint32_t w) { return Pair<double>{(double)LU[w & 0x3fff], c.S->B[w >> 14]}; },
+                  [&](uint32_t, Pair<double> g) { acc = fma(g.a, g.b, acc); });
   c.S->P[c.tid] = acc;
 }
 
