@@ -19,7 +19,12 @@
 //   double solve(const BdfU&, const State&, double r)   M^-1 r, own row
 //   double rtol, atol
 // State holds the lane's n-vector entries (zn[0..5], ewt, acor, y, ftemp,
-// wsc) plus the team's matrix registers.
+// wsc) plus the team's matrix registers.  The higher Nordsieck entries
+// zn[2..QMAX] are touched only by the step's bookkeeping (predict, restore,
+// rescale, order change, completion, interpolation), not by the Newton
+// loop; a State may keep them outside registers between those uses:
+// hist_in() brings them into zn[], hist_out() puts them back (both may do
+// nothing).
 //
 // Every update of the shared BdfU record is made by the leader between two
 // sync() calls, after every lane has read what it needs: lanes of different
@@ -108,6 +113,7 @@ __device__ __forceinline__ void rescale(T &c, BdfU LDSA &u, S &b) {
   const int q = u.q;
   const double h = u.hscale * eta;
   double factor = eta;
+  b.hist_in();
 #pragma unroll
   for (int j = 1; j <= QMAX; j++) {
     if (j <= q) {
@@ -115,6 +121,7 @@ __device__ __forceinline__ void rescale(T &c, BdfU LDSA &u, S &b) {
       factor *= eta;
     }
   }
+  b.hist_out();
   c.sync();
   if (c.lead()) {
     u.h = h;
@@ -128,11 +135,13 @@ template <class T, class S>
 __device__ __forceinline__ void predict(T &c, BdfU LDSA &u, S &b) {
   const double tn = u.tn + u.h;
   const int q = u.q;
+  b.hist_in();
 #pragma unroll
   for (int k = 1; k <= QMAX; k++)
 #pragma unroll
     for (int j = QMAX; j >= k; j--)
       if (j <= q && k <= q) b.zn[j - 1] += b.zn[j];
+  b.hist_out();
   c.sync();
   if (c.lead()) u.tn = tn;
   c.sync();
@@ -141,11 +150,13 @@ __device__ __forceinline__ void predict(T &c, BdfU LDSA &u, S &b) {
 template <class T, class S>
 __device__ __forceinline__ void restore(T &c, BdfU LDSA &u, S &b, double saved_t) {
   const int q = u.q;
+  b.hist_in();
 #pragma unroll
   for (int k = 1; k <= QMAX; k++)
 #pragma unroll
     for (int j = QMAX; j >= k; j--)
       if (j <= q && k <= q) b.zn[j - 1] -= b.zn[j];
+  b.hist_out();
   c.sync();
   if (c.lead()) u.tn = saved_t;
   c.sync();
@@ -310,6 +321,7 @@ __device__ __forceinline__ void adjust_order(BdfU LDSA &u, S &b, int deltaq) {
   const int q = u.q;
   double l[QMAX + 1];
   if ((q == 2) && (deltaq != 1)) return;
+  b.hist_in();
   if (deltaq == 1) {
 #pragma unroll
     for (int i = 0; i <= QMAX; i++) l[i] = 0.0;
@@ -362,6 +374,7 @@ __device__ __forceinline__ void adjust_order(BdfU LDSA &u, S &b, int deltaq) {
     for (int j = 2; j < QMAX; j++)
       if (j < q) b.zn[j] -= l[j] * zq;
   }
+  b.hist_out();
 }
 
 // local error test; returns 0 (pass), TRY_AGAIN or a CVODE failure flag
@@ -417,11 +430,13 @@ template <class T, class S>
 __device__ __forceinline__ void complete_step(T &c, BdfU LDSA &u, S &b) {
   const int q = u.q;
   const double h = u.h;
+  b.hist_in();
 #pragma unroll
   for (int j = 0; j <= QMAX; j++)
     if (j <= q) b.zn[j] += u.l[j] * b.acor;
   const int qwait = u.qwait - 1;
   if ((qwait == 1) && (q != QMAX)) b.zn[QMAX] = b.acor;
+  b.hist_out();
   const double tq5 = u.tq[5];
   const int nst = u.nst + 1;
   c.sync();
@@ -467,6 +482,7 @@ __device__ __forceinline__ void prepare_next_step(T &c, BdfU LDSA &u, S &b, doub
       set_eta_vals(eta, hprime, h, etamax);
     } else {
       qwait = 2;
+      b.hist_in();
       double etaqm1 = 0.0, etaqp1 = 0.0;
       if (q > 1) {
         double ddn = wrms(c, selz(b.zn, q), b.ewt) * u.tq[1];
@@ -491,6 +507,7 @@ __device__ __forceinline__ void prepare_next_step(T &c, BdfU LDSA &u, S &b, doub
         qprime = q + 1;
         b.zn[QMAX] = b.acor;
       }
+      b.hist_out();
       set_eta_vals(eta, hprime, h, etamax);
     }
   }
@@ -600,9 +617,10 @@ __device__ __forceinline__ int hin(T &c, BdfU LDSA &u, S &b, double tout) {
 }
 
 template <class S>
-__device__ __forceinline__ double get_dky(const BdfU LDSA &u, const S &b, double t) {
+__device__ __forceinline__ double get_dky(const BdfU LDSA &u, S &b, double t) {
   const double s = (t - u.tn) / u.h;
   const int q = u.q;
+  b.hist_in();
   double v = selz(b.zn, q);
 #pragma unroll
   for (int j = QMAX - 1; j >= 0; j--)
@@ -688,7 +706,7 @@ __device__ __forceinline__ int integrate(T &c, BdfU LDSA &u, S &b, const PointAr
       yout = get_dky(u, b, tout);
       done = true;
     }
-    long long nstloc = 0;
+    int nstloc = 0;
     while (!done) {
       if (u.nst > 0) b.ewt = 1.0 / (c.rtol * fabs(b.zn[0]) + c.atol);
       if (nstloc >= a.mxstep) {

@@ -69,6 +69,7 @@ struct WaveLDS {
   double X[64];                    // factor vector: y, x_e, 1
   double F[kFWords<MT>];           // fluxes (RHS) or flux partials (Jacobian)
   double Kp[NET::ROUNDS * 64];     // rate * (n_H) of each lane's flux slots
+  double Zh[(QMAX - 1) * 64];      // Nordsieck entries zn[2..QMAX] of each lane
   // used by one phase at a time: the RHS (P), the M build and the solve
   // (V, V2), the Gauss-Jordan steps (PR)
   union {
@@ -111,6 +112,18 @@ struct Bdf {
   double ewt, acor, y, ftemp, wsc;
   MT A[16];
   int invp;
+  // zn[2..QMAX] live in LDS between the step's bookkeeping phases (the
+  // Newton loop needs only zn[0], zn[1]): registers the compiler would
+  // otherwise spill to scratch around every step
+  double LDSA *zh;  // zh[(j - 2) * 64] = zn[j] of this lane
+  __device__ __forceinline__ void hist_in() {
+#pragma unroll
+    for (int j = 2; j <= QMAX; j++) zn[j] = zh[(j - 2) * 64];
+  }
+  __device__ __forceinline__ void hist_out() {
+#pragma unroll
+    for (int j = 2; j <= QMAX; j++) zh[(j - 2) * 64] = zn[j];
+  }
 };
 
 // the wavefront as the integrator's team (see bdf_core.h)
@@ -187,7 +200,7 @@ __device__ __forceinline__ void jacobian(Ctx<MT> &c, MT (&jv)[16]) {
   // groups of G ahead of use so that a lane waits for the table once per
   // group, not once per round; the column of each round is a compile-time
   // constant after unrolling
-  constexpr int G = 16;
+  constexpr int G = 8;
   double acc[17];
 #pragma unroll
   for (int j = 0; j < 17; j++) acc[j] = 0.0;
@@ -391,9 +404,11 @@ __device__ __forceinline__ void solve_point(Ctx<MT> &c, const PointArgs &a, int 
     bdf_init_uniform(u, a.t0, xe0);
   }
   wsync();
+  b.zh = c.S->Zh + ln.L;
   b.zn[0] = y0;
 #pragma unroll
   for (int j = 1; j <= QMAX; j++) b.zn[j] = 0.0;
+  b.hist_out();
   b.acor = 0.0; b.y = y0; b.ftemp = 0.0; b.wsc = 1.0; b.invp = ln.i;
 #pragma unroll
   for (int j = 0; j < 16; j++) b.A[j] = (MT)0;

@@ -85,6 +85,9 @@ struct BdfWG {
   double ewt, acor, y, ftemp, wsc;
   MT A4[16];                 // inverse row = lane, columns 16*wave .. +15 (pivot order)
   int invp;                  // wavefront 0: step at which tail row `lane` was the pivot
+  // the whole Nordsieck history stays in registers
+  __device__ __forceinline__ void hist_in() {}
+  __device__ __forceinline__ void hist_out() {}
 };
 
 template <typename MT>
