@@ -79,8 +79,8 @@ void fill_args(Args &a, const SolveRequest &r) {
   a.counter = r.counter;
 }
 
-template <typename MT, typename Args>
-int launch(void (*kern)(Args, MT *), const SolveRequest &r, int nj, hipStream_t s) {
+template <typename Args>
+int launch(void (*kern)(Args, unsigned char *), const SolveRequest &r, size_t slot_bytes, hipStream_t s) {
   Args a;
   fill_args(a, r);
   int dev = 0, ncu = 0, per_cu = 0;
@@ -92,9 +92,9 @@ int launch(void (*kern)(Args, MT *), const SolveRequest &r, int nj, hipStream_t 
   long long nb = r.npts < full ? r.npts : full;
   if (r.max_blocks > 0 && nb > r.max_blocks) nb = r.max_blocks;
   if (nb < 1) nb = 1;
-  MT *jac = (MT *)stream_workspace(s, (size_t)nb * nj * sizeof(MT));
-  if (!jac) return -1;
-  hipLaunchKernelGGL(kern, dim3((unsigned)nb), dim3(256), 0, s, a, jac);
+  auto *ws = (unsigned char *)stream_workspace(s, (size_t)nb * slot_bytes);
+  if (!ws) return -1;
+  hipLaunchKernelGGL(kern, dim3((unsigned)nb), dim3(256), 0, s, a, ws);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -117,11 +117,11 @@ int info_of(Kern fn, KernelInfo *info) {
 int launch_wg_solver(int net, const SolveRequest &r, hipStream_t s) {
   switch (net) {
     case NET_FULL:
-      return r.mixed ? launch(bdfg_full::bdf_wg_kernel<float>, r, pdrwg_full::NJ, s)
-                     : launch(bdfg_full::bdf_wg_kernel<double>, r, pdrwg_full::NJ, s);
+      return r.mixed ? launch(bdfg_full::bdf_wg_kernel<float>, r, bdfg_full::Slot<float>::bytes, s)
+                     : launch(bdfg_full::bdf_wg_kernel<double>, r, bdfg_full::Slot<double>::bytes, s);
     case NET_MYNETWORK:
-      return r.mixed ? launch(bdfg_mynetwork::bdf_wg_kernel<float>, r, pdrwg_mynetwork::NJ, s)
-                     : launch(bdfg_mynetwork::bdf_wg_kernel<double>, r, pdrwg_mynetwork::NJ, s);
+      return r.mixed ? launch(bdfg_mynetwork::bdf_wg_kernel<float>, r, bdfg_mynetwork::Slot<float>::bytes, s)
+                     : launch(bdfg_mynetwork::bdf_wg_kernel<double>, r, bdfg_mynetwork::Slot<double>::bytes, s);
     default:
       return -2;
   }

@@ -85,9 +85,28 @@ struct BdfWG {
   double ewt, acor, y, ftemp, wsc;
   MT A4[16];                 // inverse row = lane, columns 16*wave .. +15 (pivot order)
   int invp;                  // wavefront 0: step at which tail row `lane` was the pivot
-  // the whole Nordsieck history stays in registers
-  __device__ __forceinline__ void hist_in() {}
-  __device__ __forceinline__ void hist_out() {}
+  // zn[2..QMAX] live in the workgroup's device slot between the step's
+  // bookkeeping phases (the Newton loop needs only zn[0], zn[1])
+  double *zh;                // zh[(j - 2) * BLOCK] = zn[j] of this lane
+  __device__ __forceinline__ void hist_in() {
+#pragma unroll
+    for (int j = 2; j <= QMAX; j++) zn[j] = zh[(j - 2) * BLOCK];
+  }
+  __device__ __forceinline__ void hist_out() {
+#pragma unroll
+    for (int j = 2; j <= QMAX; j++) zh[(j - 2) * BLOCK] = zn[j];
+  }
+};
+
+// Per-workgroup slot of the device workspace: the saved Jacobian, the lanes'
+// slot rate constants and their Nordsieck entries zn[2..QMAX].  Everything
+// but the Jacobian is lane-private (written and read by the same lane).
+template <typename MT>
+struct Slot {
+  static constexpr size_t jac_bytes = ((size_t)NET::NJ * sizeof(MT) + 255) / 256 * 256;
+  static constexpr size_t kp_off = jac_bytes;
+  static constexpr size_t zh_off = kp_off + (size_t)NET::FR * BLOCK * sizeof(double);
+  static constexpr size_t bytes = zh_off + (size_t)(QMAX - 1) * BLOCK * sizeof(double);
 };
 
 template <typename MT>
@@ -95,12 +114,13 @@ struct Team {
   int tid, lane, wave;
   bool own_;                 // tid < NEQ
   double q;                  // charge of row tid
-  double Kp[NET::FR];        // rate * (n_H) of this lane's flux slots
-  uint32_t fab[NET::FR];
+  double *Kp;                // rate * (n_H) of this lane's flux slots: Kp[r * BLOCK]
   int rbase, rnl;            // RHS partial run of row tid
   int red_buf;
   WgLDS<MT> LDSA *S;
   MT *Jg;                    // device-memory Jacobian slot [NJ]
+  const uint32_t *fab;       // slot factor table, this lane's column: fab[r * BLOCK]
+  double *zh;                // this lane's zn[2..QMAX] in the device slot
   double rtol, atol;
 
   __device__ __forceinline__ bool lead() const { return tid == 0; }
@@ -219,8 +239,8 @@ __device__ __forceinline__ double wg_rhs(Team<MT> &c, double y, double &xe_out) 
   __syncthreads();
 #pragma unroll
   for (int r = 0; r < NET::FR; r++) {
-    const uint32_t f = c.fab[r];
-    F[r * BLOCK + c.tid] = c.Kp[r] * S.X[f & 0xff] * S.X[(f >> 8) & 0xff];
+    const uint32_t f = c.fab[r * BLOCK];
+    F[r * BLOCK + c.tid] = c.Kp[r * BLOCK] * S.X[f & 0xff] * S.X[(f >> 8) & 0xff];
   }
   __syncthreads();
   const uint16_t *rt = opaque(NET::rhs_term);
@@ -249,10 +269,11 @@ __device__ __forceinline__ void wg_jacobian(Team<MT> &c) {
   double LDSA *D = regionD(c.S);
 #pragma unroll
   for (int r = 0; r < NET::FR; r++) {
-    const uint32_t f = c.fab[r];
+    const uint32_t f = c.fab[r * BLOCK];
     const int s = r * BLOCK + c.tid;
-    D[2 * s] = c.Kp[r] * S.X[(f >> 8) & 0xff];      // d flux / d X[a]
-    D[2 * s + 1] = c.Kp[r] * S.X[f & 0xff];         // d flux / d X[b]
+    const double kp = c.Kp[r * BLOCK];
+    D[2 * s] = kp * S.X[(f >> 8) & 0xff];      // d flux / d X[a]
+    D[2 * s + 1] = kp * S.X[f & 0xff];         // d flux / d X[b]
   }
   if (c.tid == 0) D[2 * NET::NSLOT] = 0.0;
   __syncthreads();
@@ -634,7 +655,7 @@ __device__ __forceinline__ void wg_solve_point(Team<MT> &c, const PointArgs &a, 
   for (int r = 0; r < NET::FR; r++) {
     const int re = NET::slot_reac[r * BLOCK + c.tid];
     const double k = (re >= 0) ? rate[re] : 0.0;
-    c.Kp[r] = ((c.fab[r] >> 16) & 1) ? k * nH : k;
+    c.Kp[r * BLOCK] = ((c.fab[r * BLOCK] >> 16) & 1) ? k * nH : k;
   }
   double *ab = a.abundance + (size_t)sys * a.ab_stride;
   const double y0 = c.own_ ? ab[c.tid] : 0.0;
@@ -646,9 +667,11 @@ __device__ __forceinline__ void wg_solve_point(Team<MT> &c, const PointArgs &a, 
     bdf_init_uniform(u, a.t0, xe0);
   }
   __syncthreads();
+  b.zh = c.zh;
   b.zn[0] = y0;
 #pragma unroll
   for (int j = 1; j <= QMAX; j++) b.zn[j] = 0.0;
+  b.hist_out();
   b.acor = 0.0; b.y = y0; b.ftemp = 0.0; b.wsc = 1.0; b.invp = c.lane;
 #pragma unroll
   for (int j = 0; j < 16; j++) b.A4[j] = (MT)0;
@@ -684,7 +707,7 @@ __device__ __forceinline__ void wg_solve_point(Team<MT> &c, const PointArgs &a, 
 // sized for two wavefronts per SIMD
 template <typename MT>
 __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(MT) == 4 ? 2 : 1)))
-bdf_wg_kernel(PointArgs a, MT *jac) {
+bdf_wg_kernel(PointArgs a, unsigned char *ws) {
   __shared__ WgLDS<MT> lds;
   __shared__ unsigned int next;
   Team<MT> c;
@@ -695,11 +718,13 @@ bdf_wg_kernel(PointArgs a, MT *jac) {
   c.q = (double)NET::charge[c.tid];
   c.red_buf = 0;
   c.S = (WgLDS<MT> LDSA *)&lds;
-  c.Jg = jac + (size_t)blockIdx.x * NET::NJ;
+  unsigned char *slot = ws + (size_t)blockIdx.x * Slot<MT>::bytes;
+  c.Jg = (MT *)slot;
+  c.Kp = (double *)(slot + Slot<MT>::kp_off) + c.tid;
+  c.zh = (double *)(slot + Slot<MT>::zh_off) + c.tid;
   c.rtol = a.rtol;
   c.atol = a.atol;
-#pragma unroll
-  for (int r = 0; r < NET::FR; r++) c.fab[r] = NET::slot_fab[r * BLOCK + c.tid];
+  c.fab = NET::slot_fab + c.tid;
   c.rbase = (c.tid < NET::NEQ) ? NET::row_base[c.tid] : 0;
   c.rnl = (c.tid < NET::NEQ) ? NET::row_nl[c.tid] : 0;
   for (;;) {
