@@ -706,26 +706,27 @@ __device__ __forceinline__ int integrate(T &c, BdfU LDSA &u, S &b, const PointAr
       yout = get_dky(u, b, tout);
       done = true;
     }
-    int nstloc = 0;
+    // the step loop carries only the step count and the exit reason; the
+    // output variables are set once it ends (keeps them out of the loop's
+    // register pressure)
+    int nstloc = 0, why = 0;  // why: 1 = tout reached, < 0 = failure flag
     while (!done) {
       if (u.nst > 0) b.ewt = 1.0 / (c.rtol * fabs(b.zn[0]) + c.atol);
-      if (nstloc >= a.mxstep) {
-        flag = -1; t = u.tn; yout = b.zn[0]; break;
-      }
+      if (nstloc >= a.mxstep) { why = -1; break; }
       double nrm = wrms(c, b.zn[0], b.ewt);
-      if (UROUND * nrm > 1.0) {
-        flag = -2; t = u.tn; yout = b.zn[0]; break;
-      }
+      if (UROUND * nrm > 1.0) { why = -2; break; }
       int kflag = bdf_step(c, u, b);
-      if (kflag != 0) {
-        flag = kflag; t = u.tn; yout = b.zn[0]; break;
-      }
+      if (kflag != 0) { why = kflag; break; }
       nstloc++;
-      if ((u.tn - tout) * u.h >= 0.0) {
-        t = tout;
-        yout = get_dky(u, b, tout);
-        break;
-      }
+      if ((u.tn - tout) * u.h >= 0.0) { why = 1; break; }
+    }
+    if (why == 1) {
+      t = tout;
+      yout = get_dky(u, b, tout);
+    } else if (why < 0) {
+      flag = why;
+      t = u.tn;
+      yout = b.zn[0];
     }
     // ---- calculate_abundances.c retry / abort policy
     if (flag != 0) {

@@ -35,13 +35,15 @@ constexpr int WAVES = 4;  // wavefronts per block
 // dependent cross-lane reductions and LDS round trips), so throughput grows
 // with the wavefronts per SIMD that hide each other's latency, as long as
 // the register budget that buys them does not push the integrator's state
-// into scratch inside the Newton loop.  LDS is kept small enough not to be
-// the tighter limit.
+// into scratch.  Both forms are compiled for a 168-register budget (3
+// wavefronts per SIMD); the FP32 form's LDS (48.5 KB per block) admits 3
+// blocks per CU, the FP64 form's (60.8 KB) 2, where the tighter register
+// budget still measured faster than 256 registers.
 #ifndef PDR_WPE32
 #define PDR_WPE32 3
 #endif
 #ifndef PDR_WPE64
-#define PDR_WPE64 2
+#define PDR_WPE64 3
 #endif
 template <typename MT>
 constexpr int kWavesPerEU = sizeof(MT) == 4 ? PDR_WPE32 : PDR_WPE64;
