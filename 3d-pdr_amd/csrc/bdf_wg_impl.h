@@ -85,28 +85,20 @@ struct BdfWG {
   double ewt, acor, y, ftemp, wsc;
   MT A4[16];                 // inverse row = lane, columns 16*wave .. +15 (pivot order)
   int invp;                  // wavefront 0: step at which tail row `lane` was the pivot
-  // zn[2..QMAX] live in the workgroup's device slot between the step's
-  // bookkeeping phases (the Newton loop needs only zn[0], zn[1])
-  double *zh;                // zh[(j - 2) * BLOCK] = zn[j] of this lane
-  __device__ __forceinline__ void hist_in() {
-#pragma unroll
-    for (int j = 2; j <= QMAX; j++) zn[j] = zh[(j - 2) * BLOCK];
-  }
-  __device__ __forceinline__ void hist_out() {
-#pragma unroll
-    for (int j = 2; j <= QMAX; j++) zh[(j - 2) * BLOCK] = zn[j];
-  }
+  // the whole Nordsieck history stays in registers (moving zn[2..QMAX]
+  // to the device slot measured no faster and costs traffic)
+  __device__ __forceinline__ void hist_in() {}
+  __device__ __forceinline__ void hist_out() {}
 };
 
-// Per-workgroup slot of the device workspace: the saved Jacobian, the lanes'
-// slot rate constants and their Nordsieck entries zn[2..QMAX].  Everything
-// but the Jacobian is lane-private (written and read by the same lane).
+// Per-workgroup slot of the device workspace: the saved Jacobian and the
+// lanes' slot rate constants (lane-private: written and read by the same
+// lane, [r * BLOCK + lane]).
 template <typename MT>
 struct Slot {
   static constexpr size_t jac_bytes = ((size_t)NET::NJ * sizeof(MT) + 255) / 256 * 256;
   static constexpr size_t kp_off = jac_bytes;
-  static constexpr size_t zh_off = kp_off + (size_t)NET::FR * BLOCK * sizeof(double);
-  static constexpr size_t bytes = zh_off + (size_t)(QMAX - 1) * BLOCK * sizeof(double);
+  static constexpr size_t bytes = kp_off + (size_t)NET::FR * BLOCK * sizeof(double);
 };
 
 template <typename MT>
@@ -120,7 +112,6 @@ struct Team {
   WgLDS<MT> LDSA *S;
   MT *Jg;                    // device-memory Jacobian slot [NJ]
   const uint32_t *fab;       // slot factor table, this lane's column: fab[r * BLOCK]
-  double *zh;                // this lane's zn[2..QMAX] in the device slot
   double rtol, atol;
 
   __device__ __forceinline__ bool lead() const { return tid == 0; }
@@ -667,7 +658,6 @@ __device__ __forceinline__ void wg_solve_point(Team<MT> &c, const PointArgs &a, 
     bdf_init_uniform(u, a.t0, xe0);
   }
   __syncthreads();
-  b.zh = c.zh;
   b.zn[0] = y0;
 #pragma unroll
   for (int j = 1; j <= QMAX; j++) b.zn[j] = 0.0;
@@ -721,7 +711,6 @@ bdf_wg_kernel(PointArgs a, unsigned char *ws) {
   unsigned char *slot = ws + (size_t)blockIdx.x * Slot<MT>::bytes;
   c.Jg = (MT *)slot;
   c.Kp = (double *)(slot + Slot<MT>::kp_off) + c.tid;
-  c.zh = (double *)(slot + Slot<MT>::zh_off) + c.tid;
   c.rtol = a.rtol;
   c.atol = a.atol;
   c.fab = NET::slot_fab + c.tid;
