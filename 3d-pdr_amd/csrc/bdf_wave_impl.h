@@ -237,10 +237,10 @@ __device__ __forceinline__ void jacobian(Ctx<MT> &c, MT (&jv)[16]) {
 
 // ---------------------------------------------------------------------
 // Gauss-Jordan inverse (in place, partial pivoting) of the scaled matrix.
-// Step k: the pivot is the unused row with the largest |a[i][k]| (DPP
-// arg-max over the half that holds column k), the pivot row is broadcast
-// with v_readlane, the column entry of each row reaches the other half of
-// the row with v_permlane32_swap.  Rows are not moved: on return A holds
+// Step k: the pivot is the unused row with the largest |a[i][k]| (one DPP
+// max over packed value/row keys in the half that holds column k), the
+// pivot row is broadcast through LDS, the column entry of each row reaches
+// the other half of the row with v_permlane32_swap.  Rows are not moved: on return A holds
 // the inverse in pivot order and invp is the step at which this row was the
 // pivot.  Returns false if singular.
 // ---------------------------------------------------------------------

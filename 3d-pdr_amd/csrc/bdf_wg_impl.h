@@ -17,13 +17,14 @@
 // scaled by the error weights (row i by w_i, column j by 1/w_j).  A static
 // elimination order splits it into a head (pivots eliminated level by level
 // without pivoting, multipliers in place) and a dense tail of NT <= 64
-// unknowns.  The tail's Schur complement Tl - L_TH U_HT is formed with
-// matrix-core (MFMA) products, one 32x32 tile per wavefront, and inverted by
-// Gauss-Jordan with partial pivoting in wavefront 0, whose lanes keep one row
-// of the inverse each in registers.  A solve is: forward substitution over
-// the head levels, the tail rows' gather of head values, the tail inverse
-// applied as one matrix-vector product, the head rows' scatter of tail
-// values, and back substitution over the head levels.  MT is the matrix
+// unknowns.  The setup also forms the explicit inverses of the head's
+// triangular factors, level by level.  The tail's Schur complement
+// Tl - L_TH U_HT is formed with matrix-core (MFMA) products, one 32x32 tile
+// per wavefront, and inverted by Gauss-Jordan with partial pivoting spread
+// over the four wavefronts (each keeps 16 columns of the inverse in
+// registers).  A solve is four chunked sparse mat-vecs with no levels: the
+// head's L^-1, the tail rows' gather of head values, the tail inverse, the
+// head rows' scatter of tail values and the head's U^-1.  MT is the matrix
 // type (double: plain FP64 form; float: mixed form); the RHS, Newton
 // iteration, error test and every n-vector stay FP64.
 //
@@ -33,8 +34,8 @@
 // phase: the fluxes F (RHS), the flux partials D (Jacobian evaluation), and
 // the head entries Hv followed by the tail block Tl (factorisation and
 // solves; Tl lies over F, which is dead while the factorisation runs).  The
-// Jacobian of the last evaluation (CVODE keeps it for later setups) lives in
-// a per-workgroup slot of device memory.
+// Jacobian of the last evaluation (CVODE keeps it for later setups) and the
+// lanes' slot rate constants live in a per-workgroup slot of device memory.
 
 #include "wave_ops.h"
 
