@@ -165,11 +165,15 @@ __device__ __forceinline__ double wave_rhs(Ctx<MT> &c, double y, double &xe_out)
     S.F[r * 64 + ln.L] = S.Kp[r * 64 + ln.L] * S.X[f & 0xff] * S.X[(f >> 8) & 0xff];
   }
   wsync();
+  // term word: byte offset of the flux | the coefficient's FP64 high 16
+  // bits (its other bits are zero), so decoding is two masks
   double acc = 0.0;
+  const unsigned char LDSA *Fb = (const unsigned char LDSA *)S.F;
 #pragma unroll
   for (int t = 0; t < NET::TM; t++) {
-    uint32_t w = c.tb.rterm[t * 64 + ln.L];
-    acc = fma((double)(int16_t)(w >> 16), S.F[w & 0xffff], acc);
+    const uint32_t w = c.tb.rterm[t * 64 + ln.L];
+    const double f = *(const double LDSA *)(Fb + (w & 0xffff));
+    acc = fma(__hiloint2double((int)(w & 0xffff0000u), 0), f, acc);
   }
   S.P[ln.L] = acc;
   wsync();
@@ -271,7 +275,12 @@ __device__ __forceinline__ bool gj_invert(const Lane &ln, WaveLDS<MT> LDSA &S, M
     for (int j = 0; j < 16; j++) pr[j] = prow[j];
     const MT piv = S.PR[k];
     wsync();
-    const MT d = (MT)1.0 / piv;
+    // FP32: the hardware reciprocal (1 ulp) in place of an exactly rounded
+    // division (~10 instructions); the FP32 inverse only preconditions the
+    // Newton iteration
+    MT d;
+    if constexpr (sizeof(MT) == 4) d = __builtin_amdgcn_rcpf(piv);
+    else d = (MT)1.0 / piv;
     const MT oc = other_half(colk);
     const MT fi = (ln.hh == hk) ? colk : oc;
     used = used || isp;

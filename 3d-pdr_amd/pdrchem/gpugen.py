@@ -24,6 +24,7 @@ jacobian_reduced.c:26-...) with table-driven device functions laid out for a
 from __future__ import annotations
 
 import math
+import struct
 import os
 import sys
 from typing import List
@@ -91,6 +92,21 @@ def emit_rate_params(L: List[str], net: Network) -> None:
     L.append("__device__ const uint8_t dup_last[NREAC] = {")
     L.append(_fmt_list(last, 24))
     L.append("};")
+
+
+def coef_hi16(c: int) -> int:
+    """Top 16 bits of the FP64 high word of a small integer coefficient; its
+    other 48 bits are zero for |c| <= 8, so (hi16 << 48) is the double."""
+    bits = struct.unpack("<Q", struct.pack("<d", float(c)))[0]
+    if bits & ((1 << 48) - 1):
+        raise ValueError(f"coefficient {c} is not exact in 16 bits")
+    return bits >> 48
+
+
+def rhs_word(slot: int, c: int) -> int:
+    """RHS term word: LDS byte offset of flux slot `slot` in the low half,
+    the coefficient as coef_hi16 in the high half."""
+    return (8 * slot) | (coef_hi16(c) << 16)
 
 
 class WaveTables:
@@ -219,12 +235,12 @@ class WaveTables:
         L.append(f"__device__ const uint32_t slot_fab[NSLOT] = {{")
         L.append(_fmt_list([f"0x{x:x}u" for x in self.slot_fab], 8))
         L.append("};")
-        L.append("// [t][lane]: flux slot | (int16 coefficient) << 16")
+        L.append("// [t][lane]: byte offset of the flux slot (8 * slot) | top 16 bits of the")
+        L.append("// coefficient's FP64 high word << 16 (the rest of the double is zero)")
         flat = []
         for t in range(self.tm):
             for lane in range(WAVE):
-                s, c = self.rhs_term[t][lane]
-                flat.append(f"0x{(s & 0xffff) | ((c & 0xffff) << 16):x}u")
+                flat.append(f"0x{rhs_word(*self.rhs_term[t][lane]):x}u")
         L.append(f"__device__ const uint32_t rhs_term[TM * 64] = {{")
         L.append(_fmt_list(flat, 8))
         L.append("};")

@@ -215,3 +215,18 @@ def test_pivot_key_order():
     assert (keys[0] >> 5) == 0                      # exact zero: not a usable pivot
     k64 = [_pivot_key(np.float64(v), i) for i, v in enumerate([1e-300, 2.0, 2.0, 7.5])]
     assert max(k64) == k64[3] and k64[1] > k64[2]
+
+
+def test_rhs_word_encoding(case):
+    """the emitted RHS words decode (byte offset, FP64 high bits) to the
+    tables' (slot, coefficient), as the kernel decodes them"""
+    import struct
+    _, tab, _, _, _, _ = case
+    for row in tab.rhs_term:
+        for (slot, c) in row:
+            w = gpugen.rhs_word(slot, c)
+            assert (w & 0xFFFF) == 8 * slot
+            coef = struct.unpack("<d", struct.pack("<Q", (w & 0xFFFF0000) << 32))[0]
+            assert coef == float(c)
+    with pytest.raises(ValueError):
+        gpugen.coef_hi16(1000003)   # not representable in 16 bits of the high word
