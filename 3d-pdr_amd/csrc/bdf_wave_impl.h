@@ -159,21 +159,39 @@ __device__ __forceinline__ double wave_rhs(Ctx<MT> &c, double y, double &xe_out)
   xe_out = xe;
   if (ln.L == 0) S.X[NET::NEQ] = xe;
   wsync();
+  // all factor gathers of the flux rounds are issued before any product
+  uint32_t fv[NET::ROUNDS];
+  double xa[NET::ROUNDS], xb[NET::ROUNDS];
+#pragma unroll
+  for (int r = 0; r < NET::ROUNDS; r++) fv[r] = c.tb.fab[r * 64 + ln.L];
 #pragma unroll
   for (int r = 0; r < NET::ROUNDS; r++) {
-    const uint32_t f = c.tb.fab[r * 64 + ln.L];
-    S.F[r * 64 + ln.L] = S.Kp[r * 64 + ln.L] * S.X[f & 0xff] * S.X[(f >> 8) & 0xff];
+    xa[r] = S.X[fv[r] & 0xff];
+    xb[r] = S.X[(fv[r] >> 8) & 0xff];
   }
+#pragma unroll
+  for (int r = 0; r < NET::ROUNDS; r++) S.F[r * 64 + ln.L] = S.Kp[r * 64 + ln.L] * xa[r] * xb[r];
   wsync();
   // term word: byte offset of the flux | the coefficient's FP64 high 16
   // bits (its other bits are zero), so decoding is two masks
+  // terms in groups of G: the group's words, then its gathers, then the
+  // products (keeps G gathers in flight)
+  constexpr int G = 8;
   double acc = 0.0;
   const unsigned char LDSA *Fb = (const unsigned char LDSA *)S.F;
 #pragma unroll
-  for (int t = 0; t < NET::TM; t++) {
-    const uint32_t w = c.tb.rterm[t * 64 + ln.L];
-    const double f = *(const double LDSA *)(Fb + (w & 0xffff));
-    acc = fma(__hiloint2double((int)(w & 0xffff0000u), 0), f, acc);
+  for (int t0 = 0; t0 < NET::TM; t0 += G) {
+    uint32_t w[G];
+    double f[G];
+#pragma unroll
+    for (int u = 0; u < G; u++)
+      if (t0 + u < NET::TM) w[u] = c.tb.rterm[(t0 + u) * 64 + ln.L];
+#pragma unroll
+    for (int u = 0; u < G; u++)
+      if (t0 + u < NET::TM) f[u] = *(const double LDSA *)(Fb + (w[u] & 0xffff));
+#pragma unroll
+    for (int u = 0; u < G; u++)
+      if (t0 + u < NET::TM) acc = fma(__hiloint2double((int)(w[u] & 0xffff0000u), 0), f[u], acc);
   }
   S.P[ln.L] = acc;
   wsync();
