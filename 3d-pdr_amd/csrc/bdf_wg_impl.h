@@ -235,15 +235,16 @@ __device__ __forceinline__ double wg_rhs(Team<MT> &c, double y, double &xe_out) 
     F[r * BLOCK + c.tid] = c.Kp[r * BLOCK] * S.X[f & 0xff] * S.X[(f >> 8) & 0xff];
   }
   __syncthreads();
-  const uint16_t *rt = opaque(NET::rhs_term);
+  // term word: byte offset of the flux | the coefficient's FP64 high 16
+  // bits (its other bits are zero), so decoding is two masks
+  const uint32_t *rt = opaque(NET::rhs_term);
+  const unsigned char LDSA *Fb = (const unsigned char LDSA *)F;
 #pragma unroll
   for (int p = 0; p < NET::VL; p++) {
     double acc = 0.0;
-    for_items_gc<6>(rt, p * NET::TM, NET::TM, c.tid, [&](uint16_t w) { return F[w & 0xfff]; },
-                    [&](uint16_t w, double f) {
-                      const int coef = (int)((uint32_t)w << 16) >> 28;  // signed 4-bit field at bit 12
-                      acc = fma((double)coef, f, acc);
-                    });
+    for_items_gc<6>(rt, p * NET::TM, NET::TM, c.tid,
+                    [&](uint32_t w) { return *(const double LDSA *)(Fb + (w & 0xffff)); },
+                    [&](uint32_t w, double f) { acc = fma(__hiloint2double((int)(w & 0xffff0000u), 0), f, acc); });
     S.P[p * BLOCK + c.tid] = acc;
   }
   __syncthreads();

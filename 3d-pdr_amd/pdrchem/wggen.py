@@ -61,6 +61,14 @@ def _lpt(loads: List[Tuple[int, object]], nbins: int) -> List[List[object]]:
     return bins
 
 
+def rhs_word32(w16: int) -> int:
+    """Kernel word of a packed RHS term (slot | 4-bit coefficient << 12): the
+    flux's LDS byte offset and the coefficient as an FP64 high half."""
+    from pdrchem.gpugen import rhs_word
+    slot, c = w16 & 0xFFF, (w16 >> 12) & 0xF
+    return rhs_word(slot, c - 16 if c >= 8 else c)
+
+
 def _coef4(c: int) -> int:
     if not -8 <= c <= 7:
         raise ValueError(f"stoichiometric coefficient {c} does not fit 4 bits")
@@ -460,9 +468,11 @@ class WgTables:
         hx64 = lambda v: f"0x{v:x}ull"
         arr("int16_t", "slot_reac", self.slot_reac + [-1] * (self.nslot - len(self.slot_reac)))
         arr("uint32_t", "slot_fab", self.slot_fab, hx, 8)
-        flat = [self.rhs_term[p][t][l] for p in range(self.vl) for t in range(self.tm) for l in range(self.block)]
-        L.append("// [pass][t][lane]: flux slot | coef << 12 (4-bit signed)")
-        arr("uint16_t", "rhs_term", flat, hx, 12)
+        flat = [rhs_word32(self.rhs_term[p][t][l])
+                for p in range(self.vl) for t in range(self.tm) for l in range(self.block)]
+        L.append("// [pass][t][lane]: byte offset of the flux slot (8 * slot) | top 16 bits of the")
+        L.append("// coefficient's FP64 high word << 16 (the rest of the double is zero)")
+        arr("uint32_t", "rhs_term", flat, hx, 8)
         rb = [self.row_base[i] if i < self.neq else 0 for i in range(self.block)]
         rn = [self.row_nl[i] if i < self.neq else 0 for i in range(self.block)]
         arr("uint16_t", "row_base", rb)

@@ -233,3 +233,18 @@ def test_analysis_structure():
         assert all(A[k, k] for k in an.head)
         # the electron border stays in the tail (the head never divides by it)
         assert (an.nb - 1) in an.tail
+
+
+def test_rhs_word_encoding(case):
+    """the kernel's 32-bit RHS words decode to the packed terms' slot and
+    coefficient (byte offset 8*slot, coefficient as an FP64 high half)"""
+    import struct
+    name, net, tab, orc, rate, nH, x = case
+    for p in range(tab.vl):
+        for t in range(tab.tm):
+            for lane in range(tab.block):
+                w16 = tab.rhs_term[p][t][lane]
+                w = wggen.rhs_word32(w16)
+                assert (w & 0xFFFF) == 8 * (w16 & 0xFFF)
+                coef = struct.unpack("<d", struct.pack("<Q", (w & 0xFFFF0000) << 32))[0]
+                assert coef == float(_coef(w16, 12))
