@@ -48,6 +48,7 @@ struct RateRequest {
   const double *T, *Td;   // [npts]
   const double *G, *Av;   // [npts][nrays]
   const double *cols;     // [npts][nrays][5] (H2, HD, CO, C, S) or null
+  const double *zeta_pts; // [npts] cosmic-ray ionisation rate per point (units of ph.zeta) or null
   int npts, nrays;
   PhysParams ph;
   double *rate;           // [npts][nreac]

@@ -199,13 +199,20 @@ int pdrchem_solve_device(int net, double *d_abundance, const double *d_rate, con
 int pdrchem_rates_device(int net, const pdrchem_phys *phys, const double *d_T, const double *d_Td, int nrays,
                          const double *d_G, const double *d_Av, const double *d_cols, int npts, double *d_rate,
                          int *d_neg, void *stream) {
+  return pdrchem_rates_device_zeta(net, phys, d_T, d_Td, nrays, d_G, d_Av, d_cols, nullptr, npts, d_rate, d_neg,
+                                   stream);
+}
+
+int pdrchem_rates_device_zeta(int net, const pdrchem_phys *phys, const double *d_T, const double *d_Td, int nrays,
+                              const double *d_G, const double *d_Av, const double *d_cols, const double *d_zeta,
+                              int npts, double *d_rate, int *d_neg, void *stream) {
   if (net < 0 || net >= NET_COUNT || !kCompiled[net]) {
     set_err("unsupported network id");
     return -1;
   }
   if (npts <= 0) return 0;
   if (!phys || !d_T || !d_Td || !d_G || !d_Av || !d_rate || !d_neg || nrays < 0) {
-    set_err("bad argument to pdrchem_rates_device");
+    set_err("bad argument to pdrchem_rates_device_zeta");
     return -1;
   }
   RateRequest q{};
@@ -214,6 +221,7 @@ int pdrchem_rates_device(int net, const pdrchem_phys *phys, const double *d_T, c
   q.G = d_G;
   q.Av = d_Av;
   q.cols = d_cols;
+  q.zeta_pts = d_zeta;
   q.npts = npts;
   q.nrays = nrays;
   q.ph = PhysParams{phys->zeta, phys->v_turb, phys->omega, phys->grain_radius, phys->metallicity, phys->g2d};

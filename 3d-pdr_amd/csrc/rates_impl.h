@@ -217,8 +217,10 @@ __global__ void __launch_bounds__(256) rates_kernel(pdrchem::RateRequest q) {
     int p = (int)(idx / NET::NREAC);
     int j = (int)(idx - (long long)p * NET::NREAC);
     int neg = 0;
+    pdrchem::PhysParams ph = q.ph;
+    if (q.zeta_pts) ph.zeta = q.zeta_pts[p];
     double r = rate_one(j, q.T[p], q.Td[p], q.nrays, q.G + (size_t)p * q.nrays, q.Av + (size_t)p * q.nrays,
-                        q.cols ? q.cols + (size_t)p * q.nrays * 5 : nullptr, q.ph, &neg);
+                        q.cols ? q.cols + (size_t)p * q.nrays * 5 : nullptr, ph, &neg);
     q.rate[idx] = r;
     if (neg) atomicAdd(q.neg, 1);
   }

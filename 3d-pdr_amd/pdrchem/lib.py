@@ -23,7 +23,7 @@ NETWORKS = {"reduced": (33, 329), "full": (215, 2926), "mynetwork": (58, 634)}
 # every function and global declared in include/pdrchem.h
 EXPORTED_FUNCTIONS = [
     "calculate_abundances_", "pdrchem_network_id", "pdrchem_solve_device", "pdrchem_set_precision",
-    "pdrchem_rates_device", "pdrchem_kernel_info", "pdrchem_set_max_blocks", "pdrchem_last_error",
+    "pdrchem_rates_device", "pdrchem_rates_device_zeta", "pdrchem_kernel_info", "pdrchem_set_max_blocks", "pdrchem_last_error",
     "pdrchem_server_create", "pdrchem_server_submit", "pdrchem_server_progress", "pdrchem_server_poll",
     "pdrchem_server_last_error", "pdrchem_server_destroy",
 ]
@@ -74,6 +74,8 @@ def load(path: Optional[str] = None):
                                        C.c_double, C.c_int, vp, vp, vp, C.c_int, vp]
     L.pdrchem_rates_device.argtypes = [C.c_int, C.POINTER(Phys), vp, vp, C.c_int, vp, vp, vp, C.c_int, vp, vp,
                                        vp]
+    L.pdrchem_rates_device_zeta.argtypes = [C.c_int, C.POINTER(Phys), vp, vp, C.c_int, vp, vp, vp, vp, C.c_int,
+                                            vp, vp, vp]
     L.pdrchem_kernel_info.argtypes = [C.c_int, C.c_int, ip, ip, ip, ip]
     L.pdrchem_set_max_blocks.argtypes = [C.c_int]
     L.pdrchem_set_max_blocks.restype = None
@@ -191,9 +193,10 @@ def solve_device(net: int, abundance, rate, density, rtol=1e-8, atol=1e-30, t_st
 
 
 def rates_device(net: int, T, Td, G, Av, cols=None, phys: Optional[Phys] = None, out=None, neg=None,
-                 stream=None):
+                 stream=None, zeta=None):
     """Device rate evaluation: T, Td [n]; G, Av [n, nrays]; cols [n, nrays, 5]
-    (H2, HD, CO, C, S) or None.  Returns (rate [n, nreac], neg counter)."""
+    (H2, HD, CO, C, S) or None; zeta [n] per-point cosmic-ray ionisation rate
+    (units of phys.zeta) or None.  Returns (rate [n, nreac], neg counter)."""
     import torch
     L = load()
     n = T.shape[0]
@@ -205,8 +208,8 @@ def rates_device(net: int, T, Td, G, Av, cols=None, phys: Optional[Phys] = None,
         neg = torch.zeros(1, dtype=torch.int32, device=T.device)
     ph = phys or default_phys()
     s = stream if stream is not None else torch.cuda.current_stream(T.device)
-    rc = L.pdrchem_rates_device(net, C.byref(ph), _ptr(T), _ptr(Td), nrays, _ptr(G), _ptr(Av), _ptr(cols), n,
-                                _ptr(out), _ptr(neg), C.c_void_p(s.cuda_stream))
+    rc = L.pdrchem_rates_device_zeta(net, C.byref(ph), _ptr(T), _ptr(Td), nrays, _ptr(G), _ptr(Av), _ptr(cols),
+                                     _ptr(zeta), n, _ptr(out), _ptr(neg), C.c_void_p(s.cuda_stream))
     if rc != 0:
         raise PdrChemError(last_error())
     return out, neg

@@ -100,6 +100,16 @@ int pdrchem_rates_device(int net, const pdrchem_phys *phys, const double *d_T, c
                          int nrays, const double *d_G, const double *d_Av, const double *d_cols,
                          int npts, double *d_rate, int *d_neg, void *stream);
 
+/* As pdrchem_rates_device, with a cosmic-ray ionisation rate per point:
+ * d_zeta[npts] (units of 1.3e-17 s^-1, like phys->zeta; NULL = phys->zeta
+ * everywhere) replaces phys->zeta in the cosmic-ray (calc_reac_rates.F90:247),
+ * cosmic-ray-photon (:275-276) and cosmic-ray desorption (:334-340) rates.  No
+ * counterpart in the reference's rate routine, which takes one global zeta;
+ * it serves the per-point zeta of the reference's cosmic-ray fork. */
+int pdrchem_rates_device_zeta(int net, const pdrchem_phys *phys, const double *d_T, const double *d_Td,
+                              int nrays, const double *d_G, const double *d_Av, const double *d_cols,
+                              const double *d_zeta, int npts, double *d_rate, int *d_neg, void *stream);
+
 /* Kernel resource usage of the solver for a network and precision
  * (registers per lane, LDS bytes per block, scratch bytes per lane, blocks
  * per CU).  0 = ok. */
