@@ -105,6 +105,8 @@ def main():
                     help="serving workload: Poisson stream of requests through the request loop")
     ap.add_argument("--serve-rate", type=float, default=6.0, help="requests per second")
     ap.add_argument("--serve-duration", type=float, default=30.0, help="seconds of arrivals")
+    ap.add_argument("--serve-networks", default="reduced,full",
+                    help="networks of the requests, chosen with equal odds")
     a = ap.parse_args()
     if a.npts is None:
         a.npts = 1_000_000 if a.network == "reduced" else 20_000
@@ -227,7 +229,7 @@ def main():
 
 def serve(a):
     """Serving workload: requests of 1..4096 points (log-uniform) of the
-    reduced or the full network (equal odds), end times log-uniform in
+    networks in --serve-networks (equal odds), end times log-uniform in
     [1e3, 1e7] yr, Poisson arrivals, submitted to the request loop as they
     arrive; latency is submit-to-result, throughput is points per second
     from the first arrival to the last result.  Inputs (device-evaluated
@@ -237,9 +239,10 @@ def serve(a):
     from pdrchem.netgen import load_network
     dev = torch.device("cuda:0")
     prec = lib.MIXED if a.precision == "mixed" else lib.FP64
-    R = workloads.request_stream(duration=a.serve_duration, rate=a.serve_rate)
+    nets = tuple(a.serve_networks.split(","))
+    R = workloads.request_stream(duration=a.serve_duration, rate=a.serve_rate, networks=nets)
     data = {}
-    for name in ("reduced", "full"):
+    for name in nets:
         tot = sum(r.npts for r in R if r.network == name)
         net = load_network(name)
         nid = lib.network_id(name)
@@ -279,7 +282,8 @@ def serve(a):
         "vs_baseline": None, "dtype": "f64" if prec == lib.FP64 else "f64+f32-matrix",
         "data": "synthetic (seeded counter-based RNG)",
         "config": {"workload": f"serving: Poisson {a.serve_rate} req/s for {a.serve_duration} s, 1-4096 points "
-                               f"per request (log-uniform), reduced/full 50/50, t_end logU[1e3,1e7] yr",
+                               f"per request (log-uniform), networks {'/'.join(nets)} with equal odds, "
+                               f"t_end logU[1e3,1e7] yr",
                    "requests": len(R), "points": npts, "precision": a.precision, "slots": 4},
         "failed_requests": int(sum(1 for v in status.values() if v != 0)),
     }
