@@ -36,9 +36,7 @@ constexpr int WAVES = 4;  // wavefronts per block
 // with the wavefronts per SIMD that hide each other's latency, as long as
 // the register budget that buys them does not push the integrator's state
 // into scratch.  Both forms are compiled for a 168-register budget (3
-// wavefronts per SIMD); the FP32 form's LDS (48.5 KB per block) admits 3
-// blocks per CU, the FP64 form's (60.8 KB) 2, where the tighter register
-// budget still measured faster than 256 registers.
+// wavefronts per SIMD) and use 48.5 KB of LDS per block (3 blocks per CU).
 #ifndef PDR_WPE32
 #define PDR_WPE32 3
 #endif
@@ -48,6 +46,12 @@ constexpr int WAVES = 4;  // wavefronts per block
 template <typename MT>
 constexpr int kWavesPerEU = sizeof(MT) == 4 ? PDR_WPE32 : PDR_WPE64;
 
+// The slot rate constants (read by every RHS) stay in LDS in the FP32 form;
+// the FP64 form keeps them in the device workspace (L2) so that its LDS
+// (flux partials in FP64) fits three blocks per CU.
+template <typename MT>
+constexpr bool kKpInLDS = sizeof(MT) == 4;
+
 // Per-wavefront workspace in device memory for the saved Jacobian, read
 // only when the iteration matrix is formed without a new Jacobian.  Indexed
 // [k*64 + lane] and only ever read back by the lane that wrote it, so it
@@ -55,6 +59,7 @@ constexpr int kWavesPerEU = sizeof(MT) == 4 ? PDR_WPE32 : PDR_WPE64;
 template <typename MT>
 struct WaveWS {
   MT J[16 * 64];                // the lane's 16 columns
+  double Kp[kKpInLDS<MT> ? 1 : NET::ROUNDS * 64];  // FP64 form: slot rate constants
 };
 
 // LDS words (doubles) of the flux region: FP64 fluxes F[0..NSLOT) and the
@@ -70,7 +75,7 @@ template <typename MT>
 struct WaveLDS {
   double X[64];                    // factor vector: y, x_e, 1
   double F[kFWords<MT>];           // fluxes (RHS) or flux partials (Jacobian)
-  double Kp[NET::ROUNDS * 64];     // rate * (n_H) of each lane's flux slots
+  double Kp[kKpInLDS<MT> ? NET::ROUNDS * 64 : 1];  // rate * (n_H) of each lane's flux slots
   double Zh[(QMAX - 1) * 64];      // Nordsieck entries zn[2..QMAX] of each lane
   // used by one phase at a time: the RHS (P), the M build and the solve
   // (V, V2), the Gauss-Jordan steps (PR)
@@ -147,6 +152,12 @@ struct Ctx {
   __device__ __forceinline__ double solve(const BdfU LDSA &u, const Bdf<MT> &b, double r);
 };
 
+template <typename MT>
+__device__ __forceinline__ double kp_get(const Ctx<MT> &c, int s) {
+  if constexpr (kKpInLDS<MT>) return c.S->Kp[s];
+  else return c.W->Kp[s];
+}
+
 // ---------------------------------------------------------------------
 // RHS: returns ydot_i for this lane's row; x_e (uniform) via xe_out
 // ---------------------------------------------------------------------
@@ -170,7 +181,7 @@ __device__ __forceinline__ double wave_rhs(Ctx<MT> &c, double y, double &xe_out)
     xb[r] = S.X[(fv[r] >> 8) & 0xff];
   }
 #pragma unroll
-  for (int r = 0; r < NET::ROUNDS; r++) S.F[r * 64 + ln.L] = S.Kp[r * 64 + ln.L] * xa[r] * xb[r];
+  for (int r = 0; r < NET::ROUNDS; r++) S.F[r * 64 + ln.L] = kp_get(c, r * 64 + ln.L) * xa[r] * xb[r];
   wsync();
   // term word: byte offset of the flux | the coefficient's FP64 high 16
   // bits (its other bits are zero), so decoding is two masks
@@ -215,7 +226,7 @@ __device__ __forceinline__ void jacobian(Ctx<MT> &c, MT (&jv)[16]) {
   for (int r = 0; r < NET::ROUNDS; r++) {
     const int s = r * 64 + ln.L;
     const uint32_t f = c.tb.fab[s];
-    const double kp = S.Kp[s];
+    const double kp = kp_get(c, s);
     D[2 * s] = (MT)(kp * S.X[(f >> 8) & 0xff]);   // d flux / d X[a]
     D[2 * s + 1] = (MT)(kp * S.X[f & 0xff]);      // d flux / d X[b]
   }
@@ -420,7 +431,9 @@ __device__ __forceinline__ void solve_point(Ctx<MT> &c, const PointArgs &a, int 
     int re = NET::slot_reac[s];
     const uint32_t f = c.tb.fab[s];
     const double k = (re >= 0) ? rate[re] : 0.0;
-    c.S->Kp[s] = ((f >> 16) & 1) ? k * nH : k;
+    const double kps = ((f >> 16) & 1) ? k * nH : k;
+    if constexpr (kKpInLDS<MT>) c.S->Kp[s] = kps;
+    else c.W->Kp[s] = kps;
   }
   double *ab = a.abundance + (size_t)sys * a.ab_stride;
   const double y0 = ln.row_ok ? ab[ln.i] : 0.0;
