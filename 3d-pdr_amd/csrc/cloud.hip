@@ -1,0 +1,797 @@
+// Column densities along HEALPix rays through a regular grid, and the
+// device-resident chemistry iteration over the grid (pdrchem_cloud_*).
+//
+// Ray tracer: the evaluation-point construction of eval_points.F90:129-225
+// on a regular nx*ny*nz grid.  For an origin p the reference visits every
+// other point in order of distance, finds the HEALPix ray whose pixel holds
+// the direction (vec2ang + ang2pix_nest_id, healpix.F90:267-398), and accepts
+// the point as the ray's next evaluation point when the angle between
+// (point - ep) and the ray, seen from the ray's current evaluation point ep,
+// is at most theta_crit; ep then moves to the point's projection on the ray.
+// On a grid the offsets o = point - p, their pixel and their distance do not
+// depend on p, so the host builds once, per ray, the list of offsets of that
+// ray's pixel sorted by (|o|^2, linear offset) -- which is the order by
+// (distance, point index) for every origin -- and one thread per (origin,
+// ray) walks its ray's list, skipping offsets outside the grid.  Threads of
+// a wavefront share the ray and read the same list entry (scalar loads).  A
+// first pass counts evaluation points per (origin, ray), an exclusive scan
+// gives CSR offsets, a second pass writes the projected point and the step
+// length |epoint_i - epoint_{i-1}| of each evaluation point.  There is no
+// maxpoints cap (the reference STOPs beyond 600, eval_points.F90:210).
+//
+// Columns: Av per ray (CALC_UVFIELD, 3DPDR.F90:1650-1664) and the columns of
+// H2, HD, CO, C, S (calc_columndens, 3DPDR.F90:1818-1838: trapezoid of
+// n_H x_k times the step, times pc) by one thread per (point, ray).
+//
+// Both follow the reference's expressions operation by operation, without
+// fused multiply-adds, so a grid with a power-of-two spacing (exact
+// coordinates and differences) gives the same evaluation points as a
+// point-cloud tracer in double precision.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/pdrchem.h"
+
+#pragma clang fp contract(off)
+
+namespace {
+
+constexpr double kPc = 3.08568025e18;  // healpix_types.F90:98
+constexpr double kHalfPi = 1.570796326794896619231321691639751442099;
+constexpr double kTwoPi = 6.283185307179586476925286766559005768394;
+constexpr double kTwoThird = 0.6666666666666666666666666666666666666666;
+constexpr int kNsMax = 8192;
+
+thread_local std::string t_err;
+
+bool ok(hipError_t e, const char *what) {
+  if (e == hipSuccess) return true;
+  t_err = std::string(what) + ": " + hipGetErrorString(e);
+  return false;
+}
+
+// ---------------------------------------------------------------------------
+// HEALPix, nested scheme (Gorski et al. 2005, ApJ 622:759), host side
+// ---------------------------------------------------------------------------
+int spread_bits(int v) {
+  int r = 0;
+  for (int p = 0; v; p++, v >>= 1) r |= (v & 1) << (2 * p);
+  return r;
+}
+int gather_bits(int k, int odd) {
+  int r = 0;
+  k >>= odd;
+  for (int p = 0; k; p++, k >>= 2) r |= (k & 1) << p;
+  return r;
+}
+
+// centre of pixel ipix (healpix.F90:47-223, small components set to 0 as :172-173)
+void pix_centre(int nside, int ipix, double v[3]) {
+  static const int jrll[12] = {2, 2, 2, 2, 3, 3, 3, 3, 4, 4, 4, 4};
+  static const int jpll[12] = {1, 3, 5, 7, 0, 2, 4, 6, 1, 3, 5, 7};
+  const double fn = (double)nside;
+  const double fact1 = 1.0 / (3.0 * fn * fn), fact2 = 2.0 / (3.0 * fn);
+  const int nl4 = 4 * nside, npface = nside * nside;
+  const int face = ipix / npface, ipf = ipix % npface;
+  const int ix = gather_bits(ipf, 0), iy = gather_bits(ipf, 1);
+  const int jr = jrll[face] * nside - (ix + iy) - 1;
+  int nr = nside, kshift = ((jr - nside) % 2 + 2) % 2;
+  double z = (2 * nside - jr) * fact2;
+  if (jr < nside) {
+    nr = jr;
+    z = 1.0 - nr * nr * fact1;
+    kshift = 0;
+  } else if (jr > 3 * nside) {
+    nr = nl4 - jr;
+    z = -1.0 + nr * nr * fact1;
+    kshift = 0;
+  }
+  int jp = (jpll[face] * nr + (ix - iy) + 1 + kshift) / 2;
+  if (jp > nl4) jp -= nl4;
+  if (jp < 1) jp += nl4;
+  const double phi = (jp - (kshift + 1) * 0.5) * (kHalfPi / nr);
+  const double sth = std::sqrt((1.0 - z) * (1.0 + z));
+  v[0] = sth * std::cos(phi);
+  v[1] = sth * std::sin(phi);
+  v[2] = z;
+  if (std::fabs(v[1]) < 1e-10) v[1] = 0.0;
+  if (std::fabs(v[0]) < 1e-10) v[0] = 0.0;
+}
+
+// pixel holding direction r: angles first, then the pixel at resolution
+// kNsMax degraded to nside (vec2ang, ang2pix_nest_id)
+int pix_of(int nside, const double r[3]) {
+  const double dnorm = std::sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+  double theta = 0.0, phi = 0.0;
+  if (dnorm != 0.0) {
+    theta = std::acos(r[2] / dnorm);
+    if (r[0] != 0.0 || r[1] != 0.0) phi = std::atan2(r[1], r[0]);
+    if (phi < 0.0) phi = phi + kTwoPi;
+  }
+  const double z = std::cos(theta), za = std::fabs(z);
+  const double tt = (phi - std::floor(phi / kTwoPi) * kTwoPi) / kHalfPi;
+  int face, ix, iy;
+  if (za <= kTwoThird) {
+    const int jp = (int)(kNsMax * (0.5 + tt - z * 0.75));
+    const int jm = (int)(kNsMax * (0.5 + tt + z * 0.75));
+    const int ifp = jp / kNsMax, ifm = jm / kNsMax;
+    face = ifp == ifm ? (ifp % 4) + 4 : (ifp < ifm ? ifp % 4 : (ifm % 4) + 8);
+    ix = jm % kNsMax;
+    iy = kNsMax - (jp % kNsMax) - 1;
+  } else {
+    int ntt = (int)tt;
+    if (ntt >= 4) ntt = 3;
+    const double tp = tt - ntt, tmp = std::sqrt(3.0 * (1.0 - za));
+    int jp = (int)(kNsMax * tp * tmp), jm = (int)(kNsMax * (1.0 - tp) * tmp);
+    jp = std::min(kNsMax - 1, jp);
+    jm = std::min(kNsMax - 1, jm);
+    if (z >= 0) {
+      face = ntt;
+      ix = kNsMax - jm - 1;
+      iy = kNsMax - jp - 1;
+    } else {
+      face = ntt + 8;
+      ix = jp;
+      iy = jm;
+    }
+  }
+  int ipf = (spread_bits(ix / 128) + 2 * spread_bits(iy / 128)) * (128 * 128) +
+            (spread_bits(ix % 128) + 2 * spread_bits(iy % 128));
+  const int d = kNsMax / nside;
+  return ipf / (d * d) + face * nside * nside;
+}
+
+// ---------------------------------------------------------------------------
+// device kernels
+// ---------------------------------------------------------------------------
+struct RayGrid {
+  int nx, ny, nz, nrays;
+  double h, theta_crit;
+};
+
+// Walk ray j of every origin.  tab[k]: offset packed as (ox+128) |
+// (oy+128)<<8 | (oz+128)<<16; r2[k] = |o|^2; entries of ray j are
+// tab[ptr[j] .. ptr[j+1]), sorted by (|o|^2, linear offset).
+template <bool FILL>
+__global__ void __launch_bounds__(256) ray_walk_kernel(RayGrid g, const unsigned *__restrict__ tab,
+                                                       const int *__restrict__ r2, const int *__restrict__ ptr,
+                                                       const double *__restrict__ vec, int *__restrict__ count,
+                                                       const long long *__restrict__ offs, int *__restrict__ proj,
+                                                       double *__restrict__ step) {
+  const int j = blockIdx.y;
+  const int npts = g.nx * g.ny * g.nz;
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npts) return;
+  const int px = p % g.nx, py = (p / g.nx) % g.ny, pz = p / (g.nx * g.ny);
+  const double ox0 = px * g.h, oy0 = py * g.h, oz0 = pz * g.h;
+  // distance to the farthest point: a corner (the last distance of the sort)
+  double radius = 0.0;
+  int r2max = 0;
+  for (int c = 0; c < 8; c++) {
+    const int dx = ((c & 1) ? g.nx - 1 : 0) - px, dy = ((c & 2) ? g.ny - 1 : 0) - py,
+              dz = ((c & 4) ? g.nz - 1 : 0) - pz;
+    const double rx = dx * g.h, ry = dy * g.h, rz = dz * g.h;
+    radius = fmax(radius, sqrt(rx * rx + ry * ry + rz * rz));
+    r2max = max(r2max, dx * dx + dy * dy + dz * dz);
+  }
+  const double hv0 = 1.1 * radius * vec[3 * j], hv1 = 1.1 * radius * vec[3 * j + 1],
+               hv2 = 1.1 * radius * vec[3 * j + 2];
+  double e0 = 0.0, e1 = 0.0, e2 = 0.0;        // evaluation point, relative to the origin
+  double q0 = ox0, q1 = oy0, q2 = oz0;        // previous evaluation point, absolute
+  int n = 0;
+  long long w = FILL ? offs[(long long)p * g.nrays + j] : 0;
+  const int kend = ptr[j + 1];
+  for (int k = ptr[j]; k < kend; k++) {
+    if (r2[k] > r2max) break;
+    const unsigned e = tab[k];
+    const int dx = (int)(e & 255u) - 128, dy = (int)((e >> 8) & 255u) - 128, dz = (int)((e >> 16) & 255u) - 128;
+    const int tx = px + dx, ty = py + dy, tz = pz + dz;
+    if (tx < 0 || tx >= g.nx || ty < 0 || ty >= g.ny || tz < 0 || tz >= g.nz) continue;
+    const double rv0 = dx * g.h, rv1 = dy * g.h, rv2 = dz * g.h;
+    const double a0 = rv0 - e0, a1 = rv1 - e1, a2 = rv2 - e2;
+    const double b0 = hv0 - e0, b1 = hv1 - e1, b2 = hv2 - e2;
+    const double dot = a0 * b0 + a1 * b1 + a2 * b2;
+    const double na = sqrt(a0 * a0 + a1 * a1 + a2 * a2), nb = sqrt(b0 * b0 + b1 * b1 + b2 * b2);
+    const double ang = acos(dot / (na * nb));
+    if (!(ang <= g.theta_crit)) continue;
+    if (hv2 != 0.0) {
+      e2 = (hv0 * hv2 * rv0 + hv1 * hv2 * rv1 + (hv2 * hv2) * rv2) / (hv0 * hv0 + hv1 * hv1 + hv2 * hv2);
+      e1 = e2 * hv1 / hv2;
+      e0 = e2 * hv0 / hv2;
+    } else if (hv0 == 0.0) {
+      e0 = 0.0;
+      e1 = rv1;
+      e2 = 0.0;
+    } else if (hv1 == 0.0) {
+      e0 = rv0;
+      e1 = 0.0;
+      e2 = 0.0;
+    } else {
+      e2 = 0.0;
+      e0 = ((hv0 * hv0) * rv0 + hv0 * hv1 * rv1) / (hv0 * hv0 + hv1 * hv1);
+      e1 = e0 * hv1 / hv0;
+    }
+    n++;
+    if (FILL) {
+      const double c0 = e0 + ox0, c1 = e1 + oy0, c2 = e2 + oz0;
+      proj[w] = tx + g.nx * (ty + g.ny * tz);
+      step[w] = sqrt((q0 - c0) * (q0 - c0) + (q1 - c1) * (q1 - c1) + (q2 - c2) * (q2 - c2));
+      q0 = c0;
+      q1 = c1;
+      q2 = c2;
+      w++;
+    }
+  }
+  if (!FILL) count[(long long)p * g.nrays + j] = n;
+}
+
+struct ColumnArgs {
+  int npts, nrays, nspec;
+  int sel[5];          // abundance slots of H2, HD, CO, C, S (-1: absent)
+  double av_fac;
+  const long long *offs;
+  const int *count, *proj;
+  const double *step, *rho, *ab;
+  double *Av, *cols;   // [npts][nrays], [npts][nrays][5] (cols may be null)
+};
+
+__global__ void __launch_bounds__(256) ray_columns_kernel(ColumnArgs a) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)a.npts * a.nrays) return;
+  const int p = (int)(t / a.nrays);
+  const long long o = a.offs[t];
+  const int n = a.count[t];
+  double cd = 0.0, c[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  int q0 = p;
+  for (int i = 0; i < n; i++) {
+    const int q1 = a.proj[o + i];
+    const double s = a.step[o + i];
+    const double r0 = a.rho[q0], r1 = a.rho[q1];
+    cd = cd + ((r0 + r1) / 2.) * s * kPc;
+    if (a.cols) {
+#pragma unroll
+      for (int k = 0; k < 5; k++) {
+        if (a.sel[k] < 0) continue;
+        const double x0 = a.ab[(long long)q0 * a.nspec + a.sel[k]], x1 = a.ab[(long long)q1 * a.nspec + a.sel[k]];
+        c[k] = c[k] + s * kPc * (r0 * x0 + r1 * x1) / 2.;
+      }
+    }
+    q0 = q1;
+  }
+  a.Av[t] = cd * a.av_fac;
+  if (a.cols)
+    for (int k = 0; k < 5; k++) a.cols[t * 5 + k] = c[k];
+}
+
+// per-ray surface field, attenuated field and GUESS_TEMP temperature
+// (3DPDR.F90:1667-1700); gray[j] is the surface field of ray j
+__global__ void uvfield_kernel(int npts, int nrays, const double *__restrict__ gray, const double *__restrict__ Av,
+                               double uv_fac, double *__restrict__ G, double *__restrict__ uv,
+                               double *__restrict__ T) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npts) return;
+  double u = 0.0;
+  for (int j = 0; j < nrays; j++) {
+    const double g = gray[j];
+    G[(long long)p * nrays + j] = g;
+    u = u + g * exp(-Av[(long long)p * nrays + j] * uv_fac);
+    if (u < 1.0e-50) u = 0.0;
+  }
+  uv[p] = u;
+  T[p] = 10.0 * (1.0 + pow(2. * u, 1.0 / 3.0));
+}
+
+__global__ void fill_kernel(double *x, long long n, double v) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    x[i] = v;
+}
+__global__ void zero_u32_kernel(unsigned *x) { *x = 0u; }
+
+// rows of src selected by idx -> dst (gather) or back (scatter)
+template <bool SCATTER>
+__global__ void rows_kernel(double *__restrict__ dst, const double *__restrict__ src, const int *__restrict__ idx,
+                            int nrows, int rowlen) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)nrows * rowlen) return;
+  const int r = (int)(t / rowlen), c = (int)(t % rowlen);
+  const long long q = (long long)idx[r] * rowlen + c;
+  if (SCATTER)
+    dst[q] = src[t];
+  else
+    dst[t] = src[q];
+}
+
+// which points to solve: not yet converged, or their shielding columns have
+// moved by more than tol since they were last solved
+__global__ void activity_kernel(int npts, int ncol, const double *__restrict__ cols,
+                                const double *__restrict__ cols_solved, const unsigned char *__restrict__ conv,
+                                double tol, unsigned char *__restrict__ active) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npts) return;
+  bool act = !conv[p];
+  if (!act) {
+    for (int k = 0; k < ncol; k++) {
+      const double a = cols[(long long)p * ncol + k], b = cols_solved[(long long)p * ncol + k];
+      const double m = fmax(fmax(fabs(a), fabs(b)), 1.0);
+      if (fabs(a - b) > tol * m) {
+        act = true;
+        break;
+      }
+    }
+  }
+  active[p] = act ? 1 : 0;
+}
+
+// after a solve of the active points (compact index c -> point idx[c]):
+// convergence test against the previous abundances, write-back, and record
+// the columns the point was solved with
+__global__ void retire_kernel(int nact, int nspec, int ncol, const int *__restrict__ idx,
+                              const double *__restrict__ ab_c, const int *__restrict__ status_c,
+                              const double *__restrict__ cols_c, double *__restrict__ ab,
+                              double *__restrict__ cols_solved, unsigned char *__restrict__ conv, double tol,
+                              double floor_abund, unsigned *__restrict__ nconv, unsigned *__restrict__ nfail) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nact) return;
+  const int p = idx[c];
+  double dmax = 0.0;
+  for (int i = 0; i < nspec; i++) {
+    const double xn = ab_c[(long long)c * nspec + i], xo = ab[(long long)p * nspec + i];
+    const double m = fmax(fabs(xn), fabs(xo));
+    if (m > floor_abund) dmax = fmax(dmax, fabs(xn - xo) / m);
+    ab[(long long)p * nspec + i] = xn;
+  }
+  for (int k = 0; k < ncol; k++) cols_solved[(long long)p * ncol + k] = cols_c[(long long)c * ncol + k];
+  const bool cv = status_c[c] == 0 && dmax < tol;
+  conv[p] = cv ? 1 : 0;
+  if (cv) atomicAdd(nconv, 1u);
+  if (status_c[c] != 0) atomicAdd(nfail, 1u);
+}
+
+struct ToLong {
+  __host__ __device__ long long operator()(int v) const { return v; }
+};
+
+inline unsigned blocks_for(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// the cloud object
+// ---------------------------------------------------------------------------
+struct pdrchem_cloud {
+  int net, nspec, nreac, nelect, nx, ny, nz, npts, nrays, level;
+  double h, theta_crit;
+  pdrchem_phys phys;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // ray tracer
+  int *d_count = nullptr, *d_proj = nullptr;
+  long long *d_offs = nullptr, total = 0;
+  double *d_step = nullptr;
+  // per point
+  double *d_rho = nullptr, *d_T = nullptr, *d_Td = nullptr, *d_uv = nullptr;
+  double *d_G = nullptr, *d_Av = nullptr, *d_cols = nullptr, *d_cols_solved = nullptr, *d_ab = nullptr;
+  unsigned char *d_conv = nullptr, *d_active = nullptr;
+  int *d_idx = nullptr, *d_nsel = nullptr;
+  // compact buffers of the active points
+  double *c_T = nullptr, *c_Td = nullptr, *c_G = nullptr, *c_Av = nullptr, *c_cols = nullptr, *c_rho = nullptr;
+  double *c_ab = nullptr, *c_rate = nullptr;
+  int *c_status = nullptr, *d_neg = nullptr;
+  unsigned *d_counter = nullptr, *d_nconv = nullptr, *d_nfail = nullptr;
+  void *d_cub = nullptr;
+  size_t cub_bytes = 0;
+  int sel[5] = {-1, -1, -1, -1, -1};
+  double av_fac = 0.0, uv_fac = 0.0;
+  bool have_ab = false;
+  std::string err;
+  std::vector<void *> allocs;
+
+  template <class T>
+  bool alloc(T **p, size_t n, const char *what) {
+    void *q = nullptr;
+    if (!ok(hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T)), what)) return false;
+    allocs.push_back(q);
+    *p = (T *)q;
+    return true;
+  }
+  ~pdrchem_cloud() {
+    if (stream) hipStreamSynchronize(stream);
+    for (void *q : allocs) hipFree(q);
+    if (ev0) hipEventDestroy(ev0);
+    if (ev1) hipEventDestroy(ev1);
+    if (stream) hipStreamDestroy(stream);
+  }
+  bool fail(const char *what) {
+    err = t_err.empty() ? std::string(what) : std::string(what) + ": " + t_err;
+    return false;
+  }
+};
+
+namespace {
+
+bool build_rays(pdrchem_cloud *c) {
+  const int nx = c->nx, ny = c->ny, nz = c->nz, nside = 1 << c->level, nrays = c->nrays;
+  // offsets of every pixel, sorted by (|o|^2, linear offset)
+  struct Ent {
+    int r2;
+    long long lin;
+    unsigned packed;
+  };
+  std::vector<std::vector<Ent>> per(nrays);
+  for (int dz = -(nz - 1); dz <= nz - 1; dz++)
+    for (int dy = -(ny - 1); dy <= ny - 1; dy++)
+      for (int dx = -(nx - 1); dx <= nx - 1; dx++) {
+        if (!dx && !dy && !dz) continue;
+        const double r[3] = {dx * c->h, dy * c->h, dz * c->h};
+        const int j = pix_of(nside, r);
+        per[j].push_back(Ent{dx * dx + dy * dy + dz * dz, ((long long)dz * ny + dy) * nx + dx,
+                             (unsigned)(dx + 128) | ((unsigned)(dy + 128) << 8) | ((unsigned)(dz + 128) << 16)});
+      }
+  std::vector<unsigned> tab;
+  std::vector<int> r2, ptr(nrays + 1, 0);
+  for (int j = 0; j < nrays; j++) {
+    std::sort(per[j].begin(), per[j].end(),
+              [](const Ent &a, const Ent &b) { return a.r2 != b.r2 ? a.r2 < b.r2 : a.lin < b.lin; });
+    ptr[j] = (int)tab.size();
+    for (const Ent &e : per[j]) {
+      tab.push_back(e.packed);
+      r2.push_back(e.r2);
+    }
+  }
+  ptr[nrays] = (int)tab.size();
+  std::vector<double> vec(3 * nrays);
+  for (int j = 0; j < nrays; j++) pix_centre(nside, j, &vec[3 * j]);
+
+  unsigned *d_tab = nullptr;
+  int *d_r2 = nullptr, *d_ptr = nullptr;
+  double *d_vec = nullptr;
+  const long long nt = (long long)c->npts * nrays;
+  if (!c->alloc(&d_tab, tab.size(), "ray table") || !c->alloc(&d_r2, r2.size(), "ray table") ||
+      !c->alloc(&d_ptr, ptr.size(), "ray table") || !c->alloc(&d_vec, vec.size(), "ray vectors") ||
+      !c->alloc(&c->d_count, nt, "ray counts") || !c->alloc(&c->d_offs, nt + 1, "ray offsets"))
+    return c->fail("allocation");
+  hipStream_t s = c->stream;
+  if (!ok(hipMemcpyAsync(d_tab, tab.data(), tab.size() * 4, hipMemcpyHostToDevice, s), "copy") ||
+      !ok(hipMemcpyAsync(d_r2, r2.data(), r2.size() * 4, hipMemcpyHostToDevice, s), "copy") ||
+      !ok(hipMemcpyAsync(d_ptr, ptr.data(), ptr.size() * 4, hipMemcpyHostToDevice, s), "copy") ||
+      !ok(hipMemcpyAsync(d_vec, vec.data(), vec.size() * 8, hipMemcpyHostToDevice, s), "copy"))
+    return c->fail("upload of ray tables");
+  RayGrid g{nx, ny, nz, nrays, c->h, c->theta_crit};
+  const dim3 grid(blocks_for(c->npts, 256), nrays);
+  hipLaunchKernelGGL(ray_walk_kernel<false>, grid, dim3(256), 0, s, g, d_tab, d_r2, d_ptr, d_vec, c->d_count,
+                     nullptr, nullptr, nullptr);
+  if (!ok(hipGetLastError(), "ray count kernel")) return c->fail("ray tracer");
+  // exclusive scan of counts into 64-bit offsets (offs[nt] = total)
+  size_t tb = 0;
+  hipcub::TransformInputIterator<long long, ToLong, const int *> in(c->d_count, ToLong());
+  if (!ok(hipcub::DeviceScan::InclusiveSum(nullptr, tb, in, c->d_offs + 1, nt, s), "scan size"))
+    return c->fail("ray tracer");
+  void *tmp = nullptr;
+  if (!c->alloc((char **)&tmp, tb, "scan workspace")) return c->fail("allocation");
+  if (!ok(hipMemsetAsync(c->d_offs, 0, sizeof(long long), s), "memset") ||
+      !ok(hipcub::DeviceScan::InclusiveSum(tmp, tb, in, c->d_offs + 1, nt, s), "scan"))
+    return c->fail("ray tracer");
+  if (!ok(hipMemcpyAsync(&c->total, c->d_offs + nt, sizeof(long long), hipMemcpyDeviceToHost, s), "copy") ||
+      !ok(hipStreamSynchronize(s), "sync"))
+    return c->fail("ray tracer");
+  if (!c->alloc(&c->d_proj, c->total, "evaluation points") || !c->alloc(&c->d_step, c->total, "evaluation points"))
+    return c->fail("allocation");
+  hipLaunchKernelGGL(ray_walk_kernel<true>, grid, dim3(256), 0, s, g, d_tab, d_r2, d_ptr, d_vec, c->d_count,
+                     c->d_offs, c->d_proj, c->d_step);
+  if (!ok(hipGetLastError(), "ray fill kernel")) return c->fail("ray tracer");
+  return true;
+}
+
+bool columns(pdrchem_cloud *c, bool with_species) {
+  ColumnArgs a{};
+  a.npts = c->npts;
+  a.nrays = c->nrays;
+  a.nspec = c->nspec;
+  for (int k = 0; k < 5; k++) a.sel[k] = c->sel[k];
+  a.av_fac = c->av_fac;
+  a.offs = c->d_offs;
+  a.count = c->d_count;
+  a.proj = c->d_proj;
+  a.step = c->d_step;
+  a.rho = c->d_rho;
+  a.ab = c->d_ab;
+  a.Av = c->d_Av;
+  a.cols = with_species ? c->d_cols : nullptr;
+  hipLaunchKernelGGL(ray_columns_kernel, dim3(blocks_for((long long)c->npts * c->nrays, 256)), dim3(256), 0,
+                     c->stream, a);
+  return ok(hipGetLastError(), "column kernel") || c->fail("columns");
+}
+
+}  // namespace
+
+extern "C" {
+
+pdrchem_cloud *pdrchem_cloud_create(int net, int nx, int ny, int nz, double spacing_pc, const double *density,
+                                    int level, double theta_crit, int fieldchoice, const double *gext,
+                                    double dust_temperature, const pdrchem_phys *phys, const int *shield_slots,
+                                    int nelect) {
+  t_err.clear();
+  int nspec = 0, nreac = 0;
+  const int shapes[3][2] = {{33, 329}, {215, 2926}, {58, 634}};
+  if (net < 0 || net > 2 || pdrchem_network_id(shapes[net][0], shapes[net][1]) != net) {
+    t_err = "unsupported network id";
+    return nullptr;
+  }
+  nspec = shapes[net][0];
+  nreac = shapes[net][1];
+  if (nx < 2 || ny < 2 || nz < 2 || nx > 128 || ny > 128 || nz > 128 || !(spacing_pc > 0) || !density || !gext ||
+      !phys || level < 0 || level > 3 || !(theta_crit > 0) || theta_crit >= 1.5707963267948966 ||
+      nelect < 1 || nelect > nspec || (fieldchoice != 0 && fieldchoice != 1)) {
+    t_err = "bad argument to pdrchem_cloud_create (grid sides 2..128, HEALPix level 0..3, 0 < theta_crit < pi/2)";
+    return nullptr;
+  }
+  pdrchem_cloud *c = new pdrchem_cloud();
+  c->net = net;
+  c->nspec = nspec;
+  c->nreac = nreac;
+  c->nelect = nelect;
+  c->nx = nx;
+  c->ny = ny;
+  c->nz = nz;
+  c->npts = nx * ny * nz;
+  c->level = level;
+  c->nrays = 12 << (2 * level);
+  c->h = spacing_pc;
+  c->theta_crit = theta_crit;
+  c->phys = *phys;
+  c->av_fac = (double)6.289e-22f * phys->metallicity;  // input_parameters.F90:57 (single-precision literal)
+  c->uv_fac = (double)3.02f;                           // input_parameters.F90:58
+  for (int k = 0; k < 5; k++) c->sel[k] = shield_slots ? shield_slots[k] : -1;
+  const int n = c->npts, nr = c->nrays;
+  const long long nc = (long long)n * nr * 5;
+  bool good = ok(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "stream") &&
+              ok(hipEventCreate(&c->ev0), "event") && ok(hipEventCreate(&c->ev1), "event") &&
+              c->alloc(&c->d_rho, n, "density") && c->alloc(&c->d_T, n, "T") && c->alloc(&c->d_Td, n, "Td") &&
+              c->alloc(&c->d_uv, n, "uv") && c->alloc(&c->d_G, (size_t)n * nr, "G") &&
+              c->alloc(&c->d_Av, (size_t)n * nr, "Av") && c->alloc(&c->d_cols, nc, "columns") &&
+              c->alloc(&c->d_cols_solved, nc, "columns") && c->alloc(&c->d_ab, (size_t)n * nspec, "abundances") &&
+              c->alloc(&c->d_conv, n, "flags") && c->alloc(&c->d_active, n, "flags") &&
+              c->alloc(&c->d_idx, n, "index") && c->alloc(&c->d_nsel, 1, "count") &&
+              c->alloc(&c->c_T, n, "compact") && c->alloc(&c->c_Td, n, "compact") &&
+              c->alloc(&c->c_G, (size_t)n * nr, "compact") && c->alloc(&c->c_Av, (size_t)n * nr, "compact") &&
+              c->alloc(&c->c_cols, nc, "compact") && c->alloc(&c->c_rho, n, "compact") &&
+              c->alloc(&c->c_ab, (size_t)n * nspec, "compact") && c->alloc(&c->c_rate, (size_t)n * nreac, "rates") &&
+              c->alloc(&c->c_status, n, "status") && c->alloc(&c->d_neg, 1, "count") &&
+              c->alloc(&c->d_counter, 1, "count") && c->alloc(&c->d_nconv, 1, "count") &&
+              c->alloc(&c->d_nfail, 1, "count");
+  if (!good) {
+    delete c;
+    return nullptr;
+  }
+  hipStream_t s = c->stream;
+  // per-ray surface field (UNI: only the ray most opposed to gext)
+  std::vector<double> vec(3 * nr), gray(nr, 0.0);
+  for (int j = 0; j < nr; j++) pix_centre(1 << level, j, &vec[3 * j]);
+  int jmin = 0;
+  double best = 0.0;
+  for (int j = 0; j < nr; j++) {
+    const double d = -(gext[0] * vec[3 * j] + gext[1] * vec[3 * j + 1] + gext[2] * vec[3 * j + 2]);
+    if (d >= best) {
+      best = d;
+      jmin = j;
+    }
+  }
+  for (int j = 0; j < nr; j++) {
+    if (fieldchoice == 0)
+      gray[j] = gext[0] / (double)nr;
+    else if (j == jmin)
+      gray[j] = -(gext[0] * vec[3 * j] + gext[1] * vec[3 * j + 1] + gext[2] * vec[3 * j + 2]);
+  }
+  double *d_gray = nullptr;
+  good = c->alloc(&d_gray, nr, "field") &&
+         ok(hipMemcpyAsync(d_gray, gray.data(), nr * 8, hipMemcpyHostToDevice, s), "copy") &&
+         ok(hipMemcpyAsync(c->d_rho, density, (size_t)n * 8, hipMemcpyHostToDevice, s), "copy") &&
+         ok(hipMemsetAsync(c->d_conv, 0, n, s), "memset") && build_rays(c) && columns(c, false);
+  if (good) {
+    hipLaunchKernelGGL(uvfield_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, nr, d_gray, c->d_Av,
+                       c->uv_fac, c->d_G, c->d_uv, c->d_T);
+    hipLaunchKernelGGL(fill_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, c->d_Td, (long long)n,
+                       dust_temperature);
+    good = ok(hipGetLastError(), "field kernels") && ok(hipStreamSynchronize(s), "sync");
+  }
+  if (good) {
+    // stream-compaction workspace
+    hipcub::CountingInputIterator<int> it(0);
+    if (!ok(hipcub::DeviceSelect::Flagged(nullptr, c->cub_bytes, it, c->d_active, c->d_idx, c->d_nsel, n, s),
+            "select size") ||
+        !c->alloc((char **)&c->d_cub, c->cub_bytes, "select workspace"))
+      good = false;
+  }
+  if (!good) {
+    t_err = c->err.empty() ? t_err : c->err;
+    delete c;
+    return nullptr;
+  }
+  return c;
+}
+
+int pdrchem_cloud_set_abundances(pdrchem_cloud *c, const double *abundance) {
+  if (!c || !abundance) return -1;
+  c->err.clear();
+  t_err.clear();
+  const size_t b = (size_t)c->npts * c->nspec * 8;
+  if (!ok(hipMemcpyAsync(c->d_ab, abundance, b, hipMemcpyHostToDevice, c->stream), "copy") ||
+      !ok(hipMemsetAsync(c->d_conv, 0, c->npts, c->stream), "memset") || !columns(c, true) ||
+      !ok(hipStreamSynchronize(c->stream), "sync")) {
+    c->fail("set abundances");
+    return -1;
+  }
+  c->have_ab = true;
+  return 0;
+}
+
+int pdrchem_cloud_iterate(pdrchem_cloud *c, double t_end_years, double conv_tol, int precision,
+                          pdrchem_cloud_stats *st) {
+  if (!c) return -1;
+  c->err.clear();
+  t_err.clear();
+  if (!c->have_ab) {
+    c->err = "pdrchem_cloud_iterate before pdrchem_cloud_set_abundances";
+    return -1;
+  }
+  if (!(t_end_years > 0) || !(conv_tol >= 0) || (precision != PDRCHEM_FP64 && precision != PDRCHEM_MIXED)) {
+    c->err = "bad argument to pdrchem_cloud_iterate";
+    return -1;
+  }
+  hipStream_t s = c->stream;
+  const int n = c->npts, nr = c->nrays, ncol = c->nrays * 5;
+  // 1. which points to solve
+  hipLaunchKernelGGL(activity_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, ncol, c->d_cols,
+                     c->d_cols_solved, c->d_conv, conv_tol, c->d_active);
+  hipcub::CountingInputIterator<int> it(0);
+  int nact = 0;
+  if (!ok(hipGetLastError(), "activity kernel") ||
+      !ok(hipcub::DeviceSelect::Flagged(c->d_cub, c->cub_bytes, it, c->d_active, c->d_idx, c->d_nsel, n, s),
+          "select") ||
+      !ok(hipMemcpyAsync(&nact, c->d_nsel, sizeof(int), hipMemcpyDeviceToHost, s), "copy") ||
+      !ok(hipStreamSynchronize(s), "sync")) {
+    c->fail("active set");
+    return -1;
+  }
+  pdrchem_cloud_stats loc{};
+  loc.npts = n;
+  loc.solved = nact;
+  float ms_rates = 0.f, ms_solve = 0.f;
+  if (nact > 0) {
+    // 2. gather the active points' inputs, rates on device
+    hipLaunchKernelGGL(rows_kernel<false>, dim3(blocks_for(nact, 256)), dim3(256), 0, s, c->c_T, c->d_T, c->d_idx,
+                       nact, 1);
+    hipLaunchKernelGGL(rows_kernel<false>, dim3(blocks_for(nact, 256)), dim3(256), 0, s, c->c_Td, c->d_Td,
+                       c->d_idx, nact, 1);
+    hipLaunchKernelGGL(rows_kernel<false>, dim3(blocks_for(nact, 256)), dim3(256), 0, s, c->c_rho, c->d_rho,
+                       c->d_idx, nact, 1);
+    hipLaunchKernelGGL(rows_kernel<false>, dim3(blocks_for((long long)nact * nr, 256)), dim3(256), 0, s, c->c_G,
+                       c->d_G, c->d_idx, nact, nr);
+    hipLaunchKernelGGL(rows_kernel<false>, dim3(blocks_for((long long)nact * nr, 256)), dim3(256), 0, s, c->c_Av,
+                       c->d_Av, c->d_idx, nact, nr);
+    hipLaunchKernelGGL(rows_kernel<false>, dim3(blocks_for((long long)nact * ncol, 256)), dim3(256), 0, s,
+                       c->c_cols, c->d_cols, c->d_idx, nact, ncol);
+    hipLaunchKernelGGL(rows_kernel<false>, dim3(blocks_for((long long)nact * c->nspec, 256)), dim3(256), 0, s,
+                       c->c_ab, c->d_ab, c->d_idx, nact, c->nspec);
+    hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(1), 0, s, (unsigned *)c->d_neg);
+    hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(1), 0, s, c->d_counter);
+    hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(1), 0, s, c->d_nconv);
+    hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(1), 0, s, c->d_nfail);
+    if (!ok(hipGetLastError(), "gather kernels") || !ok(hipEventRecord(c->ev0, s), "event")) {
+      c->fail("gather");
+      return -1;
+    }
+    if (pdrchem_rates_device(c->net, &c->phys, c->c_T, c->c_Td, nr, c->c_G, c->c_Av, c->c_cols, nact, c->c_rate,
+                             c->d_neg, s) != 0) {
+      c->err = std::string("rates: ") + pdrchem_last_error();
+      return -1;
+    }
+    if (!ok(hipEventRecord(c->ev1, s), "event")) {
+      c->fail("rates");
+      return -1;
+    }
+    hipEvent_t e2 = nullptr, e3 = nullptr;
+    if (!ok(hipEventCreate(&e2), "event") || !ok(hipEventCreate(&e3), "event") ||
+        !ok(hipEventRecord(e2, s), "event")) {
+      c->fail("events");
+      return -1;
+    }
+    const double rtol = chemistry_module_mp_relative_abundance_tolerance_;
+    const double atol = chemistry_module_mp_absolute_abundance_tolerance_;
+    if (pdrchem_solve_device(c->net, c->c_ab, c->c_rate, c->c_rho, nact, rtol, atol, 0.0, t_end_years, c->nelect,
+                             c->c_status, nullptr, c->d_counter, precision, s) != 0) {
+      c->err = std::string("solve: ") + pdrchem_last_error();
+      hipEventDestroy(e2);
+      hipEventDestroy(e3);
+      return -1;
+    }
+    hipEventRecord(e3, s);
+    // 3. convergence, write-back
+    hipLaunchKernelGGL(retire_kernel, dim3(blocks_for(nact, 256)), dim3(256), 0, s, nact, c->nspec, ncol, c->d_idx,
+                       c->c_ab, c->c_status, c->c_cols, c->d_ab, c->d_cols_solved, c->d_conv, conv_tol, 1.0e-20,
+                       c->d_nconv, c->d_nfail);
+    unsigned cnt[3] = {0, 0, 0};
+    bool good = ok(hipGetLastError(), "retire kernel") && columns(c, true) &&
+                ok(hipMemcpyAsync(&cnt[0], c->d_nconv, 4, hipMemcpyDeviceToHost, s), "copy") &&
+                ok(hipMemcpyAsync(&cnt[1], c->d_nfail, 4, hipMemcpyDeviceToHost, s), "copy") &&
+                ok(hipMemcpyAsync(&cnt[2], c->d_neg, 4, hipMemcpyDeviceToHost, s), "copy") &&
+                ok(hipStreamSynchronize(s), "sync");
+    if (good) {
+      hipEventElapsedTime(&ms_rates, c->ev0, c->ev1);
+      hipEventElapsedTime(&ms_solve, e2, e3);
+    }
+    hipEventDestroy(e2);
+    hipEventDestroy(e3);
+    if (!good) {
+      c->fail("iteration");
+      return -1;
+    }
+    loc.converged = (int)cnt[0];
+    loc.failed = (int)cnt[1];
+    loc.negative_rates = (int)cnt[2];
+  }
+  loc.rates_ms = ms_rates;
+  loc.solve_ms = ms_solve;
+  if (st) *st = loc;
+  return 0;
+}
+
+int pdrchem_cloud_get(pdrchem_cloud *c, int field, double *out) {
+  if (!c || !out) return -1;
+  c->err.clear();
+  const size_t n = c->npts, nr = c->nrays;
+  const double *src = nullptr;
+  size_t cnt = 0;
+  switch (field) {
+    case PDRCHEM_CLOUD_ABUNDANCE: src = c->d_ab; cnt = n * c->nspec; break;
+    case PDRCHEM_CLOUD_AV: src = c->d_Av; cnt = n * nr; break;
+    case PDRCHEM_CLOUD_G: src = c->d_G; cnt = n * nr; break;
+    case PDRCHEM_CLOUD_COLUMNS: src = c->d_cols; cnt = n * nr * 5; break;
+    case PDRCHEM_CLOUD_TEMPERATURE: src = c->d_T; cnt = n; break;
+    case PDRCHEM_CLOUD_UVFIELD: src = c->d_uv; cnt = n; break;
+    default:
+      c->err = "unknown field";
+      return -1;
+  }
+  if (!ok(hipMemcpyAsync(out, src, cnt * 8, hipMemcpyDeviceToHost, c->stream), "copy") ||
+      !ok(hipStreamSynchronize(c->stream), "sync")) {
+    c->fail("get");
+    return -1;
+  }
+  return 0;
+}
+
+long long pdrchem_cloud_num_evaluation_points(const pdrchem_cloud *c) { return c ? c->total : -1; }
+
+int pdrchem_cloud_evaluation_points(pdrchem_cloud *c, int *count, long long *offsets, int *proj, double *step) {
+  if (!c) return -1;
+  c->err.clear();
+  hipStream_t s = c->stream;
+  const size_t nt = (size_t)c->npts * c->nrays;
+  bool good = (!count || ok(hipMemcpyAsync(count, c->d_count, nt * 4, hipMemcpyDeviceToHost, s), "copy")) &&
+              (!offsets || ok(hipMemcpyAsync(offsets, c->d_offs, (nt + 1) * 8, hipMemcpyDeviceToHost, s), "copy")) &&
+              (!proj || ok(hipMemcpyAsync(proj, c->d_proj, (size_t)c->total * 4, hipMemcpyDeviceToHost, s), "copy")) &&
+              (!step || ok(hipMemcpyAsync(step, c->d_step, (size_t)c->total * 8, hipMemcpyDeviceToHost, s), "copy")) &&
+              ok(hipStreamSynchronize(s), "sync");
+  if (!good) {
+    c->fail("evaluation points");
+    return -1;
+  }
+  return 0;
+}
+
+const char *pdrchem_cloud_last_error(const pdrchem_cloud *c) {
+  if (!c) return t_err.c_str();
+  return c->err.c_str();
+}
+
+void pdrchem_cloud_destroy(pdrchem_cloud *c) { delete c; }
+
+}  // extern "C"

@@ -1,0 +1,107 @@
+"""Grid cloud: ray-traced column densities and the chemistry iteration, on
+the device (the pdrchem_cloud_* functions of libpdrchem.so).
+
+The reference runs these steps inside its main program: evaluation points
+along the HEALPix rays (evaluation_points, eval_points.F90:129-225), Av and
+the surface field per ray (CALC_UVFIELD, 3DPDR.F90:1636-1708), column
+densities (calc_columndens, 3DPDR.F90:1788-1844) and the chemistry
+iterations of rates, solve and new columns (3DPDR.F90:474-520).  Here a
+``Cloud`` holds a regular grid on the device and runs them there; only
+statistics and requested fields come back to the host.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import lib
+from .netgen import Network
+
+ABUNDANCE, AV, G, COLUMNS, TEMPERATURE, UVFIELD = range(6)
+SHIELD_SPECIES = ("H2", "HD", "CO", "C", "S")   # column order expected by the rate kernel
+ISO, UNI = 0, 1
+
+
+def shield_slots(net: Network) -> np.ndarray:
+    names = [s.name for s in net.species]
+    return np.array([names.index(s) if s in names else -1 for s in SHIELD_SPECIES], dtype=np.int32)
+
+
+class Cloud:
+    """A regular nx*ny*nz grid (point index ix + nx*(iy + ny*iz)) with
+    spacing ``spacing_pc`` and density n_H per point, illuminated by
+    ``gext`` (UNI: a vector in Draine units; ISO: gext[0] spread over the
+    rays), HEALPix ``level`` and ``theta_crit`` as params.dat."""
+
+    def __init__(self, net: Network, shape: Sequence[int], spacing_pc: float, density: np.ndarray,
+                 gext=(100.0, 0.0, 0.0), fieldchoice: int = UNI, level: int = 0, theta_crit: float = 1.3,
+                 dust_temperature: float = 20.0, phys: Optional[lib.Phys] = None):
+        L = lib.load()
+        self.net = net
+        self.nx, self.ny, self.nz = (int(v) for v in shape)
+        self.npts = self.nx * self.ny * self.nz
+        self.nrays = 12 << (2 * level)
+        self.nspec = net.nspec
+        rho = np.ascontiguousarray(density, dtype=np.float64).reshape(-1)
+        if rho.shape[0] != self.npts:
+            raise lib.PdrChemError("density must have nx*ny*nz entries")
+        nid = lib.network_id(net.name)
+        g = np.ascontiguousarray(gext, dtype=np.float64)
+        self._phys = phys or lib.default_phys()
+        slots = shield_slots(net)
+        dp = C.POINTER(C.c_double)
+        self._h = L.pdrchem_cloud_create(nid, self.nx, self.ny, self.nz, spacing_pc, rho.ctypes.data_as(dp), level,
+                                         theta_crit, fieldchoice, g.ctypes.data_as(dp), dust_temperature,
+                                         C.byref(self._phys), slots.ctypes.data_as(C.POINTER(C.c_int)), net.nspec)
+        if not self._h:
+            raise lib.PdrChemError(L.pdrchem_cloud_last_error(None).decode())
+
+    def _err(self) -> str:
+        return lib.load().pdrchem_cloud_last_error(self._h).decode()
+
+    def set_abundances(self, x: np.ndarray) -> None:
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        if x.shape != (self.npts, self.nspec):
+            raise lib.PdrChemError(f"abundances must be [{self.npts}, {self.nspec}]")
+        if lib.load().pdrchem_cloud_set_abundances(self._h, x.ctypes.data_as(C.POINTER(C.c_double))) != 0:
+            raise lib.PdrChemError(self._err())
+
+    def iterate(self, t_end: float = 1e7, conv_tol: float = 1e-3, precision: int = lib.MIXED) -> dict:
+        st = lib.CloudStats()
+        if lib.load().pdrchem_cloud_iterate(self._h, t_end, conv_tol, precision, C.byref(st)) != 0:
+            raise lib.PdrChemError(self._err())
+        return {k: getattr(st, k) for k, _ in st._fields_}
+
+    def get(self, field: int) -> np.ndarray:
+        shapes = {ABUNDANCE: (self.npts, self.nspec), AV: (self.npts, self.nrays), G: (self.npts, self.nrays),
+                  COLUMNS: (self.npts, self.nrays, 5), TEMPERATURE: (self.npts,), UVFIELD: (self.npts,)}
+        out = np.zeros(shapes[field])
+        if lib.load().pdrchem_cloud_get(self._h, field, out.ctypes.data_as(C.POINTER(C.c_double))) != 0:
+            raise lib.PdrChemError(self._err())
+        return out
+
+    def evaluation_points(self):
+        """(count[npts, nrays], offsets[npts*nrays+1], proj[total], step[total])."""
+        L = lib.load()
+        total = L.pdrchem_cloud_num_evaluation_points(self._h)
+        count = np.zeros((self.npts, self.nrays), dtype=np.int32)
+        offs = np.zeros(self.npts * self.nrays + 1, dtype=np.int64)
+        proj = np.zeros(max(total, 1), dtype=np.int32)
+        step = np.zeros(max(total, 1))
+        rc = L.pdrchem_cloud_evaluation_points(self._h, count.ctypes.data_as(C.POINTER(C.c_int)),
+                                               offs.ctypes.data_as(C.POINTER(C.c_longlong)),
+                                               proj.ctypes.data_as(C.POINTER(C.c_int)),
+                                               step.ctypes.data_as(C.POINTER(C.c_double)))
+        if rc != 0:
+            raise lib.PdrChemError(self._err())
+        return count, offs, proj[:total], step[:total]
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib.load().pdrchem_cloud_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()

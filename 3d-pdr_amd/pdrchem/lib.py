@@ -26,6 +26,9 @@ EXPORTED_FUNCTIONS = [
     "pdrchem_rates_device", "pdrchem_rates_device_zeta", "pdrchem_kernel_info", "pdrchem_set_max_blocks", "pdrchem_last_error",
     "pdrchem_server_create", "pdrchem_server_submit", "pdrchem_server_progress", "pdrchem_server_poll",
     "pdrchem_server_last_error", "pdrchem_server_destroy",
+    "pdrchem_cloud_create", "pdrchem_cloud_set_abundances", "pdrchem_cloud_iterate", "pdrchem_cloud_get",
+    "pdrchem_cloud_num_evaluation_points", "pdrchem_cloud_evaluation_points", "pdrchem_cloud_last_error",
+    "pdrchem_cloud_destroy",
 ]
 EXPORTED_GLOBALS = [
     ("chemistry_module_mp_relative_abundance_tolerance_", C.c_double),
@@ -48,6 +51,11 @@ def default_phys(**kw) -> Phys:
              metallicity=1.0, g2d=100.0)
     d.update(kw)
     return Phys(**d)
+
+
+class CloudStats(C.Structure):
+    _fields_ = [("npts", C.c_int), ("solved", C.c_int), ("converged", C.c_int), ("failed", C.c_int),
+                ("negative_rates", C.c_int), ("rates_ms", C.c_double), ("solve_ms", C.c_double)]
 
 
 class PdrChemError(RuntimeError):
@@ -91,6 +99,19 @@ def load(path: Optional[str] = None):
     L.pdrchem_server_last_error.restype = C.c_char_p
     L.pdrchem_server_destroy.argtypes = [vp]
     L.pdrchem_server_destroy.restype = None
+    L.pdrchem_cloud_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, dp, C.c_int, C.c_double,
+                                       C.c_int, dp, C.c_double, C.POINTER(Phys), ip, C.c_int]
+    L.pdrchem_cloud_create.restype = vp
+    L.pdrchem_cloud_set_abundances.argtypes = [vp, dp]
+    L.pdrchem_cloud_iterate.argtypes = [vp, C.c_double, C.c_double, C.c_int, C.POINTER(CloudStats)]
+    L.pdrchem_cloud_get.argtypes = [vp, C.c_int, dp]
+    L.pdrchem_cloud_num_evaluation_points.argtypes = [vp]
+    L.pdrchem_cloud_num_evaluation_points.restype = C.c_longlong
+    L.pdrchem_cloud_evaluation_points.argtypes = [vp, ip, C.POINTER(C.c_longlong), ip, dp]
+    L.pdrchem_cloud_last_error.argtypes = [vp]
+    L.pdrchem_cloud_last_error.restype = C.c_char_p
+    L.pdrchem_cloud_destroy.argtypes = [vp]
+    L.pdrchem_cloud_destroy.restype = None
     _lib = L
     return L
 

@@ -127,3 +127,23 @@ def request_stream(duration: float = 60.0, rate: float = 200.0, seed: int = 2025
         out.append(Request(float(arrival[k]), nm, int(npts[k]), float(t_end[k]), firsts[nm]))
         firsts[nm] += int(npts[k])
     return out
+
+
+def cloud_density(side: int = 64, mean: float = 1.0e3, sigma_ln: float = 1.0, corr_cells: float = 4.0,
+                  seed: int = 20251205) -> np.ndarray:
+    """Lognormal n_H on a side^3 grid (index ix + side*(iy + side*iz)):
+    ln n_H is a Gaussian random field (white noise from the counter-based
+    generator, smoothed by a periodic Gaussian of ``corr_cells`` cells and
+    rescaled to standard deviation ``sigma_ln``) with mean ln(mean) -
+    sigma_ln^2/2, so that the mean of n_H is ``mean``."""
+    n = side ** 3
+    idx = np.arange(n, dtype=np.uint64)
+    u1 = np.clip(uniform(seed, 11, idx), 1e-300, 1.0)
+    u2 = uniform(seed, 12, idx)
+    w = np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * np.pi * u2)
+    w = w.reshape(side, side, side)           # [iz, iy, ix]
+    k = np.fft.fftfreq(side)
+    kk = k[:, None, None] ** 2 + k[None, :, None] ** 2 + k[None, None, :] ** 2
+    f = np.real(np.fft.ifftn(np.fft.fftn(w) * np.exp(-2.0 * (np.pi * corr_cells) ** 2 * kk)))
+    f = (f - f.mean()) / f.std()
+    return np.exp(np.log(mean) - 0.5 * sigma_ln ** 2 + sigma_ln * f).reshape(-1)

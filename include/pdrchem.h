@@ -158,6 +158,65 @@ int pdrchem_server_poll(pdrchem_server *server, int max, int *tickets, int *stat
 const char *pdrchem_server_last_error(pdrchem_server *server);
 void pdrchem_server_destroy(pdrchem_server *server);
 
+/* ---------------------------------------------------------------------
+ * Grid cloud: ray-traced column densities and the chemistry iteration
+ * ------------------------------------------------------------------- */
+/* A regular nx*ny*nz grid of points (sides 2..128, spacing in pc; point
+ * index ix + nx*(iy + ny*iz)) held on the device with everything the
+ * chemistry iteration of 3DPDR.F90:474-520 needs: evaluation points along
+ * the 12*4^level HEALPix rays (evaluation_points, eval_points.F90:129-225, in
+ * (distance, point index) order, no maxpoints cap), Av per ray and the
+ * surface field per ray (CALC_UVFIELD, 3DPDR.F90:1636-1694; fieldchoice 0 =
+ * "ISO", 1 = "UNI" with vector gext[3]), the GUESS_TEMP gas temperature
+ * (3DPDR.F90:1698-1700, held fixed), dust temperature, abundances, and the
+ * columns of H2, HD, CO, C, S per ray (calc_columndens, 3DPDR.F90:1788-1844).
+ * shield_slots[5]: 0-based abundance slots of H2, HD, CO, C, S (-1 =
+ * absent); nelect 1-based electron slot.  NULL on failure
+ * (pdrchem_cloud_last_error(NULL) says why).  No counterpart in the
+ * reference as one call; the reference computes the same quantities inside
+ * its main program. */
+typedef struct pdrchem_cloud pdrchem_cloud;
+typedef struct {
+  int npts;            /* points in the grid */
+  int solved;          /* points solved in this iteration */
+  int converged;       /* of those, points whose abundances moved by < conv_tol */
+  int failed;          /* of those, points whose solve failed (status != 0) */
+  int negative_rates;  /* points with a negative rate coefficient */
+  double rates_ms, solve_ms;  /* device time of the rate evaluation and of the solve */
+} pdrchem_cloud_stats;
+pdrchem_cloud *pdrchem_cloud_create(int net, int nx, int ny, int nz, double spacing_pc, const double *density,
+                                    int level, double theta_crit, int fieldchoice, const double *gext,
+                                    double dust_temperature, const pdrchem_phys *phys, const int *shield_slots,
+                                    int nelect);
+/* Abundances of every point (host [npts*nspec]); recomputes the columns and
+ * marks every point unconverged. */
+int pdrchem_cloud_set_abundances(pdrchem_cloud *cloud, const double *abundance);
+/* One chemistry iteration (3DPDR.F90:474-520): rates from the current
+ * columns, a solve from t = 0 to t_end_years warm-started from the current
+ * abundances (calculate_abundances_ semantics, tolerances from the globals
+ * above), write-back, new columns.  Points whose last solve moved their
+ * abundances (> 1e-20) by less than conv_tol relative, and whose columns have
+ * not moved by more than conv_tol since, are not solved again (the
+ * reference's converged-point skip, 3DPDR.F90:641-643, applied to the
+ * chemistry). */
+int pdrchem_cloud_iterate(pdrchem_cloud *cloud, double t_end_years, double conv_tol, int precision,
+                          pdrchem_cloud_stats *stats);
+#define PDRCHEM_CLOUD_ABUNDANCE 0    /* [npts*nspec] */
+#define PDRCHEM_CLOUD_AV 1           /* [npts*nrays] */
+#define PDRCHEM_CLOUD_G 2            /* [npts*nrays] surface field per ray */
+#define PDRCHEM_CLOUD_COLUMNS 3      /* [npts*nrays*5] H2, HD, CO, C, S (cm^-2) */
+#define PDRCHEM_CLOUD_TEMPERATURE 4  /* [npts] */
+#define PDRCHEM_CLOUD_UVFIELD 5      /* [npts] attenuated field */
+int pdrchem_cloud_get(pdrchem_cloud *cloud, int field, double *out);
+/* Evaluation points: total count, then per (point, ray) the count and CSR
+ * offset (offsets[npts*nrays+1]) and per evaluation point the projected
+ * point and the step |epoint_i - epoint_{i-1}| in pc (any pointer may be
+ * NULL). */
+long long pdrchem_cloud_num_evaluation_points(const pdrchem_cloud *cloud);
+int pdrchem_cloud_evaluation_points(pdrchem_cloud *cloud, int *count, long long *offsets, int *proj, double *step);
+const char *pdrchem_cloud_last_error(const pdrchem_cloud *cloud);
+void pdrchem_cloud_destroy(pdrchem_cloud *cloud);
+
 #ifdef __cplusplus
 }
 #endif

@@ -272,3 +272,135 @@ def ref_rates(name: str, T, Td, G, Av, cols=None, phys: Optional[Phys] = None,
             continue
     vals = np.array(vals, dtype=np.float64)
     return vals.reshape(npts, o.nreac)
+
+
+# --------------------------------------------------------------------------
+# ray tracer (raytrace.c) and the reference's own (oracle/_ref/ref_raytrace)
+# --------------------------------------------------------------------------
+
+ORDER_HEAPSORT = 0
+ORDER_STABLE = 1
+AV_FAC = float(np.float32(6.289e-22))   # single-precision literal, input_parameters.F90:57
+UV_FAC = float(np.float32(3.02))        # input_parameters.F90:58
+PC_CM = 3.08568025e18                   # healpix_types.F90:98
+
+
+def _raylib():
+    L = lib()
+    if not getattr(L, "_ray_ready", False):
+        L.pdr_hp_pix2vec_nest.argtypes = [C.c_int, C.c_int, _dp]
+        L.pdr_hp_vec2pix_nest.argtypes = [C.c_int, _dp]
+        L.pdr_eval_points.argtypes = [C.c_int, _dp, C.POINTER(C.c_int), C.c_int, C.c_double, C.c_int, C.c_int,
+                                      C.POINTER(C.c_int), C.POINTER(C.c_int), _dp]
+        L.pdr_ray_columns.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), _dp, _dp,
+                                      C.c_int, _dp, C.c_double, _dp, _dp]
+        L.pdr_uvfield.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, C.c_double, _dp, _dp, _dp]
+        L._ray_ready = True
+    return L
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def _d(a):
+    return a
+
+
+def hp_vectors(level: int) -> np.ndarray:
+    L = _raylib()
+    nside = 1 << level
+    out = np.zeros((12 * nside * nside, 3))
+    for j in range(out.shape[0]):
+        v = np.zeros(3)
+        L.pdr_hp_pix2vec_nest(nside, j, _d(v))
+        out[j] = v
+    return out
+
+
+def hp_vec2pix(level: int, r) -> int:
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    return int(_raylib().pdr_hp_vec2pix_nest(1 << level, _d(r)))
+
+
+def eval_points(xyz, level=0, theta_crit=1.3, order=ORDER_STABLE, maxpoints=600, etype=None):
+    """Evaluation points of a point cloud (eval_points.F90:129-225).
+    Returns (epray[n, nrays], proj[n, nrays, maxpoints+1], epoint[n, nrays, maxpoints+1, 3])."""
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+    n = xyz.shape[0]
+    nrays = 12 << (2 * level)
+    M = maxpoints + 1
+    epray = np.zeros((n, nrays), dtype=np.int32)
+    proj = np.zeros((n, nrays, M), dtype=np.int32)
+    epoint = np.zeros((n, nrays, M, 3))
+    et = None if etype is None else np.ascontiguousarray(etype, dtype=np.int32)
+    rc = _raylib().pdr_eval_points(n, _d(xyz), None if et is None else _ip(et), level, theta_crit, order,
+                                   maxpoints, _ip(epray), _ip(proj), _d(epoint))
+    if rc != 0:
+        raise RuntimeError("a ray needs more than maxpoints evaluation points")
+    return epray, proj, epoint
+
+
+def ray_columns(epray, proj, epoint, rho, xsel=None, av_fac=AV_FAC):
+    """Av[n, nrays] and columns[n, nrays, nsel] along the rays (3DPDR.F90:1650-1664, 1818-1838)."""
+    n, nrays, M = proj.shape
+    rho = np.ascontiguousarray(rho, dtype=np.float64)
+    nsel = 0 if xsel is None else xsel.shape[1]
+    xs = np.ascontiguousarray(xsel if xsel is not None else np.zeros((n, 1)), dtype=np.float64)
+    Av = np.zeros((n, nrays))
+    cols = np.zeros((n, nrays, max(nsel, 1)))
+    _raylib().pdr_ray_columns(n, nrays, M - 1, _ip(np.ascontiguousarray(epray)), _ip(np.ascontiguousarray(proj)),
+                              _d(np.ascontiguousarray(epoint)), _d(rho), nsel, _d(xs), av_fac, _d(Av), _d(cols))
+    return Av, cols[:, :, :nsel]
+
+
+def uvfield(Av, level=0, gext=(100.0, 0.0, 0.0), fieldchoice=1, uv_fac=UV_FAC):
+    """Per-ray surface field G, attenuated field and GUESS_TEMP temperature (3DPDR.F90:1667-1700)."""
+    Av = np.ascontiguousarray(Av, dtype=np.float64)
+    n, nrays = Av.shape
+    G = np.zeros((n, nrays))
+    uv = np.zeros(n)
+    tg = np.zeros(n)
+    g = np.ascontiguousarray(gext, dtype=np.float64)
+    _raylib().pdr_uvfield(n, level, fieldchoice, _d(g), _d(Av), uv_fac, _d(G), _d(uv), _d(tg))
+    return G, uv, tg
+
+
+def have_ref_raytrace() -> bool:
+    return os.path.exists(os.path.join(REF_BIN_DIR, "ref_raytrace"))
+
+
+def ref_eval_points(xyz, level=0, theta_crit=1.3, maxpoints=600, etype=None, workdir=None):
+    """Run the amdflang-built reference evaluation_points; same return layout as eval_points."""
+    import tempfile
+    xyz = np.asarray(xyz, dtype=np.float64).reshape(-1, 3)
+    n = xyz.shape[0]
+    et = np.ones(n, dtype=int) if etype is None else np.asarray(etype)
+    lines = [f"{n} {level} {theta_crit!r} {maxpoints}"]
+    lines += [f"{x!r} {y!r} {z!r} {int(t)}" for (x, y, z), t in zip(xyz.tolist(), et.tolist())]
+    with tempfile.TemporaryDirectory(dir=workdir) as td:
+        r = subprocess.run([os.path.join(REF_BIN_DIR, "ref_raytrace")], input="\n".join(lines) + "\n",
+                           capture_output=True, text=True, cwd=td, timeout=600)
+        if r.returncode != 0:
+            raise RuntimeError(f"reference ray tracer failed: {r.stdout[-1000:]} {r.stderr[-1000:]}")
+        txt = open(os.path.join(td, "raytrace_out.txt")).read().split("\n")
+    nrays = 12 << (2 * level)
+    M = maxpoints + 1
+    vec = np.array([[float(v) for v in txt[j].split()] for j in range(nrays)])
+    epray = np.zeros((n, nrays), dtype=np.int32)
+    proj = np.zeros((n, nrays, M), dtype=np.int32)
+    epoint = np.zeros((n, nrays, M, 3))
+    k = nrays
+    for p in range(n):
+        for j in range(nrays):
+            q, jj, cnt = (int(v) for v in txt[k].split())
+            k += 1
+            epray[p, j] = cnt
+            proj[p, j, 0] = p
+            epoint[p, j, 0] = xyz[p]
+            for i in range(1, cnt + 1):
+                f = txt[k].split()
+                k += 1
+                proj[p, j, i] = int(f[0])
+                epoint[p, j, i] = [float(v) for v in f[1:4]]
+    return vec, epray, proj, epoint

@@ -22,6 +22,10 @@
  *                    considerations"); the library itself is absent from the
  *                    reference tree (.MISSING_LARGE_BLOBS:1)
  *   pdr_calculate_abundances   calculate_abundances_ calculate_abundances.c:64-354
+ *   pdr_eval_points, pdr_ray_columns, pdr_uvfield, pdr_hp_*
+ *                    evaluation points, Av and column densities along the
+ *                    HEALPix rays (eval_points.F90:129-225, healpix.F90,
+ *                    3DPDR.F90:1636-1708,1818-1838), in raytrace.c
  */
 #ifndef PDR_ORACLE_H
 #define PDR_ORACLE_H
@@ -155,6 +159,19 @@ int pdr_integrate_generic(int n, pdr_rhs_fn f, pdr_jac_fn jac, void *ud,
                           const double *y0, double t0, int ntout,
                           const double *tout, double *yout,
                           const pdr_cvode_opts *opts, pdr_cvode_stats *st);
+
+/* ray tracer (raytrace.c) */
+#define PDR_ORDER_HEAPSORT 0  /* the reference's heapsort order (ties as it leaves them) */
+#define PDR_ORDER_STABLE 1    /* strict (distance, point index) order */
+void pdr_hp_pix2vec_nest(int nside, int ipix, double v[3]);
+int pdr_hp_vec2pix_nest(int nside, const double r[3]);
+int pdr_eval_points(int npts, const double *xyz, const int *etype, int level, double theta_crit, int order,
+                    int maxpoints, int *epray, int *proj, double *epoint);
+void pdr_ray_columns(int npts, int nrays, int maxpoints, const int *epray, const int *proj,
+                     const double *epoint, const double *rho, int nsel, const double *xsel, double av_fac,
+                     double *Av, double *cols);
+void pdr_uvfield(int npts, int level, int fieldchoice, const double gext[3], const double *Av, double uv_fac,
+                 double *G, double *uv, double *tguess);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,132 @@
+"""Device ray tracer, columns and chemistry iteration of a grid cloud
+(pdrchem_cloud_*) against the oracle: evaluation points bitwise equal to the
+oracle's (distance, index) order, Av / surface field / temperature / columns
+to rounding, and one chemistry iteration within the 1e-4 parity bar of the
+FP64 oracle fed with the oracle's own rates."""
+import numpy as np
+import pytest
+
+import oracle as O
+from pdrchem import cloud as CL
+from pdrchem import lib, workloads
+from pdrchem.netgen import load_network
+
+pytestmark = pytest.mark.gpu
+H = 0.0625
+
+
+def grid_xyz(nx, ny, nz):
+    iz, iy, ix = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    return np.stack([ix, iy, iz], -1).reshape(-1, 3).astype(float) * H
+
+
+def smooth_density(nx, ny, nz, seed=5):
+    rng = np.random.default_rng(seed)
+    xyz = grid_xyz(nx, ny, nz) / H
+    c = np.array([nx, ny, nz]) / 2.0
+    return 1e3 * np.exp(0.3 * np.sin(xyz[:, 0] / 2.0) + 0.2 * np.cos(xyz[:, 1] / 3.0)
+                        - 0.05 * np.sum((xyz - c) ** 2, axis=1) / max(nx, ny, nz) + 0.05 * rng.normal(size=len(xyz)))
+
+
+def oracle_chains(shape):
+    xyz = grid_xyz(*shape)
+    e, p, ep = O.eval_points(xyz, order=O.ORDER_STABLE, maxpoints=400)
+    return xyz, e, p, ep
+
+
+@pytest.mark.parametrize("shape", [(6, 7, 8), (9, 5, 4)])
+def test_evaluation_points_match_oracle(shape):
+    print(f"cloud evaluation points {shape}", flush=True)
+    net = load_network("reduced")
+    n = int(np.prod(shape))
+    c = CL.Cloud(net, shape, H, np.full(n, 1e3))
+    count, offs, proj, step = c.evaluation_points()
+    xyz, e, p, ep = oracle_chains(shape)
+    assert np.array_equal(count, e)
+    nr = e.shape[1]
+    for q in range(n):
+        for j in range(nr):
+            o = offs[q * nr + j]
+            k = e[q, j]
+            assert np.array_equal(proj[o:o + k], p[q, j, 1:k + 1]), (q, j)
+            d = np.linalg.norm(np.diff(ep[q, j, :k + 1], axis=0), axis=1)
+            assert np.allclose(step[o:o + k], d, rtol=1e-15, atol=0), (q, j)
+    c.close()
+
+
+def test_fields_and_columns_match_oracle():
+    shape = (7, 6, 5)
+    print(f"cloud fields {shape}", flush=True)
+    net = load_network("reduced")
+    n = int(np.prod(shape))
+    rho = smooth_density(*shape)
+    c = CL.Cloud(net, shape, H, rho, gext=(100.0, 0.0, 0.0))
+    x0 = workloads.initial_abundances(net, n)
+    x0 *= np.exp(0.1 * np.random.default_rng(2).normal(size=x0.shape))
+    c.set_abundances(x0)
+    xyz, e, p, ep = oracle_chains(shape)
+    slots = CL.shield_slots(net)
+    xs = np.where(slots[None, :] >= 0, x0[:, np.maximum(slots, 0)], 0.0)
+    Av, cols = O.ray_columns(e, p, ep, rho, xsel=xs)
+    G, uv, T = O.uvfield(Av, gext=(100.0, 0.0, 0.0))
+    assert np.allclose(c.get(CL.AV), Av, rtol=1e-14, atol=0)
+    assert np.allclose(c.get(CL.COLUMNS), cols, rtol=1e-14, atol=0)
+    assert np.array_equal(c.get(CL.G), G)
+    assert np.allclose(c.get(CL.UVFIELD), uv, rtol=1e-13, atol=1e-300)
+    assert np.allclose(c.get(CL.TEMPERATURE), T, rtol=1e-13, atol=0)
+    c.close()
+
+
+def test_iteration_matches_oracle_and_skips_converged():
+    shape = (5, 5, 4)
+    print(f"cloud iteration {shape}", flush=True)
+    net = load_network("reduced")
+    n = int(np.prod(shape))
+    rho = smooth_density(*shape)
+    c = CL.Cloud(net, shape, H, rho, gext=(100.0, 0.0, 0.0))
+    x0 = workloads.initial_abundances(net, n)
+    c.set_abundances(x0)
+    st = c.iterate(t_end=1e7, conv_tol=1e-3, precision=lib.MIXED)
+    assert st["solved"] == n and st["failed"] == 0 and st["negative_rates"] == 0
+    got = c.get(CL.ABUNDANCE)
+    # oracle pipeline: its own tracer, columns, field, rates and FP64 solve
+    xyz, e, p, ep = oracle_chains(shape)
+    slots = CL.shield_slots(net)
+    xs = np.where(slots[None, :] >= 0, x0[:, np.maximum(slots, 0)], 0.0)
+    Av, cols = O.ray_columns(e, p, ep, rho, xsel=xs)
+    G, uv, T = O.uvfield(Av, gext=(100.0, 0.0, 0.0))
+    rate = O.Oracle("reduced").rates(T, 20.0, G, Av, cols)
+    ref = x0.copy()
+    s, ref, _ = O.Oracle("reduced").calculate_abundances(ref, rate, rho, T, mode=O.MODE_GPU)
+    assert s == 0
+    m = ref[:, :-1] > 1e-20
+    err = np.max(np.abs(got[:, :-1][m] - ref[:, :-1][m]) / ref[:, :-1][m])
+    print(f"iteration 1 max rel err vs oracle {err:.2e}", flush=True)
+    assert err <= 1e-4
+    # new columns are those of the new abundances
+    xs1 = np.where(slots[None, :] >= 0, got[:, np.maximum(slots, 0)], 0.0)
+    _, cols1 = O.ray_columns(e, p, ep, rho, xsel=xs1)
+    assert np.allclose(c.get(CL.COLUMNS), cols1, rtol=1e-14, atol=0)
+    # second iteration: unconverged points re-solved from their warm start,
+    # points not solved keep their abundances bitwise
+    st2 = c.iterate(t_end=1e7, conv_tol=1e-3, precision=lib.MIXED)
+    got2 = c.get(CL.ABUNDANCE)
+    assert st2["failed"] == 0 and st2["solved"] <= n
+    moved = np.any(got2 != got, axis=1)
+    assert moved.sum() <= st2["solved"]
+    # a third iteration with a loose tolerance solves no more points than the second
+    st3 = c.iterate(t_end=1e7, conv_tol=1e-1, precision=lib.MIXED)
+    assert st3["solved"] <= st2["solved"]
+    c.close()
+
+
+def test_rejects_bad_arguments():
+    net = load_network("reduced")
+    with pytest.raises(lib.PdrChemError):
+        CL.Cloud(net, (1, 4, 4), H, np.ones(16))
+    with pytest.raises(lib.PdrChemError):
+        CL.Cloud(net, (4, 4, 4), H, np.ones(64), theta_crit=2.0)
+    c = CL.Cloud(net, (3, 3, 3), H, np.full(27, 1e3))
+    with pytest.raises(lib.PdrChemError):
+        c.iterate()           # abundances not set
+    c.close()
