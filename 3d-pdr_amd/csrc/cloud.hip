@@ -293,6 +293,9 @@ __global__ void fill_kernel(double *x, long long n, double v) {
     x[i] = v;
 }
 __global__ void zero_u32_kernel(unsigned *x) { *x = 0u; }
+__global__ void zero_u64_kernel(unsigned long long *x, int n) {
+  if ((int)threadIdx.x < n) x[threadIdx.x] = 0ull;
+}
 
 // rows of src selected by idx -> dst (gather) or back (scatter)
 template <bool SCATTER>
@@ -308,37 +311,24 @@ __global__ void rows_kernel(double *__restrict__ dst, const double *__restrict__
     dst[t] = src[q];
 }
 
-// which points to solve: not yet converged, or their shielding columns have
-// moved by more than tol since they were last solved
-__global__ void activity_kernel(int npts, int ncol, const double *__restrict__ cols,
-                                const double *__restrict__ cols_solved, const unsigned char *__restrict__ conv,
-                                double tol, unsigned char *__restrict__ active) {
+// which points to solve: those not yet converged (a converged point keeps
+// its abundances, as the reference keeps them, 3DPDR.F90:628)
+__global__ void activity_kernel(int npts, const unsigned char *__restrict__ conv, unsigned char *__restrict__ active) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= npts) return;
-  bool act = !conv[p];
-  if (!act) {
-    for (int k = 0; k < ncol; k++) {
-      const double a = cols[(long long)p * ncol + k], b = cols_solved[(long long)p * ncol + k];
-      const double m = fmax(fmax(fabs(a), fabs(b)), 1.0);
-      if (fabs(a - b) > tol * m) {
-        act = true;
-        break;
-      }
-    }
-  }
-  active[p] = act ? 1 : 0;
+  active[p] = conv[p] ? 0 : 1;
 }
 
 // after a solve of the active points (compact index c -> point idx[c]):
-// convergence test against the previous abundances, write-back, and record
-// the columns the point was solved with
-__global__ void retire_kernel(int nact, int nspec, int ncol, const int *__restrict__ idx,
-                              const double *__restrict__ ab_c, const int *__restrict__ status_c,
-                              const double *__restrict__ cols_c, double *__restrict__ ab,
-                              double *__restrict__ cols_solved, unsigned char *__restrict__ conv, double tol,
-                              double floor_abund, unsigned *__restrict__ nconv, unsigned *__restrict__ nfail) {
+// convergence test against the previous abundances and write-back
+__global__ void retire_kernel(int nact, int nspec, const int *__restrict__ idx, const double *__restrict__ ab_c,
+                              const int *__restrict__ status_c, double *__restrict__ ab,
+                              unsigned char *__restrict__ conv, double tol,
+                              double floor_abund, unsigned *__restrict__ nconv, unsigned *__restrict__ nfail,
+                              const int *__restrict__ stats_c, unsigned long long *__restrict__ ctr) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nact) return;
+  for (int k = 0; k < 5; k++) atomicAdd(&ctr[k], (unsigned long long)stats_c[(long long)c * 8 + k]);
   const int p = idx[c];
   double dmax = 0.0;
   for (int i = 0; i < nspec; i++) {
@@ -347,7 +337,6 @@ __global__ void retire_kernel(int nact, int nspec, int ncol, const int *__restri
     if (m > floor_abund) dmax = fmax(dmax, fabs(xn - xo) / m);
     ab[(long long)p * nspec + i] = xn;
   }
-  for (int k = 0; k < ncol; k++) cols_solved[(long long)p * ncol + k] = cols_c[(long long)c * ncol + k];
   const bool cv = status_c[c] == 0 && dmax < tol;
   conv[p] = cv ? 1 : 0;
   if (cv) atomicAdd(nconv, 1u);
@@ -377,13 +366,14 @@ struct pdrchem_cloud {
   double *d_step = nullptr;
   // per point
   double *d_rho = nullptr, *d_T = nullptr, *d_Td = nullptr, *d_uv = nullptr;
-  double *d_G = nullptr, *d_Av = nullptr, *d_cols = nullptr, *d_cols_solved = nullptr, *d_ab = nullptr;
+  double *d_G = nullptr, *d_Av = nullptr, *d_cols = nullptr, *d_ab = nullptr;
   unsigned char *d_conv = nullptr, *d_active = nullptr;
   int *d_idx = nullptr, *d_nsel = nullptr;
   // compact buffers of the active points
   double *c_T = nullptr, *c_Td = nullptr, *c_G = nullptr, *c_Av = nullptr, *c_cols = nullptr, *c_rho = nullptr;
   double *c_ab = nullptr, *c_rate = nullptr;
-  int *c_status = nullptr, *d_neg = nullptr;
+  int *c_status = nullptr, *c_stats = nullptr, *d_neg = nullptr;
+  unsigned long long *d_ctr = nullptr;
   unsigned *d_counter = nullptr, *d_nconv = nullptr, *d_nfail = nullptr;
   void *d_cub = nullptr;
   size_t cub_bytes = 0;
@@ -556,14 +546,15 @@ pdrchem_cloud *pdrchem_cloud_create(int net, int nx, int ny, int nz, double spac
               c->alloc(&c->d_rho, n, "density") && c->alloc(&c->d_T, n, "T") && c->alloc(&c->d_Td, n, "Td") &&
               c->alloc(&c->d_uv, n, "uv") && c->alloc(&c->d_G, (size_t)n * nr, "G") &&
               c->alloc(&c->d_Av, (size_t)n * nr, "Av") && c->alloc(&c->d_cols, nc, "columns") &&
-              c->alloc(&c->d_cols_solved, nc, "columns") && c->alloc(&c->d_ab, (size_t)n * nspec, "abundances") &&
+              c->alloc(&c->d_ab, (size_t)n * nspec, "abundances") &&
               c->alloc(&c->d_conv, n, "flags") && c->alloc(&c->d_active, n, "flags") &&
               c->alloc(&c->d_idx, n, "index") && c->alloc(&c->d_nsel, 1, "count") &&
               c->alloc(&c->c_T, n, "compact") && c->alloc(&c->c_Td, n, "compact") &&
               c->alloc(&c->c_G, (size_t)n * nr, "compact") && c->alloc(&c->c_Av, (size_t)n * nr, "compact") &&
               c->alloc(&c->c_cols, nc, "compact") && c->alloc(&c->c_rho, n, "compact") &&
               c->alloc(&c->c_ab, (size_t)n * nspec, "compact") && c->alloc(&c->c_rate, (size_t)n * nreac, "rates") &&
-              c->alloc(&c->c_status, n, "status") && c->alloc(&c->d_neg, 1, "count") &&
+              c->alloc(&c->c_status, n, "status") && c->alloc(&c->c_stats, (size_t)n * 8, "counters") &&
+              c->alloc(&c->d_ctr, 8, "counters") && c->alloc(&c->d_neg, 1, "count") &&
               c->alloc(&c->d_counter, 1, "count") && c->alloc(&c->d_nconv, 1, "count") &&
               c->alloc(&c->d_nfail, 1, "count");
   if (!good) {
@@ -648,8 +639,7 @@ int pdrchem_cloud_iterate(pdrchem_cloud *c, double t_end_years, double conv_tol,
   hipStream_t s = c->stream;
   const int n = c->npts, nr = c->nrays, ncol = c->nrays * 5;
   // 1. which points to solve
-  hipLaunchKernelGGL(activity_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, ncol, c->d_cols,
-                     c->d_cols_solved, c->d_conv, conv_tol, c->d_active);
+  hipLaunchKernelGGL(activity_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, c->d_conv, c->d_active);
   hipcub::CountingInputIterator<int> it(0);
   int nact = 0;
   if (!ok(hipGetLastError(), "activity kernel") ||
@@ -684,6 +674,7 @@ int pdrchem_cloud_iterate(pdrchem_cloud *c, double t_end_years, double conv_tol,
     hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(1), 0, s, c->d_counter);
     hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(1), 0, s, c->d_nconv);
     hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(1), 0, s, c->d_nfail);
+    hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, s, c->d_ctr, 8);
     if (!ok(hipGetLastError(), "gather kernels") || !ok(hipEventRecord(c->ev0, s), "event")) {
       c->fail("gather");
       return -1;
@@ -706,7 +697,7 @@ int pdrchem_cloud_iterate(pdrchem_cloud *c, double t_end_years, double conv_tol,
     const double rtol = chemistry_module_mp_relative_abundance_tolerance_;
     const double atol = chemistry_module_mp_absolute_abundance_tolerance_;
     if (pdrchem_solve_device(c->net, c->c_ab, c->c_rate, c->c_rho, nact, rtol, atol, 0.0, t_end_years, c->nelect,
-                             c->c_status, nullptr, c->d_counter, precision, s) != 0) {
+                             c->c_status, c->c_stats, c->d_counter, precision, s) != 0) {
       c->err = std::string("solve: ") + pdrchem_last_error();
       hipEventDestroy(e2);
       hipEventDestroy(e3);
@@ -714,14 +705,16 @@ int pdrchem_cloud_iterate(pdrchem_cloud *c, double t_end_years, double conv_tol,
     }
     hipEventRecord(e3, s);
     // 3. convergence, write-back
-    hipLaunchKernelGGL(retire_kernel, dim3(blocks_for(nact, 256)), dim3(256), 0, s, nact, c->nspec, ncol, c->d_idx,
-                       c->c_ab, c->c_status, c->c_cols, c->d_ab, c->d_cols_solved, c->d_conv, conv_tol, 1.0e-20,
-                       c->d_nconv, c->d_nfail);
+    hipLaunchKernelGGL(retire_kernel, dim3(blocks_for(nact, 256)), dim3(256), 0, s, nact, c->nspec, c->d_idx,
+                       c->c_ab, c->c_status, c->d_ab, c->d_conv, conv_tol, 1.0e-20,
+                       c->d_nconv, c->d_nfail, c->c_stats, c->d_ctr);
     unsigned cnt[3] = {0, 0, 0};
+    unsigned long long ctr[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     bool good = ok(hipGetLastError(), "retire kernel") && columns(c, true) &&
                 ok(hipMemcpyAsync(&cnt[0], c->d_nconv, 4, hipMemcpyDeviceToHost, s), "copy") &&
                 ok(hipMemcpyAsync(&cnt[1], c->d_nfail, 4, hipMemcpyDeviceToHost, s), "copy") &&
                 ok(hipMemcpyAsync(&cnt[2], c->d_neg, 4, hipMemcpyDeviceToHost, s), "copy") &&
+                ok(hipMemcpyAsync(ctr, c->d_ctr, sizeof(ctr), hipMemcpyDeviceToHost, s), "copy") &&
                 ok(hipStreamSynchronize(s), "sync");
     if (good) {
       hipEventElapsedTime(&ms_rates, c->ev0, c->ev1);
@@ -736,6 +729,11 @@ int pdrchem_cloud_iterate(pdrchem_cloud *c, double t_end_years, double conv_tol,
     loc.converged = (int)cnt[0];
     loc.failed = (int)cnt[1];
     loc.negative_rates = (int)cnt[2];
+    loc.nst = (long long)ctr[0];
+    loc.nfe = (long long)ctr[1];
+    loc.nje = (long long)ctr[2];
+    loc.nsetups = (long long)ctr[3];
+    loc.nni = (long long)ctr[4];
   }
   loc.rates_ms = ms_rates;
   loc.solve_ms = ms_solve;

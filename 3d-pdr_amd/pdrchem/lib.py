@@ -55,7 +55,9 @@ def default_phys(**kw) -> Phys:
 
 class CloudStats(C.Structure):
     _fields_ = [("npts", C.c_int), ("solved", C.c_int), ("converged", C.c_int), ("failed", C.c_int),
-                ("negative_rates", C.c_int), ("rates_ms", C.c_double), ("solve_ms", C.c_double)]
+                ("negative_rates", C.c_int), ("rates_ms", C.c_double), ("solve_ms", C.c_double),
+                ("nst", C.c_longlong), ("nfe", C.c_longlong), ("nje", C.c_longlong), ("nsetups", C.c_longlong),
+                ("nni", C.c_longlong)]
 
 
 class PdrChemError(RuntimeError):

@@ -107,6 +107,13 @@ def main():
     ap.add_argument("--serve-duration", type=float, default=30.0, help="seconds of arrivals")
     ap.add_argument("--serve-networks", default="reduced,full",
                     help="networks of the requests, chosen with equal odds")
+    ap.add_argument("--cloud", action="store_true",
+                    help="outer chemistry iteration over a lognormal grid cloud (ray-traced columns)")
+    ap.add_argument("--cloud-side", type=int, default=64, help="grid side (side^3 points)")
+    ap.add_argument("--cloud-iters", type=int, default=8, help="chemistry iterations (CHEMITERATIONS of params.dat)")
+    ap.add_argument("--cloud-tol", type=float, default=1e-3, help="convergence tolerance of a point")
+    ap.add_argument("--cloud-fp64-side", type=int, default=24,
+                    help="side of the cloud on which the plain FP64 form is compared (0 = skip)")
     a = ap.parse_args()
     if a.npts is None:
         a.npts = 1_000_000 if a.network == "reduced" else 20_000
@@ -121,6 +128,9 @@ def main():
 
     if a.serve:
         serve(a)
+        return
+    if a.cloud:
+        cloud_bench(a)
         return
     dev = torch.device("cuda:0")
     name = a.network
@@ -287,6 +297,133 @@ def serve(a):
                    "requests": len(R), "points": npts, "precision": a.precision, "slots": 4},
         "failed_requests": int(sum(1 for v in status.values() if v != 0)),
     }
+    print(json.dumps(res))
+
+
+def cloud_bench(a):
+    """Outer chemistry iteration over a synthetic cloud (the chemistry
+    iterations of 3DPDR.F90:474-520 with the ray-traced Av and columns of
+    eval_points.F90 / 3DPDR.F90:1636-1844): a side^3 grid of 1/16 pc cells,
+    lognormal n_H (mean 1e3 cm^-3, sigma_ln 1, correlation 4 cells), G0 = 100
+    Draine on the -x face (UNI field along +x), HEALPix level 0 (12 rays),
+    theta_crit 1.3, T from GUESS_TEMP, T_d = 20 K, full network by default.
+    Every iteration computes rates from the current columns, solves the
+    points that are not converged (warm-started from their abundances, t = 0
+    to 1e7 yr), writes back and recomputes the columns; state stays on the
+    device.  value = points solved / wall time of the iterations; the ray
+    tracer (set-up) is timed separately."""
+    from pdrchem import cloud as CL, lib, workloads
+    from pdrchem.netgen import load_network
+    import oracle
+    name = a.network if a.network != "reduced" or "--network" in sys.argv else "full"
+    net = load_network(name)
+    prec = lib.MIXED if a.precision == "mixed" else lib.FP64
+    side = a.cloud_side
+    lib.set_globals(rtol=1e-8, atol=1e-30)
+
+    def make(sd):
+        rho = workloads.cloud_density(sd)
+        t0 = time.perf_counter()
+        c = CL.Cloud(net, (sd, sd, sd), 0.0625, rho, gext=(100.0, 0.0, 0.0))
+        t_setup = time.perf_counter() - t0
+        x0 = workloads.initial_abundances(net, sd ** 3)
+        c.set_abundances(x0)
+        return c, rho, x0, t_setup
+
+    c, rho, x0, t_setup = make(side)
+    T = c.get(CL.TEMPERATURE)
+    G = c.get(CL.G)
+    Av = c.get(CL.AV)
+    cols0 = c.get(CL.COLUMNS)
+    n_ep = int(lib.load().pdrchem_cloud_num_evaluation_points(c._h))
+    iters, wall, solved = [], 0.0, 0
+    tot = dict(nst=0, nfe=0, nje=0, nsetups=0, nni=0)
+    ab1 = None
+    for it in range(a.cloud_iters):
+        t0 = time.perf_counter()
+        st = c.iterate(t_end=1e7, conv_tol=a.cloud_tol, precision=prec)
+        dt = time.perf_counter() - t0
+        wall += dt
+        solved += st["solved"]
+        for k in tot:
+            tot[k] += st[k]
+        iters.append({"iteration": it + 1, "solved": st["solved"], "converged": st["converged"],
+                      "failed": st["failed"], "wall_ms": dt * 1e3, "solve_ms": st["solve_ms"],
+                      "rates_ms": st["rates_ms"]})
+        print(f"cloud iteration {it + 1}: solved {st['solved']} converged {st['converged']} "
+              f"failed {st['failed']} in {dt:.2f} s", file=sys.stderr, flush=True)
+        if it == 0:
+            ab1 = c.get(CL.ABUNDANCE)
+    f_rhs, f_jac = network_flops(net)
+    stt = np.array([[tot["nst"], tot["nfe"], tot["nje"], tot["nsetups"], tot["nni"], 0, 0, 0]], dtype=np.float64)
+    w64, wlu = algorithmic_flops(stt, net.neq, f_rhs, f_jac)
+    ksec = sum(i["solve_ms"] for i in iters) / 1e3
+    t_roof = w64 / (FP64_PEAK_TFLOPS * 1e12) + wlu / ((FP32_PEAK_TFLOPS if prec == lib.MIXED else
+                                                       FP64_PEAK_TFLOPS) * 1e12)
+    achieved = (w64 + wlu) / ksec / 1e12
+    peak = (w64 + wlu) / t_roof / 1e12
+    kname = ("bdf_wave_kernel" if name == "reduced" else "bdf_wg_kernel") + f"<{name},{a.precision}>"
+    res = {
+        "metric": "grid-point chemistry solves/s at <=1e-4 rel err vs FP64 CVODE; p50/p99 batch ms",
+        "value": solved / wall, "unit": "solves/s", "n_gpus": 1, "steps": a.cloud_iters, "warmup": 0,
+        "ms_per_step": wall * 1e3 / max(len(iters), 1),
+        "p50_batch_ms": float(np.percentile([i["wall_ms"] for i in iters], 50)),
+        "p99_batch_ms": float(np.percentile([i["wall_ms"] for i in iters], 99)),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if prec == lib.FP64 else "f64+f32-matrix",
+        "data": "synthetic (seeded counter-based RNG)",
+        "config": {"workload": f"outer chemistry iteration over a {side}^3 lognormal cloud (n_H mean 1e3, "
+                               f"sigma_ln 1), 1/16 pc cells, G0=100 on the -x face, 12 HEALPix rays, "
+                               f"{name} network, warm starts, converged points skipped (tol {a.cloud_tol})",
+                   "points": side ** 3, "iterations": a.cloud_iters, "precision": a.precision,
+                   "rtol": 1e-8, "atol": 1e-30},
+        "iterations": iters,
+        "raytrace_setup_s": t_setup, "evaluation_points": n_ep,
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                     "frac": achieved / peak, "traffic": measured_traffic(kname, solved // max(len(iters), 1)),
+                     "kernel": kname,
+                     "note": "work W of CVODE's algorithm (dense LU) for the counters of every solve of the run"},
+    }
+    # iteration 1 checked point by point against the FP64 oracle fed with the
+    # same inputs (device T, G, Av, columns; oracle rates)
+    k = min(a.check, side ** 3) if a.check > 0 else 0
+    thr = min(16, os.cpu_count() or 1)
+    if k:
+        sel = np.linspace(0, side ** 3 - 1, k).astype(int)
+        o = oracle.Oracle(name)
+        rate = o.rates(T[sel], 20.0, G[sel], Av[sel], cols0[sel])
+        _, ref, _ = o.calculate_abundances(x0[sel].copy(), rate, rho[sel], T[sel], mode=oracle_mode(name),
+                                           nthreads=thr)
+        got = ab1[sel]
+        m = ref[:, :-1] > 1e-20
+        res["max_rel_err_vs_oracle"] = float(np.max(np.abs(got[:, :-1][m] - ref[:, :-1][m]) / ref[:, :-1][m]))
+        res["checked_points"] = k
+    if not a.no_cpu:
+        o = oracle.Oracle(name)
+        order = np.random.default_rng(7).permutation(side ** 3)
+        done, t0, chunk = 0, time.perf_counter(), thr * 2
+        while time.perf_counter() - t0 < a.cpu_seconds and done < side ** 3:
+            sel = order[done:done + chunk]
+            rate = o.rates(T[sel], 20.0, G[sel], Av[sel], cols0[sel])
+            o.calculate_abundances(x0[sel].copy(), rate, rho[sel], T[sel], mode=oracle_mode(name), nthreads=thr)
+            done += len(sel)
+        el = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": done / el, "unit": "solves/s", "cores": thr, "kind": "port",
+                               "sample": f"{done} random points of the cloud's first iteration (oracle rates and "
+                                         f"solve) in {el:.1f} s on {thr} threads"}
+    if a.cloud_fp64_side > 0 and prec == lib.MIXED:
+        # plain FP64 form against the mixed form on the same (smaller) cloud, first iteration
+        sd = a.cloud_fp64_side
+        out = {}
+        for pname, pv in (("mixed", lib.MIXED), ("fp64", lib.FP64)):
+            cc, _, _, _ = make(sd)
+            t0 = time.perf_counter()
+            st = cc.iterate(t_end=1e7, conv_tol=a.cloud_tol, precision=pv)
+            out[pname] = st["solved"] / (time.perf_counter() - t0)
+            cc.close()
+        res["fp64_comparison"] = {"points": sd ** 3, "iteration": 1, "mixed_solves_per_s": out["mixed"],
+                                  "fp64_solves_per_s": out["fp64"], "ratio": out["mixed"] / out["fp64"]}
+    c.close()
     print(json.dumps(res))
 
 

@@ -183,6 +183,7 @@ typedef struct {
   int failed;          /* of those, points whose solve failed (status != 0) */
   int negative_rates;  /* points with a negative rate coefficient */
   double rates_ms, solve_ms;  /* device time of the rate evaluation and of the solve */
+  long long nst, nfe, nje, nsetups, nni;  /* integrator counters summed over the solved points */
 } pdrchem_cloud_stats;
 pdrchem_cloud *pdrchem_cloud_create(int net, int nx, int ny, int nz, double spacing_pc, const double *density,
                                     int level, double theta_crit, int fieldchoice, const double *gext,
@@ -194,11 +195,10 @@ int pdrchem_cloud_set_abundances(pdrchem_cloud *cloud, const double *abundance);
 /* One chemistry iteration (3DPDR.F90:474-520): rates from the current
  * columns, a solve from t = 0 to t_end_years warm-started from the current
  * abundances (calculate_abundances_ semantics, tolerances from the globals
- * above), write-back, new columns.  Points whose last solve moved their
- * abundances (> 1e-20) by less than conv_tol relative, and whose columns have
- * not moved by more than conv_tol since, are not solved again (the
- * reference's converged-point skip, 3DPDR.F90:641-643, applied to the
- * chemistry). */
+ * above), write-back, new columns.  A point whose solve moved its abundances
+ * (those > 1e-20) by less than conv_tol relative is converged and is not
+ * solved again: it keeps its abundances, as the reference keeps those of its
+ * converged points (3DPDR.F90:628, columns :1822), which it re-solves and discards. */
 int pdrchem_cloud_iterate(pdrchem_cloud *cloud, double t_end_years, double conv_tol, int precision,
                           pdrchem_cloud_stats *stats);
 #define PDRCHEM_CLOUD_ABUNDANCE 0    /* [npts*nspec] */
