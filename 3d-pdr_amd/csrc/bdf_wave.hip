@@ -66,6 +66,23 @@ int launch_wave_solver(int net, const SolveRequest &r, hipStream_t s) {
   }
 }
 
+template <typename MT>
+static size_t full_workspace(void (*kern)(bdfw_reduced::PointArgs, bdfw_reduced::WaveWS<MT> *)) {
+  constexpr int WAVES = bdfw_reduced::WAVES;
+  int dev = 0, ncu = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64 * WAVES, 0) != hipSuccess) return 0;
+  if (per_cu < 1) per_cu = 1;
+  return (size_t)ncu * per_cu * WAVES * sizeof(bdfw_reduced::WaveWS<MT>);
+}
+
+size_t wave_solver_workspace_bytes(int net, int mixed) {
+  if (net != NET_REDUCED) return 0;
+  return mixed ? full_workspace<float>(bdfw_reduced::bdf_wave_kernel<float>)
+               : full_workspace<double>(bdfw_reduced::bdf_wave_kernel<double>);
+}
+
 template <typename Kern>
 static int kernel_info(Kern fn, KernelInfo *info) {
   hipFuncAttributes attr;

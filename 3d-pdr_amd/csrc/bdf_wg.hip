@@ -98,6 +98,16 @@ int launch(void (*kern)(Args, unsigned char *), const SolveRequest &r, size_t sl
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
+template <typename Args>
+size_t full_workspace(void (*kern)(Args, unsigned char *), size_t slot_bytes) {
+  int dev = 0, ncu = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 256, 0) != hipSuccess) return 0;
+  if (per_cu < 1) per_cu = 1;
+  return (size_t)ncu * per_cu * slot_bytes;
+}
+
 template <typename Kern>
 int info_of(Kern fn, KernelInfo *info) {
   hipFuncAttributes attr;
@@ -124,6 +134,19 @@ int launch_wg_solver(int net, const SolveRequest &r, hipStream_t s) {
                      : launch(bdfg_mynetwork::bdf_wg_kernel<double>, r, bdfg_mynetwork::Slot<double>::bytes, s);
     default:
       return -2;
+  }
+}
+
+size_t wg_solver_workspace_bytes(int net, int mixed) {
+  switch (net) {
+    case NET_FULL:
+      return mixed ? full_workspace(bdfg_full::bdf_wg_kernel<float>, bdfg_full::Slot<float>::bytes)
+                   : full_workspace(bdfg_full::bdf_wg_kernel<double>, bdfg_full::Slot<double>::bytes);
+    case NET_MYNETWORK:
+      return mixed ? full_workspace(bdfg_mynetwork::bdf_wg_kernel<float>, bdfg_mynetwork::Slot<float>::bytes)
+                   : full_workspace(bdfg_mynetwork::bdf_wg_kernel<double>, bdfg_mynetwork::Slot<double>::bytes);
+    default:
+      return 0;
   }
 }
 

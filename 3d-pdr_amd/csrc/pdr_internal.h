@@ -39,6 +39,13 @@ int wave_solver_info(int net, int mixed, KernelInfo *info);
 // full network and MYNETWORK: one system per workgroup (bdf_wg.hip)
 int launch_wg_solver(int net, const SolveRequest &r, hipStream_t s);
 int wg_solver_info(int net, int mixed, KernelInfo *info);
+// workspace a launch of the whole device needs (0 if the network has no
+// kernel of that kind); sizing the stream's workspace to it beforehand makes
+// every later launch on the stream allocation-free (capturable)
+size_t wave_solver_workspace_bytes(int net, int mixed);
+size_t wg_solver_workspace_bytes(int net, int mixed);
+// one unsigned int set to zero by a kernel (capturable, unlike a memset node)
+int launch_zero_u32(unsigned int *p, hipStream_t s);
 
 // rate evaluation (calc_reac_rates.F90)
 struct PhysParams {

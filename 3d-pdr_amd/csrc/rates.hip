@@ -24,6 +24,13 @@ namespace NET = pdrwg_mynetwork;
 
 namespace pdrchem {
 
+__global__ void zero_u32_kernel(unsigned int *p) { *p = 0u; }
+
+int launch_zero_u32(unsigned int *p, hipStream_t s) {
+  hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(1), 0, s, p);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 int launch_rates(int net, const RateRequest &r, hipStream_t s) {
   long long total = 0;
   void (*kern)(RateRequest) = nullptr;

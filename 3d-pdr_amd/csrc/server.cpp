@@ -13,9 +13,21 @@
 // on the device.  Requests that arrive while others run are admitted as
 // slots and arena space free up.  Admission is by device-memory footprint:
 // a batch starts only when its inputs and outputs fit the arena.
+//
+// A batch goes to the device as one HIP graph: the slot's stream captures
+// the input copy, the work-queue reset (a kernel, not a memset node), the
+// persistent solver launch and the result copies; the slot keeps one
+// executable graph and updates it in place for the next batch
+// (hipGraphExecUpdate; re-instantiated only when the update is refused), so
+// a batch costs one graph launch.  The solver's stream workspace is sized
+// for a whole-device launch when the server is created, so nothing inside
+// the capture allocates.  PDRCHEM_SERVER_GRAPHS=0 in the environment at
+// creation selects plain stream launches instead.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -110,6 +122,7 @@ struct Slot {
   std::vector<Req> reqs;
   char *pin = nullptr;  // pinned staging of the whole block
   size_t pin_cap = 0;
+  hipGraphExec_t exec = nullptr;  // executable graph of the last batch, updated per batch
 };
 
 }  // namespace
@@ -124,6 +137,8 @@ struct pdrchem_server {
   int precision = PDRCHEM_MIXED;
   size_t max_batch_pts = 65536;
   double rtol = 1e-8, atol = 1e-30;
+  bool graphs = true;
+  long long graph_launches = 0, graph_instantiations = 0;
   std::string err;
 };
 
@@ -168,8 +183,6 @@ bool launch_batch(pdrchem_server *S, Slot &sl) {
     k += r.npts;
   }
   char *dev = S->arena.base + sl.off;
-  if (hipMemcpyAsync(dev, sl.pin, L.status, hipMemcpyHostToDevice, sl.stream) != hipSuccess) return false;
-  if (hipMemsetAsync(dev + L.counter, 0, sizeof(unsigned int), sl.stream) != hipSuccess) return false;
   SolveRequest q{};
   q.abundance = (double *)(dev + L.ab);
   q.rate = (const double *)(dev + L.rate);
@@ -190,14 +203,52 @@ bool launch_batch(pdrchem_server *S, Slot &sl) {
   q.counter = (unsigned int *)(dev + L.counter);
   q.max_blocks = 0;
   q.mixed = S->precision == PDRCHEM_MIXED ? 1 : 0;
-  const int rc = net == NET_REDUCED ? launch_wave_solver(net, q, sl.stream) : launch_wg_solver(net, q, sl.stream);
-  if (rc != 0) return false;
-  // abundances and statuses back to the staging block
-  if (hipMemcpyAsync(sl.pin + L.ab, dev + L.ab, (size_t)n * s.nspec * 8, hipMemcpyDeviceToHost, sl.stream) !=
-      hipSuccess)
+  // the batch's work on the slot's stream
+  auto enqueue = [&]() -> bool {
+    if (hipMemcpyAsync(dev, sl.pin, L.status, hipMemcpyHostToDevice, sl.stream) != hipSuccess) return false;
+    if (launch_zero_u32(q.counter, sl.stream) != 0) return false;
+    const int rc = net == NET_REDUCED ? launch_wave_solver(net, q, sl.stream) : launch_wg_solver(net, q, sl.stream);
+    if (rc != 0) return false;
+    // abundances and statuses back to the staging block
+    if (hipMemcpyAsync(sl.pin + L.ab, dev + L.ab, (size_t)n * s.nspec * 8, hipMemcpyDeviceToHost, sl.stream) !=
+        hipSuccess)
+      return false;
+    return hipMemcpyAsync(sl.pin + L.status, dev + L.status, n * 4, hipMemcpyDeviceToHost, sl.stream) == hipSuccess;
+  };
+  if (!S->graphs) {
+    if (!enqueue()) return false;
+    return hipEventRecord(sl.done, sl.stream) == hipSuccess;
+  }
+  hipGraph_t g = nullptr;
+  if (hipStreamBeginCapture(sl.stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return false;
+  const bool queued = enqueue();
+  const hipError_t ec = hipStreamEndCapture(sl.stream, &g);
+  if (!queued || ec != hipSuccess || !g) {
+    if (g) (void)hipGraphDestroy(g);
     return false;
-  if (hipMemcpyAsync(sl.pin + L.status, dev + L.status, n * 4, hipMemcpyDeviceToHost, sl.stream) != hipSuccess)
-    return false;
+  }
+  bool ready = false;
+  if (sl.exec) {
+    hipGraphNode_t bad = nullptr;
+    hipGraphExecUpdateResult res;
+    ready = hipGraphExecUpdate(sl.exec, g, &bad, &res) == hipSuccess && res == hipGraphExecUpdateSuccess;
+    if (!ready) {
+      (void)hipGetLastError();
+      (void)hipGraphExecDestroy(sl.exec);
+      sl.exec = nullptr;
+    }
+  }
+  if (!ready) {
+    if (hipGraphInstantiate(&sl.exec, g, nullptr, nullptr, 0) != hipSuccess) {
+      sl.exec = nullptr;
+      (void)hipGraphDestroy(g);
+      return false;
+    }
+    S->graph_instantiations++;
+  }
+  (void)hipGraphDestroy(g);
+  if (hipGraphLaunch(sl.exec, sl.stream) != hipSuccess) return false;
+  S->graph_launches++;
   return hipEventRecord(sl.done, sl.stream) == hipSuccess;
 }
 
@@ -298,6 +349,20 @@ pdrchem_server *pdrchem_server_create(size_t device_bytes, int nslots, int preci
   S->precision = precision == PDRCHEM_FP64 ? PDRCHEM_FP64 : PDRCHEM_MIXED;
   S->rtol = chemistry_module_mp_relative_abundance_tolerance_;
   S->atol = chemistry_module_mp_absolute_abundance_tolerance_;
+  const char *e = std::getenv("PDRCHEM_SERVER_GRAPHS");
+  S->graphs = !(e && std::strcmp(e, "0") == 0);
+  // solver workspace of every slot sized for a whole-device launch of any
+  // network, so that no launch inside a capture allocates
+  const int mixed = S->precision == PDRCHEM_MIXED ? 1 : 0;
+  size_t ws = 0;
+  for (int net = 0; net < NET_COUNT; net++)
+    ws = std::max(ws, std::max(wave_solver_workspace_bytes(net, mixed), wg_solver_workspace_bytes(net, mixed)));
+  for (Slot &sl : S->slots) {
+    if (!stream_workspace(sl.stream, ws)) {
+      pdrchem_server_destroy(S);
+      return nullptr;
+    }
+  }
   return S;
 }
 
@@ -349,12 +414,21 @@ void pdrchem_server_destroy(pdrchem_server *S) {
   if (!S) return;
   for (Slot &sl : S->slots) {
     if (sl.stream) (void)hipStreamSynchronize(sl.stream);
+    if (sl.exec) (void)hipGraphExecDestroy(sl.exec);
     if (sl.done) (void)hipEventDestroy(sl.done);
     if (sl.stream) (void)hipStreamDestroy(sl.stream);
     if (sl.pin) (void)hipHostFree(sl.pin);
   }
   if (S->arena.base) (void)hipFree(S->arena.base);
   delete S;
+}
+
+int pdrchem_server_graph_stats(pdrchem_server *S, long long *launches, long long *instantiations) {
+  if (!S) return -1;
+  std::lock_guard<std::mutex> lock(S->mu);
+  if (launches) *launches = S->graph_launches;
+  if (instantiations) *instantiations = S->graph_instantiations;
+  return S->graphs ? 1 : 0;
 }
 
 const char *pdrchem_server_last_error(pdrchem_server *S) { return S ? S->err.c_str() : "null server"; }

@@ -25,7 +25,7 @@ EXPORTED_FUNCTIONS = [
     "calculate_abundances_", "pdrchem_network_id", "pdrchem_solve_device", "pdrchem_set_precision",
     "pdrchem_rates_device", "pdrchem_rates_device_zeta", "pdrchem_kernel_info", "pdrchem_set_max_blocks", "pdrchem_last_error",
     "pdrchem_server_create", "pdrchem_server_submit", "pdrchem_server_progress", "pdrchem_server_poll",
-    "pdrchem_server_last_error", "pdrchem_server_destroy",
+    "pdrchem_server_last_error", "pdrchem_server_destroy", "pdrchem_server_graph_stats",
     "pdrchem_cloud_create", "pdrchem_cloud_set_abundances", "pdrchem_cloud_iterate", "pdrchem_cloud_get",
     "pdrchem_cloud_num_evaluation_points", "pdrchem_cloud_evaluation_points", "pdrchem_cloud_last_error",
     "pdrchem_cloud_destroy",
@@ -101,6 +101,7 @@ def load(path: Optional[str] = None):
     L.pdrchem_server_last_error.restype = C.c_char_p
     L.pdrchem_server_destroy.argtypes = [vp]
     L.pdrchem_server_destroy.restype = None
+    L.pdrchem_server_graph_stats.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
     L.pdrchem_cloud_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, dp, C.c_int, C.c_double,
                                        C.c_int, dp, C.c_double, C.POINTER(Phys), ip, C.c_int]
     L.pdrchem_cloud_create.restype = vp
@@ -282,6 +283,12 @@ class Server:
         for t, _, _ in out:
             self.live.pop(t, None)
         return out
+
+    def graph_stats(self):
+        """(uses_graphs, graph launches, graph instantiations)."""
+        la, ins = C.c_longlong(), C.c_longlong()
+        g = self.L.pdrchem_server_graph_stats(self.h, C.byref(la), C.byref(ins))
+        return bool(g), la.value, ins.value
 
     def close(self):
         if self.h:

@@ -64,6 +64,18 @@ def network_flops(net):
     return f_rhs, f_jac
 
 
+def per_species_errors(net, got, ref, floor=1e-20):
+    """Max relative error per species over the checked points (abundances
+    above `floor` in the oracle), largest first; the electron slot is x_e of
+    the last RHS evaluation and is reported as well."""
+    out = {}
+    for i, sp in enumerate(net.species):
+        m = ref[:, i] > floor
+        if m.any():
+            out[sp.name] = float(np.max(np.abs(got[m, i] - ref[m, i]) / ref[m, i]))
+    return dict(sorted(out.items(), key=lambda kv: -kv[1]))
+
+
 def oracle_mode(name):
     import oracle
     return oracle.MODE_GPU if name == "reduced" else oracle.MODE_GPU_WG
@@ -229,6 +241,7 @@ def main():
         m = ref[:, :-1] > 1e-20
         res["max_rel_err_vs_oracle"] = float(np.max(np.abs(got[:, :-1][m] - ref[:, :-1][m]) / ref[:, :-1][m]))
         res["checked_points"] = k
+        res["per_species_max_rel_err"] = per_species_errors(net, got, ref)
     if not a.no_cpu:
         thr = min(16, os.cpu_count() or 1)
         k = min(a.npts, 200000)
@@ -398,6 +411,7 @@ def cloud_bench(a):
         m = ref[:, :-1] > 1e-20
         res["max_rel_err_vs_oracle"] = float(np.max(np.abs(got[:, :-1][m] - ref[:, :-1][m]) / ref[:, :-1][m]))
         res["checked_points"] = k
+        res["per_species_max_rel_err"] = per_species_errors(net, got, ref)
     if not a.no_cpu:
         o = oracle.Oracle(name)
         order = np.random.default_rng(7).permutation(side ** 3)

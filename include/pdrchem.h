@@ -155,6 +155,12 @@ int pdrchem_server_progress(pdrchem_server *server, int wait);
  * not be launched) and submit-to-completion latency in ms.  Returns the count. */
 int pdrchem_server_poll(pdrchem_server *server, int max, int *tickets, int *status, double *latency_ms);
 
+/* Batches go to the device as HIP graphs (capture of copy-in, work-queue
+ * reset, solver launch and copy-out on the slot's stream; one executable
+ * graph per slot, updated in place per batch).  Returns 1 if the server
+ * launches graphs (0: plain stream launches, PDRCHEM_SERVER_GRAPHS=0 in the
+ * environment at creation) and the graph launches and instantiations so far. */
+int pdrchem_server_graph_stats(pdrchem_server *server, long long *launches, long long *instantiations);
 const char *pdrchem_server_last_error(pdrchem_server *server);
 void pdrchem_server_destroy(pdrchem_server *server);
 

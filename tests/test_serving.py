@@ -67,7 +67,9 @@ def test_server_mix_equals_alone():
         srv.progress()
     srv.progress(wait=True)
     done = srv.poll()
+    graphs, launches, _ = srv.graph_stats()
     srv.close()
+    assert graphs and launches >= 1
     assert sorted(t for t, _, _ in done) == sorted(tickets)
     assert all(st == 0 for _, st, _ in done)
     assert all(lat > 0 for _, _, lat in done)
@@ -81,6 +83,39 @@ def test_server_mix_equals_alone():
         assert int(status.abs().sum().item()) == 0
         alone = ab.cpu().numpy()
         assert np.array_equal(alone, ab_mix), (name, te, float(np.max(np.abs(alone - ab_mix))))
+
+
+@pytest.mark.gpu
+def test_server_graph_replays_equal_stream_launches(monkeypatch):
+    """One slot, so its executable graph is updated and replayed batch after
+    batch (different sizes, networks and end times, with host work and
+    synchronisation between replays); every replay's results equal those of
+    plain stream launches bitwise."""
+    import oracle
+    from pdrchem import lib
+    orc = {nm: oracle.Oracle(nm) for nm in ("reduced", "full")}
+    sizes = [("reduced", 50), ("reduced", 50), ("reduced", 13), ("full", 4), ("full", 4), ("reduced", 50)]
+    reqs = _requests(orc, sizes, seed=7)
+    lib.set_globals(rtol=1e-8, atol=1e-30)
+    out = {}
+    for mode in ("0", "1"):
+        monkeypatch.setenv("PDRCHEM_SERVER_GRAPHS", mode)
+        srv = lib.Server(device_bytes=256 << 20, nslots=1, precision=lib.MIXED)
+        res = []
+        for name, x0, r, nH, te in reqs:
+            ab = x0.copy()
+            srv.submit(lib.network_id(name), ab, r, nH, te)
+            srv.progress(wait=True)          # one batch per request: a replay each
+            assert all(st == 0 for _, st, _ in srv.poll())
+            res.append(ab)
+        g, launches, inst = srv.graph_stats()
+        srv.close()
+        assert g == (mode == "1")
+        if g:
+            assert launches == len(reqs) and 1 <= inst <= len(reqs)
+        out[mode] = res
+    for a, b in zip(out["0"], out["1"]):
+        assert np.array_equal(a, b)
 
 
 @pytest.mark.gpu
