@@ -2,6 +2,8 @@
 // launchers.  See bdf_wave_impl.h for the algorithm and layout.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "wave_ops.h"
 #include "gen/net_reduced.h"
 #include "pdr_internal.h"
@@ -34,53 +36,73 @@ static void fill_args(Args &a, const SolveRequest &r) {
   a.counter = r.counter;
 }
 
-template <typename MT>
-static int launch_persistent(void (*kern)(bdfw_reduced::PointArgs, bdfw_reduced::WaveWS<MT> *),
-                             const bdfw_reduced::PointArgs &a, int npts, hipStream_t s, int max_blocks) {
-  constexpr int WAVES = bdfw_reduced::WAVES;
+template <typename MT, int NW>
+static int launch_persistent(const bdfw_reduced::PointArgs &a, int npts, hipStream_t s, int max_blocks,
+                             int cap_per_cu) {
+  auto kern = bdfw_reduced::bdf_wave_kernel<MT, NW>;
   int dev = 0, ncu = 0, per_cu = 0;
   if (hipGetDevice(&dev) != hipSuccess) return -1;
   if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return -1;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64 * WAVES, 0) != hipSuccess) return -1;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64 * NW, 0) != hipSuccess) return -1;
   if (per_cu < 1) per_cu = 1;
-  long long want = ((long long)npts + WAVES - 1) / WAVES;
+  if (cap_per_cu > 0 && per_cu > cap_per_cu) per_cu = cap_per_cu;
+  long long want = ((long long)npts + NW - 1) / NW;
   long long full = (long long)ncu * per_cu;
   long long nb = want < full ? want : full;
   if (max_blocks > 0 && nb > max_blocks) nb = max_blocks;
   if (nb < 1) nb = 1;
-  auto *ws = (bdfw_reduced::WaveWS<MT> *)stream_workspace(s, (size_t)nb * WAVES * sizeof(bdfw_reduced::WaveWS<MT>));
+  auto *ws = (bdfw_reduced::WaveWS<MT> *)stream_workspace(s, (size_t)nb * NW * sizeof(bdfw_reduced::WaveWS<MT>));
   if (!ws) return -1;
-  hipLaunchKernelGGL(kern, dim3((unsigned)nb), dim3(64 * WAVES), 0, s, a, ws);
+  hipLaunchKernelGGL(kern, dim3((unsigned)nb), dim3(64 * NW), 0, s, a, ws);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
+
+template <typename MT>
+static int launch_shape(const bdfw_reduced::PointArgs &a, int npts, hipStream_t s, int max_blocks,
+                        LaunchConfig cfg) {
+  switch (cfg.systems_per_block) {
+    case 1: return launch_persistent<MT, 1>(a, npts, s, max_blocks, cfg.blocks_per_cu);
+    case 2: return launch_persistent<MT, 2>(a, npts, s, max_blocks, cfg.blocks_per_cu);
+    default: return launch_persistent<MT, 4>(a, npts, s, max_blocks, cfg.blocks_per_cu);
+  }
+}
+
 int launch_wave_solver(int net, const SolveRequest &r, hipStream_t s) {
   switch (net) {
     case NET_REDUCED: {
       bdfw_reduced::PointArgs a;
       fill_args(a, r);
-      if (r.mixed) return launch_persistent<float>(bdfw_reduced::bdf_wave_kernel<float>, a, r.npts, s, r.max_blocks);
-      return launch_persistent<double>(bdfw_reduced::bdf_wave_kernel<double>, a, r.npts, s, r.max_blocks);
+      LaunchConfig cfg = r.cfg;
+      if (cfg.systems_per_block == 0 && cfg.blocks_per_cu == 0) cfg = tuned_config(net, r.mixed, r.npts);
+      if (r.mixed) return launch_shape<float>(a, r.npts, s, r.max_blocks, cfg);
+      return launch_shape<double>(a, r.npts, s, r.max_blocks, cfg);
     }
     default:
       return -2;
   }
 }
 
+// the largest whole-device workspace over the block shapes
+template <typename MT, int NW>
+static size_t shape_workspace(int ncu) {
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bdfw_reduced::bdf_wave_kernel<MT, NW>, 64 * NW, 0) !=
+      hipSuccess)
+    return 0;
+  return (size_t)ncu * std::max(per_cu, 1) * NW * sizeof(bdfw_reduced::WaveWS<MT>);
+}
+
 template <typename MT>
-static size_t full_workspace(void (*kern)(bdfw_reduced::PointArgs, bdfw_reduced::WaveWS<MT> *)) {
-  constexpr int WAVES = bdfw_reduced::WAVES;
-  int dev = 0, ncu = 0, per_cu = 0;
+static size_t full_workspace() {
+  int dev = 0, ncu = 0;
   if (hipGetDevice(&dev) != hipSuccess) return 0;
   if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64 * WAVES, 0) != hipSuccess) return 0;
-  if (per_cu < 1) per_cu = 1;
-  return (size_t)ncu * per_cu * WAVES * sizeof(bdfw_reduced::WaveWS<MT>);
+  return std::max(shape_workspace<MT, 1>(ncu), std::max(shape_workspace<MT, 2>(ncu), shape_workspace<MT, 4>(ncu)));
 }
 
 size_t wave_solver_workspace_bytes(int net, int mixed) {
   if (net != NET_REDUCED) return 0;
-  return mixed ? full_workspace<float>(bdfw_reduced::bdf_wave_kernel<float>)
-               : full_workspace<double>(bdfw_reduced::bdf_wave_kernel<double>);
+  return mixed ? full_workspace<float>() : full_workspace<double>();
 }
 
 template <typename Kern>
@@ -100,8 +122,8 @@ static int kernel_info(Kern fn, KernelInfo *info) {
 int wave_solver_info(int net, int mixed, KernelInfo *info) {
   switch (net) {
     case NET_REDUCED:
-      return mixed ? kernel_info(bdfw_reduced::bdf_wave_kernel<float>, info)
-                   : kernel_info(bdfw_reduced::bdf_wave_kernel<double>, info);
+      return mixed ? kernel_info(bdfw_reduced::bdf_wave_kernel<float, 4>, info)
+                   : kernel_info(bdfw_reduced::bdf_wave_kernel<double, 4>, info);
     default:
       return -2;
   }

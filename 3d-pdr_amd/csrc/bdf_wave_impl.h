@@ -29,7 +29,9 @@ using namespace pdrw;
 
 #include "bdf_core.h"
 
-constexpr int WAVES = 4;  // wavefronts per block
+// Systems (wavefronts) per block, NW, is a template parameter chosen per
+// launch by the autotuner (1, 2 or 4; the per-block RHS tables are shared by
+// the block's wavefronts).
 
 // Occupancy: the kernel is latency-bound (every step is a chain of
 // dependent cross-lane reductions and LDS round trips), so throughput grows
@@ -92,11 +94,11 @@ struct WaveLDS {
 
 // per-block shared memory: the RHS term table and the slot factors are the
 // same for every wave
-template <typename MT>
+template <typename MT, int NW>
 struct BlockLDS {
   uint32_t rterm[NET::TM * 64];
   uint32_t fab[NET::ROUNDS * 64];
-  WaveLDS<MT> w[WAVES];
+  WaveLDS<MT> w[NW];
 };
 
 struct Lane {
@@ -481,10 +483,10 @@ __device__ __forceinline__ void solve_point(Ctx<MT> &c, const PointArgs &a, int 
 }
 
 // persistent kernel: every wavefront pulls grid points from a queue
-template <typename MT>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWavesPerEU<MT>)))
+template <typename MT, int NW>
+__global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(kWavesPerEU<MT>)))
 bdf_wave_kernel(PointArgs a, WaveWS<MT> *ws) {
-  __shared__ BlockLDS<MT> blk;
+  __shared__ BlockLDS<MT, NW> blk;
   for (int k = threadIdx.x; k < NET::TM * 64; k += blockDim.x) blk.rterm[k] = NET::rhs_term[k];
   for (int k = threadIdx.x; k < NET::ROUNDS * 64; k += blockDim.x) blk.fab[k] = NET::slot_fab[k];
   __syncthreads();
@@ -495,7 +497,7 @@ bdf_wave_kernel(PointArgs a, WaveWS<MT> *ws) {
   c.ln.row_ok = c.ln.i < NET::NEQ;
   c.ln.owner = c.ln.row_ok && c.ln.hh == 0;
   c.S = (WaveLDS<MT> LDSA *)&blk.w[threadIdx.x >> 6];
-  c.W = ws + (size_t)blockIdx.x * WAVES + (threadIdx.x >> 6);
+  c.W = ws + (size_t)blockIdx.x * NW + (threadIdx.x >> 6);
   c.rtol = a.rtol;
   c.atol = a.atol;
   c.tb.rbase = NET::row_base[c.ln.L];

@@ -80,7 +80,7 @@ void fill_args(Args &a, const SolveRequest &r) {
 }
 
 template <typename Args>
-int launch(void (*kern)(Args, unsigned char *), const SolveRequest &r, size_t slot_bytes, hipStream_t s) {
+int launch(int net, void (*kern)(Args, unsigned char *), const SolveRequest &r, size_t slot_bytes, hipStream_t s) {
   Args a;
   fill_args(a, r);
   int dev = 0, ncu = 0, per_cu = 0;
@@ -88,6 +88,9 @@ int launch(void (*kern)(Args, unsigned char *), const SolveRequest &r, size_t sl
   if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return -1;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 256, 0) != hipSuccess) return -1;
   if (per_cu < 1) per_cu = 1;
+  LaunchConfig cfg = r.cfg;
+  if (cfg.systems_per_block == 0 && cfg.blocks_per_cu == 0) cfg = tuned_config(net, r.mixed, r.npts);
+  if (cfg.blocks_per_cu > 0 && per_cu > cfg.blocks_per_cu) per_cu = cfg.blocks_per_cu;
   long long full = (long long)ncu * per_cu;
   long long nb = r.npts < full ? r.npts : full;
   if (r.max_blocks > 0 && nb > r.max_blocks) nb = r.max_blocks;
@@ -127,11 +130,11 @@ int info_of(Kern fn, KernelInfo *info) {
 int launch_wg_solver(int net, const SolveRequest &r, hipStream_t s) {
   switch (net) {
     case NET_FULL:
-      return r.mixed ? launch(bdfg_full::bdf_wg_kernel<float>, r, bdfg_full::Slot<float>::bytes, s)
-                     : launch(bdfg_full::bdf_wg_kernel<double>, r, bdfg_full::Slot<double>::bytes, s);
+      return r.mixed ? launch(net, bdfg_full::bdf_wg_kernel<float>, r, bdfg_full::Slot<float>::bytes, s)
+                     : launch(net, bdfg_full::bdf_wg_kernel<double>, r, bdfg_full::Slot<double>::bytes, s);
     case NET_MYNETWORK:
-      return r.mixed ? launch(bdfg_mynetwork::bdf_wg_kernel<float>, r, bdfg_mynetwork::Slot<float>::bytes, s)
-                     : launch(bdfg_mynetwork::bdf_wg_kernel<double>, r, bdfg_mynetwork::Slot<double>::bytes, s);
+      return r.mixed ? launch(net, bdfg_mynetwork::bdf_wg_kernel<float>, r, bdfg_mynetwork::Slot<float>::bytes, s)
+                     : launch(net, bdfg_mynetwork::bdf_wg_kernel<double>, r, bdfg_mynetwork::Slot<double>::bytes, s);
     default:
       return -2;
   }

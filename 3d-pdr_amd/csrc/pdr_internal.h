@@ -7,6 +7,18 @@ namespace pdrchem {
 
 enum NetId { NET_REDUCED = 0, NET_FULL = 1, NET_MYNETWORK = 2, NET_COUNT = 3 };
 
+// launch shape of the persistent solver (0 = the default / the device's limit)
+struct LaunchConfig {
+  int systems_per_block;  // wave kernel: wavefronts (systems) per block, 1, 2 or 4
+  int blocks_per_cu;      // cap on resident blocks per CU
+};
+
+// tuned launch shapes per (network, precision, batch-size bucket), written
+// by the autotuner (autotune.cpp); a launch with cfg = {0, 0} uses the
+// tuned shape of its bucket, else the default
+LaunchConfig tuned_config(int net, int mixed, int npts);
+int size_bucket(int npts);
+
 // one batched solve: every pointer is device memory
 struct SolveRequest {
   double *abundance;      // [npts][nspec], point-major (calculate_abundances.c:155)
@@ -23,6 +35,7 @@ struct SolveRequest {
   unsigned int *counter;  // work-queue counter, zero on entry
   int max_blocks;         // 0 = fill the device
   int mixed;              // 0: FP64 iteration matrix; 1: FP32 matrix, FP64 Newton
+  LaunchConfig cfg;       // {0, 0}: tuned or default launch shape
 };
 
 struct KernelInfo {

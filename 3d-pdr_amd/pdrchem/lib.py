@@ -28,7 +28,8 @@ EXPORTED_FUNCTIONS = [
     "pdrchem_server_last_error", "pdrchem_server_destroy", "pdrchem_server_graph_stats",
     "pdrchem_cloud_create", "pdrchem_cloud_set_abundances", "pdrchem_cloud_iterate", "pdrchem_cloud_get",
     "pdrchem_cloud_num_evaluation_points", "pdrchem_cloud_evaluation_points", "pdrchem_cloud_last_error",
-    "pdrchem_cloud_destroy",
+    "pdrchem_cloud_destroy", "pdrchem_autotune", "pdrchem_tuned_config",
+    "pdrchem_set_tuned_config", "pdrchem_clear_tuning",
 ]
 EXPORTED_GLOBALS = [
     ("chemistry_module_mp_relative_abundance_tolerance_", C.c_double),
@@ -113,6 +114,11 @@ def load(path: Optional[str] = None):
     L.pdrchem_cloud_evaluation_points.argtypes = [vp, ip, C.POINTER(C.c_longlong), ip, dp]
     L.pdrchem_cloud_last_error.argtypes = [vp]
     L.pdrchem_cloud_last_error.restype = C.c_char_p
+    L.pdrchem_autotune.argtypes = [C.c_int, C.c_int, vp, vp, vp, C.c_int, C.c_double, C.c_int, vp, ip, ip, dp]
+    L.pdrchem_tuned_config.argtypes = [C.c_int, C.c_int, C.c_int, ip, ip]
+    L.pdrchem_set_tuned_config.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
+    L.pdrchem_clear_tuning.argtypes = []
+    L.pdrchem_clear_tuning.restype = None
     L.pdrchem_cloud_destroy.argtypes = [vp]
     L.pdrchem_cloud_destroy.restype = None
     _lib = L
@@ -214,6 +220,39 @@ def solve_device(net: int, abundance, rate, density, rtol=1e-8, atol=1e-30, t_st
     if rc != 0:
         raise PdrChemError(last_error())
     return status, stats
+
+
+def autotune(net: int, abundance, rate, density, t_end=1e7, nelect=None, precision=MIXED, stream=None) -> dict:
+    """Time every candidate launch shape on this batch (torch device tensors,
+    as solve_device; abundance is left unchanged) and record the fastest for
+    the batch's size bucket.  Returns the chosen shape and its time."""
+    import torch
+    L = load()
+    n = abundance.shape[0]
+    if nelect is None:
+        nelect = abundance.shape[1]
+    s = stream if stream is not None else torch.cuda.current_stream(abundance.device)
+    spb, bpc, ms = C.c_int(), C.c_int(), C.c_double()
+    rc = L.pdrchem_autotune(net, precision, _ptr(abundance), _ptr(rate), _ptr(density), n, t_end, nelect,
+                            C.c_void_p(s.cuda_stream), C.byref(spb), C.byref(bpc), C.byref(ms))
+    if rc != 0:
+        raise PdrChemError("pdrchem_autotune failed")
+    return {"systems_per_block": spb.value, "blocks_per_cu": bpc.value, "ms": ms.value}
+
+
+def tuned_config(net: int, npts: int, precision=MIXED):
+    spb, bpc = C.c_int(), C.c_int()
+    r = load().pdrchem_tuned_config(net, precision, npts, C.byref(spb), C.byref(bpc))
+    return (spb.value, bpc.value) if r == 1 else None
+
+
+def clear_tuning():
+    load().pdrchem_clear_tuning()
+
+
+def set_tuned_config(net: int, npts: int, systems_per_block: int, blocks_per_cu: int, precision=MIXED):
+    if load().pdrchem_set_tuned_config(net, precision, npts, systems_per_block, blocks_per_cu) != 0:
+        raise PdrChemError("bad launch shape")
 
 
 def rates_device(net: int, T, Td, G, Av, cols=None, phys: Optional[Phys] = None, out=None, neg=None,

@@ -119,6 +119,8 @@ def main():
     ap.add_argument("--serve-duration", type=float, default=30.0, help="seconds of arrivals")
     ap.add_argument("--serve-networks", default="reduced,full",
                     help="networks of the requests, chosen with equal odds")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="launch with the default shape instead of the autotuner's choice")
     ap.add_argument("--cloud", action="store_true",
                     help="outer chemistry iteration over a lognormal grid cloud (ray-traced columns)")
     ap.add_argument("--cloud-side", type=int, default=64, help="grid side (side^3 points)")
@@ -178,6 +180,17 @@ def main():
         if ev:
             ev[1].record(stream)
 
+    shape = None
+    if not a.no_autotune:
+        # launch shape chosen by the autotuner on a sample of the batch (larger
+        # batches use the shape of the largest tuned size bucket)
+        k = min(a.npts, 40000 if name == "reduced" else 4000)
+        lib.rates_device(nid, T[:k], Td[:k], G[:k], Av[:k], cols[:k], out=rate[:k], neg=neg, stream=stream)
+        t0 = time.perf_counter()
+        r = lib.autotune(nid, x0[:k].clone(), rate[:k], nH[:k], precision=prec)
+        shape = {"systems_per_block": r["systems_per_block"], "blocks_per_cu": r["blocks_per_cu"],
+                 "chosen_by": f"autotuner on the first {k} points ({time.perf_counter() - t0:.1f} s, untimed)",
+                 "applies": lib.tuned_config(nid, a.npts, precision=prec)}
     for _ in range(a.warmup):
         step()
     torch.cuda.synchronize()
@@ -223,6 +236,7 @@ def main():
                                "log10 G0~U[0,5], to 1e7 yr",
                    "points": a.npts, "precision": a.precision, "rtol": 1e-8, "atol": 1e-30},
         "failed_points": nfail,
+        "launch_shape": shape,
         "mean_steps_per_point": float(st_np[:, 0].mean()),
         "solver_kernel_ms": kt * 1e3,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
@@ -275,6 +289,16 @@ def serve(a):
         data[name] = dict(nid=nid, rate=rate.cpu().numpy(), nH=P.n_H.copy(),
                           ab=workloads.initial_abundances(net, max(tot, 1)))
     lib.set_globals(rtol=1e-8, atol=1e-30)
+    tuned = {}
+    if not a.no_autotune:
+        # launch shapes for the request sizes of the stream, tuned before it starts
+        for name in nets:
+            d = data[name]
+            for n in (64, 1024, 4096):
+                n = min(n, d["ab"].shape[0])
+                r = lib.autotune(d["nid"], torch.tensor(d["ab"][:n], device=dev), torch.tensor(d["rate"][:n], device=dev),
+                                 torch.tensor(d["nH"][:n], device=dev), precision=prec)
+                tuned[f"{name}:{n}"] = (r["systems_per_block"], r["blocks_per_cu"])
     srv = lib.Server(device_bytes=4 << 30, nslots=4, precision=prec)
     lat, status = {}, {}
     t0 = time.perf_counter()
@@ -309,6 +333,7 @@ def serve(a):
                                f"t_end logU[1e3,1e7] yr",
                    "requests": len(R), "points": npts, "precision": a.precision, "slots": 4},
         "failed_requests": int(sum(1 for v in status.values() if v != 0)),
+        "launch_shapes": tuned or None,
     }
     print(json.dumps(res))
 

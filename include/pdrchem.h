@@ -116,6 +116,32 @@ int pdrchem_rates_device_zeta(int net, const pdrchem_phys *phys, const double *d
 int pdrchem_kernel_info(int net, int precision, int *num_regs, int *lds_bytes, int *scratch_bytes,
                         int *blocks_per_cu);
 
+/* Autotuner of the solver's launch shape (no counterpart in the reference).
+ * Runs every candidate shape on the given batch (device memory, as
+ * pdrchem_solve_device; d_abundance is restored before each run and on
+ * return, so the call leaves it unchanged) and records the fastest for this
+ * network, precision and batch-size bucket (point counts in (4^(b-1), 4^b]).
+ * Candidates: wavefront-per-system kernel (reduced network): 1, 2 or 4
+ * systems per block, and 4 per block with at most 2 blocks per CU;
+ * workgroup-per-system kernel: 1 or 2 resident workgroups per CU.  Every
+ * later launch of the same network and precision uses the shape recorded for
+ * its bucket, or else the one of the largest tuned bucket below it (a device
+ * saturated by a few thousand points ranks the shapes the same for any larger
+ * batch), or else the default.  Synchronous on `stream`.  Returns 0 and the chosen shape
+ * (blocks_per_cu 0 = as many as fit) and its time. */
+int pdrchem_autotune(int net, int precision, double *d_abundance, const double *d_rate, const double *d_density,
+                     int npts, double t_end_years, int nelect, void *stream, int *systems_per_block,
+                     int *blocks_per_cu, double *best_ms);
+/* Shape a launch of npts points of this network and precision uses: returns
+ * 1 and the shape if a tuned one applies, 0 (and zeros) if the default (4
+ * systems per block, full occupancy). */
+int pdrchem_tuned_config(int net, int precision, int npts, int *systems_per_block, int *blocks_per_cu);
+/* Record a shape for the bucket of npts by hand (a saved tuning restored;
+ * systems_per_block 1, 2 or 4, ignored by the workgroup kernel; 0, 0 clears
+ * the bucket back to the default). */
+int pdrchem_set_tuned_config(int net, int precision, int npts, int systems_per_block, int blocks_per_cu);
+/* Forget every tuned shape. */
+void pdrchem_clear_tuning(void);
 /* Upper bound on resident blocks for the persistent solver (0 = fill the
  * device).  For experiments and tests of the work queue. */
 void pdrchem_set_max_blocks(int max_blocks);
