@@ -31,6 +31,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -380,6 +381,7 @@ struct pdrchem_cloud {
   int sel[5] = {-1, -1, -1, -1, -1};
   double av_fac = 0.0, uv_fac = 0.0;
   bool have_ab = false;
+  pdrchem_server *server = nullptr;  // when set, solves go through this request loop
   std::string err;
   std::vector<void *> allocs;
 
@@ -696,8 +698,32 @@ int pdrchem_cloud_iterate(pdrchem_cloud *c, double t_end_years, double conv_tol,
     }
     const double rtol = chemistry_module_mp_relative_abundance_tolerance_;
     const double atol = chemistry_module_mp_absolute_abundance_tolerance_;
-    if (pdrchem_solve_device(c->net, c->c_ab, c->c_rate, c->c_rho, nact, rtol, atol, 0.0, t_end_years, c->nelect,
-                             c->c_status, c->c_stats, c->d_counter, precision, s) != 0) {
+    double server_ms = -1.0;
+    if (c->server) {
+      // through the request loop: the compact batch as device requests of at
+      // most one batch each, solved in place on the loop's streams
+      const int chunk = pdrchem_server_max_batch(c->server);
+      bool good = chunk > 0 && ok(hipStreamSynchronize(s), "sync");
+      const auto t0 = std::chrono::steady_clock::now();
+      for (int b = 0; good && b < nact; b += chunk) {
+        const int m = std::min(chunk, nact - b);
+        good = pdrchem_server_submit_device(c->server, c->net, m, c->c_ab + (size_t)b * c->nspec,
+                                            c->c_rate + (size_t)b * c->nreac, c->c_rho + b, t_end_years,
+                                            c->c_status + b, c->c_stats + (size_t)b * 8) >= 0;
+      }
+      good = good && pdrchem_server_progress(c->server, 1) >= 0;
+      std::vector<int> tk(64), st(64);
+      while (good && pdrchem_server_poll(c->server, 64, tk.data(), st.data(), nullptr) > 0) {
+      }
+      server_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+      if (!good) {
+        c->err = std::string("solve through the request loop: ") + pdrchem_server_last_error(c->server);
+        hipEventDestroy(e2);
+        hipEventDestroy(e3);
+        return -1;
+      }
+    } else if (pdrchem_solve_device(c->net, c->c_ab, c->c_rate, c->c_rho, nact, rtol, atol, 0.0, t_end_years,
+                                    c->nelect, c->c_status, c->c_stats, c->d_counter, precision, s) != 0) {
       c->err = std::string("solve: ") + pdrchem_last_error();
       hipEventDestroy(e2);
       hipEventDestroy(e3);
@@ -719,6 +745,7 @@ int pdrchem_cloud_iterate(pdrchem_cloud *c, double t_end_years, double conv_tol,
     if (good) {
       hipEventElapsedTime(&ms_rates, c->ev0, c->ev1);
       hipEventElapsedTime(&ms_solve, e2, e3);
+      if (server_ms >= 0) ms_solve = (float)server_ms;
     }
     hipEventDestroy(e2);
     hipEventDestroy(e3);
@@ -763,6 +790,12 @@ int pdrchem_cloud_get(pdrchem_cloud *c, int field, double *out) {
     c->fail("get");
     return -1;
   }
+  return 0;
+}
+
+int pdrchem_cloud_set_server(pdrchem_cloud *c, pdrchem_server *server) {
+  if (!c) return -1;
+  c->server = server;
   return 0;
 }
 

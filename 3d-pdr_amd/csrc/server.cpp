@@ -23,6 +23,12 @@
 // for a whole-device launch when the server is created, so nothing inside
 // the capture allocates.  PDRCHEM_SERVER_GRAPHS=0 in the environment at
 // creation selects plain stream launches instead.
+//
+// Requests may also hold device arrays (pdrchem_server_submit_device): their
+// batch gathers inputs into the arena block by device-to-device copies and
+// scatters abundances (and, if asked, per-point statuses and integrator
+// counters) back the same way, so state that lives in HBM never crosses PCIe;
+// only the per-point end times and the statuses go through pinned staging.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -106,6 +112,9 @@ struct Req {
   double *ab;  // caller's arrays (ab in/out), valid until the request completes
   const double *rate, *dens;
   double tend_s, t_submit;
+  bool device = false;        // arrays are device memory (no host staging)
+  int *d_status = nullptr;    // device requests: per-point status out (optional)
+  int *d_stats = nullptr;     // device requests: per-point [8] counters out (optional)
 };
 
 struct Done {
@@ -117,11 +126,14 @@ struct Slot {
   hipStream_t stream = nullptr;
   hipEvent_t done = nullptr;
   bool busy = false;
+  bool device = false;  // the batch's requests hold device arrays
   int net = 0;
   size_t npts = 0, off = 0, bytes = 0;
   std::vector<Req> reqs;
-  char *pin = nullptr;  // pinned staging of the whole block
+  char *pin = nullptr;  // pinned staging of the whole block (host requests)
   size_t pin_cap = 0;
+  char *pin2 = nullptr;  // pinned end times and statuses (device requests)
+  size_t pin2_cap = 0;
   hipGraphExec_t exec = nullptr;  // executable graph of the last batch, updated per batch
 };
 
@@ -146,7 +158,7 @@ namespace {
 
 // device (and pinned) block layout of a batch
 struct Layout {
-  size_t ab, rate, dens, tend, status, counter, total;
+  size_t ab, rate, dens, tend, status, stats, counter, total;
 };
 Layout layout(int net, size_t n) {
   const Shape &s = kShape[net];
@@ -156,7 +168,8 @@ Layout layout(int net, size_t n) {
   L.dens = L.rate + align256(n * s.nreac * 8);
   L.tend = L.dens + align256(n * 8);
   L.status = L.tend + align256(n * 8);
-  L.counter = L.status + align256(n * 4);
+  L.stats = L.status + align256(n * 4);
+  L.counter = L.stats + align256(n * 8 * 4);
   L.total = L.counter + 256;
   return L;
 }
@@ -166,20 +179,28 @@ bool launch_batch(pdrchem_server *S, Slot &sl) {
   const Shape &s = kShape[net];
   const size_t n = sl.npts;
   const Layout L = layout(net, n);
-  if (sl.pin_cap < L.total) {
-    if (sl.pin) (void)hipHostFree(sl.pin);
-    sl.pin = nullptr;
-    sl.pin_cap = 0;
-    if (hipHostMalloc((void **)&sl.pin, L.total, hipHostMallocDefault) != hipSuccess) return false;
-    sl.pin_cap = L.total;
+  // pinned staging: the whole block for host requests; end times and
+  // statuses only for device requests
+  char **pin = sl.device ? &sl.pin2 : &sl.pin;
+  size_t *cap = sl.device ? &sl.pin2_cap : &sl.pin_cap;
+  const size_t need = sl.device ? align256(n * 8) + n * 4 : L.total;
+  if (*cap < need) {
+    if (*pin) (void)hipHostFree(*pin);
+    *pin = nullptr;
+    *cap = 0;
+    if (hipHostMalloc((void **)pin, need, hipHostMallocDefault) != hipSuccess) return false;
+    *cap = need;
   }
+  double *tend_pin = (double *)(sl.device ? sl.pin2 : sl.pin + L.tend);
+  char *status_pin = sl.device ? sl.pin2 + align256(n * 8) : sl.pin + L.status;
   size_t k = 0;
   for (const Req &r : sl.reqs) {
-    std::memcpy(sl.pin + L.ab + k * s.nspec * 8, r.ab, (size_t)r.npts * s.nspec * 8);
-    std::memcpy(sl.pin + L.rate + k * s.nreac * 8, r.rate, (size_t)r.npts * s.nreac * 8);
-    std::memcpy(sl.pin + L.dens + k * 8, r.dens, (size_t)r.npts * 8);
-    double *te = (double *)(sl.pin + L.tend) + k;
-    for (int i = 0; i < r.npts; i++) te[i] = r.tend_s;
+    if (!sl.device) {
+      std::memcpy(sl.pin + L.ab + k * s.nspec * 8, r.ab, (size_t)r.npts * s.nspec * 8);
+      std::memcpy(sl.pin + L.rate + k * s.nreac * 8, r.rate, (size_t)r.npts * s.nreac * 8);
+      std::memcpy(sl.pin + L.dens + k * 8, r.dens, (size_t)r.npts * 8);
+    }
+    for (int i = 0; i < r.npts; i++) tend_pin[k + i] = r.tend_s;
     k += r.npts;
   }
   char *dev = S->arena.base + sl.off;
@@ -199,21 +220,55 @@ bool launch_batch(pdrchem_server *S, Slot &sl) {
   q.nelect0 = s.nspec - 1;   // electron last (read_species.F90:164)
   q.mxstep = 10000000LL;     // calculate_abundances.c:111
   q.status = (int *)(dev + L.status);
-  q.stats = nullptr;
+  q.stats = sl.device ? (int *)(dev + L.stats) : nullptr;
   q.counter = (unsigned int *)(dev + L.counter);
   q.max_blocks = 0;
   q.mixed = S->precision == PDRCHEM_MIXED ? 1 : 0;
   // the batch's work on the slot's stream
   auto enqueue = [&]() -> bool {
-    if (hipMemcpyAsync(dev, sl.pin, L.status, hipMemcpyHostToDevice, sl.stream) != hipSuccess) return false;
+    if (sl.device) {
+      // inputs gathered from the requests' device arrays (device to device)
+      size_t k = 0;
+      for (const Req &r : sl.reqs) {
+        const size_t m = (size_t)r.npts;
+        if (hipMemcpyAsync(dev + L.ab + k * s.nspec * 8, r.ab, m * s.nspec * 8, hipMemcpyDeviceToDevice,
+                           sl.stream) != hipSuccess ||
+            hipMemcpyAsync(dev + L.rate + k * s.nreac * 8, r.rate, m * s.nreac * 8, hipMemcpyDeviceToDevice,
+                           sl.stream) != hipSuccess ||
+            hipMemcpyAsync(dev + L.dens + k * 8, r.dens, m * 8, hipMemcpyDeviceToDevice, sl.stream) != hipSuccess)
+          return false;
+        k += m;
+      }
+      if (hipMemcpyAsync(dev + L.tend, tend_pin, n * 8, hipMemcpyHostToDevice, sl.stream) != hipSuccess)
+        return false;
+    } else if (hipMemcpyAsync(dev, sl.pin, L.status, hipMemcpyHostToDevice, sl.stream) != hipSuccess) {
+      return false;
+    }
     if (launch_zero_u32(q.counter, sl.stream) != 0) return false;
     const int rc = net == NET_REDUCED ? launch_wave_solver(net, q, sl.stream) : launch_wg_solver(net, q, sl.stream);
     if (rc != 0) return false;
-    // abundances and statuses back to the staging block
-    if (hipMemcpyAsync(sl.pin + L.ab, dev + L.ab, (size_t)n * s.nspec * 8, hipMemcpyDeviceToHost, sl.stream) !=
-        hipSuccess)
+    if (sl.device) {
+      // results scattered back to the requests' device arrays
+      size_t k = 0;
+      for (const Req &r : sl.reqs) {
+        const size_t m = (size_t)r.npts;
+        if (hipMemcpyAsync(r.ab, dev + L.ab + k * s.nspec * 8, m * s.nspec * 8, hipMemcpyDeviceToDevice,
+                           sl.stream) != hipSuccess)
+          return false;
+        if (r.d_status && hipMemcpyAsync(r.d_status, dev + L.status + k * 4, m * 4, hipMemcpyDeviceToDevice,
+                                         sl.stream) != hipSuccess)
+          return false;
+        if (r.d_stats && hipMemcpyAsync(r.d_stats, dev + L.stats + k * 32, m * 32, hipMemcpyDeviceToDevice,
+                                        sl.stream) != hipSuccess)
+          return false;
+        k += m;
+      }
+    } else if (hipMemcpyAsync(sl.pin + L.ab, dev + L.ab, (size_t)n * s.nspec * 8, hipMemcpyDeviceToHost,
+                              sl.stream) != hipSuccess) {
       return false;
-    return hipMemcpyAsync(sl.pin + L.status, dev + L.status, n * 4, hipMemcpyDeviceToHost, sl.stream) == hipSuccess;
+    }
+    // statuses back to the staging block (request status for both kinds)
+    return hipMemcpyAsync(status_pin, dev + L.status, n * 4, hipMemcpyDeviceToHost, sl.stream) == hipSuccess;
   };
   if (!S->graphs) {
     if (!enqueue()) return false;
@@ -259,10 +314,11 @@ void retire(pdrchem_server *S) {
     const Shape &s = kShape[sl.net];
     const Layout L = layout(sl.net, sl.npts);
     const double t = now_s();
+    const int *status_pin = (const int *)(sl.device ? sl.pin2 + align256(sl.npts * 8) : sl.pin + L.status);
     size_t k = 0;
     for (const Req &r : sl.reqs) {
-      std::memcpy(r.ab, sl.pin + L.ab + k * s.nspec * 8, (size_t)r.npts * s.nspec * 8);
-      const int *st = (const int *)(sl.pin + L.status) + k;
+      if (!sl.device) std::memcpy(r.ab, sl.pin + L.ab + k * s.nspec * 8, (size_t)r.npts * s.nspec * 8);
+      const int *st = status_pin + k;
       int status = 0;
       for (int i = 0; i < r.npts && !status; i++) status = st[i];
       S->done.push_back(Done{r.ticket, status, (t - r.t_submit) * 1e3});
@@ -279,10 +335,11 @@ bool admit(pdrchem_server *S) {
   for (Slot &sl : S->slots) {
     if (sl.busy || S->pending.empty()) continue;
     const int net = S->pending.front().net;
+    const bool device = S->pending.front().device;
     std::vector<Req> take;
     size_t n = 0;
     for (auto it = S->pending.begin(); it != S->pending.end();) {
-      if (it->net == net && n + it->npts <= S->max_batch_pts) {
+      if (it->net == net && it->device == device && n + it->npts <= S->max_batch_pts) {
         n += it->npts;
         take.push_back(*it);
         it = S->pending.erase(it);
@@ -306,6 +363,7 @@ bool admit(pdrchem_server *S) {
     }
     if (take.empty()) return true;  // the oldest request waits for arena space
     sl.net = net;
+    sl.device = device;
     sl.npts = n;
     sl.off = (size_t)off;
     sl.bytes = bytes;
@@ -380,6 +438,25 @@ int pdrchem_server_submit(pdrchem_server *S, int net, int npts, double *abundanc
   return r.ticket;
 }
 
+int pdrchem_server_submit_device(pdrchem_server *S, int net, int npts, double *d_abundance, const double *d_rate,
+                                 const double *d_density, double t_end_years, int *d_status, int *d_stats) {
+  if (!S || net < 0 || net >= NET_COUNT || npts <= 0 || !d_abundance || !d_rate || !d_density) return -1;
+  std::lock_guard<std::mutex> lock(S->mu);
+  if ((size_t)npts > S->max_batch_pts || layout(net, npts).total > S->arena.cap) {
+    S->err = "request larger than the serving budget";
+    return -1;
+  }
+  Req r{S->next_ticket++, net, npts, d_abundance, d_rate, d_density, t_end_years * kYear, now_s()};
+  r.device = true;
+  r.d_status = d_status;
+  r.d_stats = d_stats;
+  S->pending.push_back(r);
+  if (!admit(S)) return -1;
+  return r.ticket;
+}
+
+int pdrchem_server_max_batch(pdrchem_server *S) { return S ? (int)S->max_batch_pts : -1; }
+
 int pdrchem_server_progress(pdrchem_server *S, int wait) {
   if (!S) return -1;
   std::lock_guard<std::mutex> lock(S->mu);
@@ -418,6 +495,7 @@ void pdrchem_server_destroy(pdrchem_server *S) {
     if (sl.done) (void)hipEventDestroy(sl.done);
     if (sl.stream) (void)hipStreamDestroy(sl.stream);
     if (sl.pin) (void)hipHostFree(sl.pin);
+    if (sl.pin2) (void)hipHostFree(sl.pin2);
   }
   if (S->arena.base) (void)hipFree(S->arena.base);
   delete S;

@@ -61,6 +61,13 @@ class Cloud:
     def _err(self) -> str:
         return lib.load().pdrchem_cloud_last_error(self._h).decode()
 
+    def use_server(self, server: Optional[lib.Server]) -> None:
+        """Send the solves of later iterations through this request loop
+        (None: direct launches)."""
+        self._server = server   # keep it alive while in use
+        if lib.load().pdrchem_cloud_set_server(self._h, server.h if server is not None else None) != 0:
+            raise lib.PdrChemError(self._err())
+
     def set_abundances(self, x: np.ndarray) -> None:
         x = np.ascontiguousarray(x, dtype=np.float64)
         if x.shape != (self.npts, self.nspec):

@@ -26,6 +26,7 @@ EXPORTED_FUNCTIONS = [
     "pdrchem_rates_device", "pdrchem_rates_device_zeta", "pdrchem_kernel_info", "pdrchem_set_max_blocks", "pdrchem_last_error",
     "pdrchem_server_create", "pdrchem_server_submit", "pdrchem_server_progress", "pdrchem_server_poll",
     "pdrchem_server_last_error", "pdrchem_server_destroy", "pdrchem_server_graph_stats",
+    "pdrchem_server_submit_device", "pdrchem_server_max_batch", "pdrchem_cloud_set_server",
     "pdrchem_cloud_create", "pdrchem_cloud_set_abundances", "pdrchem_cloud_iterate", "pdrchem_cloud_get",
     "pdrchem_cloud_num_evaluation_points", "pdrchem_cloud_evaluation_points", "pdrchem_cloud_last_error",
     "pdrchem_cloud_destroy", "pdrchem_autotune", "pdrchem_tuned_config",
@@ -103,6 +104,9 @@ def load(path: Optional[str] = None):
     L.pdrchem_server_destroy.argtypes = [vp]
     L.pdrchem_server_destroy.restype = None
     L.pdrchem_server_graph_stats.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
+    L.pdrchem_server_submit_device.argtypes = [vp, C.c_int, C.c_int, vp, vp, vp, C.c_double, vp, vp]
+    L.pdrchem_server_max_batch.argtypes = [vp]
+    L.pdrchem_cloud_set_server.argtypes = [vp, vp]
     L.pdrchem_cloud_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, dp, C.c_int, C.c_double,
                                        C.c_int, dp, C.c_double, C.POINTER(Phys), ip, C.c_int]
     L.pdrchem_cloud_create.restype = vp
@@ -305,6 +309,22 @@ class Server:
         if t < 0:
             raise PdrChemError(self.L.pdrchem_server_last_error(self.h).decode())
         self.live[t] = (abundance, rate, density)
+        return t
+
+    def submit_device(self, net: int, abundance, rate, density, t_end_years: float, status=None, stats=None) -> int:
+        """Device request: torch device tensors (abundance [n, nspec] float64,
+        updated in place; rate [n, nreac]; density [n]; optional status [n]
+        int32 and stats [n, 8] int32 outputs).  Kept referenced until done."""
+        import torch
+        for a in (abundance, rate, density):
+            if not a.is_cuda or a.dtype != torch.float64 or not a.is_contiguous():
+                raise PdrChemError("device requests take contiguous float64 cuda tensors")
+        n = abundance.shape[0]
+        t = self.L.pdrchem_server_submit_device(self.h, net, n, _ptr(abundance), _ptr(rate), _ptr(density),
+                                                float(t_end_years), _ptr(status), _ptr(stats))
+        if t < 0:
+            raise PdrChemError(self.L.pdrchem_server_last_error(self.h).decode())
+        self.live[t] = (abundance, rate, density, status, stats)
         return t
 
     def progress(self, wait: bool = False) -> int:

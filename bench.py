@@ -126,6 +126,8 @@ def main():
     ap.add_argument("--cloud-side", type=int, default=64, help="grid side (side^3 points)")
     ap.add_argument("--cloud-iters", type=int, default=8, help="chemistry iterations (CHEMITERATIONS of params.dat)")
     ap.add_argument("--cloud-tol", type=float, default=1e-3, help="convergence tolerance of a point")
+    ap.add_argument("--cloud-direct", action="store_true",
+                    help="solve the cloud's batches by direct launches instead of through the request loop")
     ap.add_argument("--cloud-fp64-side", type=int, default=24,
                     help="side of the cloud on which the plain FP64 form is compared (0 = skip)")
     a = ap.parse_args()
@@ -369,6 +371,12 @@ def cloud_bench(a):
         return c, rho, x0, t_setup
 
     c, rho, x0, t_setup = make(side)
+    srv = None
+    if not a.cloud_direct:
+        # the shrinking, uneven batches of the iterations go through the request
+        # loop as device requests (state stays in HBM; graph-launched batches)
+        srv = lib.Server(device_bytes=8 << 30, nslots=4, precision=prec)
+        c.use_server(srv)
     T = c.get(CL.TEMPERATURE)
     G = c.get(CL.G)
     Av = c.get(CL.AV)
@@ -414,7 +422,8 @@ def cloud_bench(a):
                                f"sigma_ln 1), 1/16 pc cells, G0=100 on the -x face, 12 HEALPix rays, "
                                f"{name} network, warm starts, converged points skipped (tol {a.cloud_tol})",
                    "points": side ** 3, "iterations": a.cloud_iters, "precision": a.precision,
-                   "rtol": 1e-8, "atol": 1e-30},
+                   "rtol": 1e-8, "atol": 1e-30,
+                   "solves": "direct launches" if srv is None else "request loop (device requests, 4 slots)"},
         "iterations": iters,
         "raytrace_setup_s": t_setup, "evaluation_points": n_ep,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
@@ -462,7 +471,11 @@ def cloud_bench(a):
             cc.close()
         res["fp64_comparison"] = {"points": sd ** 3, "iteration": 1, "mixed_solves_per_s": out["mixed"],
                                   "fp64_solves_per_s": out["fp64"], "ratio": out["mixed"] / out["fp64"]}
+    c.use_server(None)
     c.close()
+    if srv is not None:
+        res["request_loop_graph_launches"] = srv.graph_stats()[1]
+        srv.close()
     print(json.dumps(res))
 
 

@@ -171,6 +171,16 @@ pdrchem_server *pdrchem_server_create(size_t device_bytes, int nslots, int preci
 int pdrchem_server_submit(pdrchem_server *server, int net, int npts, double *abundance, const double *rate,
                           const double *density, double t_end_years);
 
+/* As pdrchem_server_submit with device arrays (state resident in HBM): the
+ * batch stages them by device-to-device copies and writes the abundances
+ * back in place.  d_status[npts] and d_stats[npts][8] (device, optional)
+ * receive each point's status and integrator counters as from
+ * pdrchem_solve_device.  Device and host requests are never in one batch. */
+int pdrchem_server_submit_device(pdrchem_server *server, int net, int npts, double *d_abundance,
+                                 const double *d_rate, const double *d_density, double t_end_years, int *d_status,
+                                 int *d_stats);
+/* Largest number of points of one request (and of one batch). */
+int pdrchem_server_max_batch(pdrchem_server *server);
 /* Retire finished batches (results copied into the callers' arrays) and
  * start pending ones; wait != 0 blocks until nothing is pending or in
  * flight.  Returns the number of requests pending or in flight, -1 on error. */
@@ -233,6 +243,12 @@ int pdrchem_cloud_set_abundances(pdrchem_cloud *cloud, const double *abundance);
  * converged points (3DPDR.F90:628, columns :1822), which it re-solves and discards. */
 int pdrchem_cloud_iterate(pdrchem_cloud *cloud, double t_end_years, double conv_tol, int precision,
                           pdrchem_cloud_stats *stats);
+/* Send the iterations' solves through a request loop (device requests of at
+ * most pdrchem_server_max_batch points; the loop's precision applies and
+ * the precision argument of pdrchem_cloud_iterate is ignored); NULL goes back
+ * to direct launches on the cloud's stream.  The server must outlive its use
+ * here. */
+int pdrchem_cloud_set_server(pdrchem_cloud *cloud, pdrchem_server *server);
 #define PDRCHEM_CLOUD_ABUNDANCE 0    /* [npts*nspec] */
 #define PDRCHEM_CLOUD_AV 1           /* [npts*nrays] */
 #define PDRCHEM_CLOUD_G 2            /* [npts*nrays] surface field per ray */

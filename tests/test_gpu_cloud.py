@@ -120,6 +120,34 @@ def test_iteration_matches_oracle_and_skips_converged():
     c.close()
 
 
+def test_iterations_through_request_loop_equal_direct():
+    """The cloud's solves sent through the request loop (device requests,
+    graph-launched batches) give the direct launches' abundances bitwise."""
+    shape = (5, 4, 4)
+    print(f"cloud via server {shape}", flush=True)
+    net = load_network("reduced")
+    n = int(np.prod(shape))
+    rho = smooth_density(*shape)
+    x0 = workloads.initial_abundances(net, n)
+    out = []
+    for via in (False, True):
+        c = CL.Cloud(net, shape, H, rho, gext=(100.0, 0.0, 0.0))
+        c.set_abundances(x0)
+        srv = None
+        if via:
+            srv = lib.Server(device_bytes=256 << 20, nslots=2, precision=lib.MIXED)
+            c.use_server(srv)
+        sts = [c.iterate(t_end=1e7, conv_tol=1e-3, precision=lib.MIXED) for _ in range(2)]
+        out.append((c.get(CL.ABUNDANCE), [(s["solved"], s["converged"], s["failed"], s["nst"]) for s in sts]))
+        c.use_server(None)
+        c.close()
+        if srv is not None:
+            assert srv.graph_stats()[1] >= 2
+            srv.close()
+    assert np.array_equal(out[0][0], out[1][0])
+    assert out[0][1] == out[1][1]
+
+
 def test_rejects_bad_arguments():
     net = load_network("reduced")
     with pytest.raises(lib.PdrChemError):

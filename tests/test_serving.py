@@ -119,6 +119,43 @@ def test_server_graph_replays_equal_stream_launches(monkeypatch):
 
 
 @pytest.mark.gpu
+def test_device_requests_match_host_requests():
+    """Requests whose arrays live on the device (staged device to device,
+    abundances written back in place, per-point statuses and counters
+    returned) give the host requests' results bitwise, also when both kinds
+    are in flight in one server."""
+    import torch
+    import oracle
+    from pdrchem import lib
+    orc = {nm: oracle.Oracle(nm) for nm in ("reduced", "full")}
+    sizes = [("reduced", 40), ("full", 3), ("reduced", 120), ("reduced", 7), ("full", 2)]
+    reqs = _requests(orc, sizes, seed=11)
+    lib.set_globals(rtol=1e-8, atol=1e-30)
+    dev = torch.device("cuda:0")
+    srv = lib.Server(device_bytes=256 << 20, nslots=2, precision=lib.MIXED)
+    host_out, dev_out = [], []
+    for name, x0, r, nH, te in reqs:
+        ab = x0.copy()
+        host_out.append(ab)
+        srv.submit(lib.network_id(name), ab, r, nH, te)
+        d = dict(ab=torch.tensor(x0, device=dev), r=torch.tensor(r, device=dev), nH=torch.tensor(nH, device=dev),
+                 st=torch.full((len(nH),), 99, dtype=torch.int32, device=dev),
+                 stats=torch.zeros((len(nH), 8), dtype=torch.int32, device=dev))
+        dev_out.append(d)
+        srv.submit_device(lib.network_id(name), d["ab"], d["r"], d["nH"], te, status=d["st"], stats=d["stats"])
+        srv.progress()
+    srv.progress(wait=True)
+    done = srv.poll()
+    srv.close()
+    torch.cuda.synchronize()
+    assert len(done) == 2 * len(reqs) and all(st == 0 for _, st, _ in done)
+    for h, d in zip(host_out, dev_out):
+        assert np.array_equal(h, d["ab"].cpu().numpy())
+        assert int(d["st"].abs().sum().item()) == 0
+        assert int(d["stats"][:, 0].min().item()) > 0          # nst of every point
+
+
+@pytest.mark.gpu
 def test_server_rejects_oversized():
     from pdrchem import lib
     srv = lib.Server(device_bytes=1 << 20, nslots=1)
