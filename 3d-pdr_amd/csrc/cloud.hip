@@ -31,8 +31,10 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
+#include <thread>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -147,6 +149,168 @@ int pix_of(int nside, const double r[3]) {
             (spread_bits(ix % 128) + 2 * spread_bits(iy % 128));
   const int d = kNsMax / nside;
   return ipf / (d * d) + face * nside * nside;
+}
+
+// ---------------------------------------------------------------------------
+// Point clouds: evaluation points on the host (eval_points.F90:129-225)
+// ---------------------------------------------------------------------------
+// The reference's candidate order: heapsort.F90 as it runs -- the initial
+// heap covers elements 1..n-1 only, and a value moved through the swap
+// temporary is rounded to single precision (real(kind=PR), PR = SP without
+// -DDOUBLE_PRECISION).  plist/rlist are 1-based.
+void heapsort_reference(int n, int *plist, double *rlist) {
+  auto sift = [&](int root, int end) {
+    for (;;) {
+      int child = root * 2;
+      if (child > end) break;
+      if (child < end && rlist[child] < rlist[child + 1]) child++;
+      if (!(rlist[root] < rlist[child])) break;
+      const float t = (float)rlist[root];
+      rlist[root] = rlist[child];
+      rlist[child] = (double)t;
+      std::swap(plist[root], plist[child]);
+      root = child;
+    }
+  };
+  for (int start = n / 2; start > 0; start--) sift(start, n - 1);
+  for (int end = n; end > 1;) {
+    const float t = (float)rlist[end];
+    rlist[end] = rlist[1];
+    rlist[1] = (double)t;
+    std::swap(plist[end], plist[1]);
+    end--;
+    sift(1, end);
+  }
+}
+
+struct OriginChains {
+  std::vector<int> count;       // [nrays]
+  std::vector<int> proj;        // ray-major
+  std::vector<double> step;
+};
+
+// the evaluation points of one origin, all rays
+void trace_origin(int p, int npts, const double *xyz, const int *etype, int nside, int nrays, const double *vec,
+                  double theta_crit, int order, std::vector<int> &rb, std::vector<double> &ra,
+                  std::vector<std::pair<double, int>> &srt, OriginChains &out) {
+  const double *o = xyz + 3 * (size_t)p;
+  const int n1 = npts - 1;
+  int kk = 0;
+  for (int i = 0; i < npts; i++) {
+    if (i == p) continue;
+    kk++;
+    const double r0 = xyz[3 * (size_t)i] - o[0], r1 = xyz[3 * (size_t)i + 1] - o[1], r2 = xyz[3 * (size_t)i + 2] - o[2];
+    ra[kk] = std::sqrt(r0 * r0 + r1 * r1 + r2 * r2);
+    rb[kk] = i;
+  }
+  if (order == PDRCHEM_ORDER_REFERENCE) {
+    heapsort_reference(n1, rb.data(), ra.data());
+  } else {
+    for (int k = 1; k <= n1; k++) srt[k - 1] = {ra[k], rb[k]};
+    std::sort(srt.begin(), srt.begin() + n1);
+    for (int k = 1; k <= n1; k++) {
+      ra[k] = srt[k - 1].first;
+      rb[k] = srt[k - 1].second;
+    }
+  }
+  const double radius = n1 > 0 ? ra[n1] : 0.0;
+  std::vector<double> ep(3 * nrays, 0.0), prev(3 * nrays);
+  std::vector<char> kill(nrays, 0);
+  std::vector<std::vector<int>> pj(nrays);
+  std::vector<std::vector<double>> st(nrays);
+  for (int j = 0; j < nrays; j++) {
+    prev[3 * j] = o[0];
+    prev[3 * j + 1] = o[1];
+    prev[3 * j + 2] = o[2];
+  }
+  for (int k = 1; k <= n1; k++) {
+    const int id = rb[k];
+    const double rv[3] = {xyz[3 * (size_t)id] - o[0], xyz[3 * (size_t)id + 1] - o[1], xyz[3 * (size_t)id + 2] - o[2]};
+    const int j = pix_of(nside, rv);
+    if (kill[j]) continue;
+    double *e = &ep[3 * j];
+    double h[3];
+    for (int c = 0; c < 3; c++) h[c] = 1.1 * radius * vec[3 * j + c];
+    const double a0 = rv[0] - e[0], a1 = rv[1] - e[1], a2 = rv[2] - e[2];
+    const double b0 = h[0] - e[0], b1 = h[1] - e[1], b2 = h[2] - e[2];
+    const double dot = a0 * b0 + a1 * b1 + a2 * b2;
+    const double na = std::sqrt(a0 * a0 + a1 * a1 + a2 * a2), nb = std::sqrt(b0 * b0 + b1 * b1 + b2 * b2);
+    const double ang = std::acos(dot / (na * nb));
+    if (!(ang <= theta_crit)) continue;
+    if (h[2] != 0.0) {
+      e[2] = (h[0] * h[2] * rv[0] + h[1] * h[2] * rv[1] + (h[2] * h[2]) * rv[2]) /
+             (h[0] * h[0] + h[1] * h[1] + h[2] * h[2]);
+      e[1] = e[2] * h[1] / h[2];
+      e[0] = e[2] * h[0] / h[2];
+    } else if (h[0] == 0.0) {
+      e[0] = 0.0;
+      e[1] = rv[1];
+      e[2] = 0.0;
+    } else if (h[1] == 0.0) {
+      e[0] = rv[0];
+      e[1] = 0.0;
+      e[2] = 0.0;
+    } else {
+      e[2] = 0.0;
+      e[0] = ((h[0] * h[0]) * rv[0] + h[0] * h[1] * rv[1]) / (h[0] * h[0] + h[1] * h[1]);
+      e[1] = e[0] * h[1] / h[0];
+    }
+    const double c0 = e[0] + o[0], c1 = e[1] + o[1], c2 = e[2] + o[2];
+    double *q = &prev[3 * j];
+    pj[j].push_back(id);
+    st[j].push_back(std::sqrt((q[0] - c0) * (q[0] - c0) + (q[1] - c1) * (q[1] - c1) + (q[2] - c2) * (q[2] - c2)));
+    q[0] = c0;
+    q[1] = c1;
+    q[2] = c2;
+    if (etype && etype[id] == 2) kill[j] = 1;
+  }
+  out.count.assign(nrays, 0);
+  out.proj.clear();
+  out.step.clear();
+  for (int j = 0; j < nrays; j++) {
+    out.count[j] = (int)pj[j].size();
+    out.proj.insert(out.proj.end(), pj[j].begin(), pj[j].end());
+    out.step.insert(out.step.end(), st[j].begin(), st[j].end());
+  }
+}
+
+// every origin, threads over origins; CSR in (origin, ray) order
+void trace_points_host(int npts, const double *xyz, const int *etype, int level, double theta_crit, int order,
+                       std::vector<int> &count, std::vector<long long> &offs, std::vector<int> &proj,
+                       std::vector<double> &step) {
+  const int nside = 1 << level, nrays = 12 * nside * nside;
+  std::vector<double> vec(3 * nrays);
+  for (int j = 0; j < nrays; j++) pix_centre(nside, j, &vec[3 * j]);
+  std::vector<OriginChains> res(npts);
+  unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  std::atomic<int> next(0);
+  auto work = [&]() {
+    std::vector<int> rb(npts + 1);
+    std::vector<double> ra(npts + 1);
+    std::vector<std::pair<double, int>> srt(npts + 1);
+    for (int p; (p = next.fetch_add(1)) < npts;)
+      trace_origin(p, npts, xyz, etype, nside, nrays, vec.data(), theta_crit, order, rb, ra, srt, res[p]);
+  };
+  std::vector<std::thread> th;
+  for (unsigned t = 1; t < nth; t++) th.emplace_back(work);
+  work();
+  for (auto &t : th) t.join();
+  count.assign((size_t)npts * nrays, 0);
+  offs.assign((size_t)npts * nrays + 1, 0);
+  long long tot = 0;
+  for (int p = 0; p < npts; p++)
+    for (int j = 0; j < nrays; j++) {
+      offs[(size_t)p * nrays + j] = tot;
+      count[(size_t)p * nrays + j] = res[p].count[j];
+      tot += res[p].count[j];
+    }
+  offs[(size_t)npts * nrays] = tot;
+  proj.resize(tot);
+  step.resize(tot);
+  for (int p = 0; p < npts; p++) {
+    std::copy(res[p].proj.begin(), res[p].proj.end(), proj.begin() + offs[(size_t)p * nrays]);
+    std::copy(res[p].step.begin(), res[p].step.end(), step.begin() + offs[(size_t)p * nrays]);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -481,6 +645,27 @@ bool build_rays(pdrchem_cloud *c) {
   return true;
 }
 
+// evaluation points of a point cloud traced on the host, uploaded as CSR
+bool upload_chains(pdrchem_cloud *c, const double *xyz, const int *etype, int order) {
+  std::vector<int> cnt, pj;
+  std::vector<long long> of;
+  std::vector<double> st;
+  trace_points_host(c->npts, xyz, etype, c->level, c->theta_crit, order, cnt, of, pj, st);
+  c->total = (long long)pj.size();
+  hipStream_t s = c->stream;
+  if (!c->alloc(&c->d_count, cnt.size(), "ray counts") || !c->alloc(&c->d_offs, of.size(), "ray offsets") ||
+      !c->alloc(&c->d_proj, pj.size(), "evaluation points") || !c->alloc(&c->d_step, st.size(), "evaluation points"))
+    return c->fail("allocation");
+  const bool good = ok(hipMemcpyAsync(c->d_count, cnt.data(), cnt.size() * 4, hipMemcpyHostToDevice, s), "copy") &&
+                    ok(hipMemcpyAsync(c->d_offs, of.data(), of.size() * 8, hipMemcpyHostToDevice, s), "copy") &&
+                    (pj.empty() || ok(hipMemcpyAsync(c->d_proj, pj.data(), pj.size() * 4, hipMemcpyHostToDevice, s),
+                                      "copy")) &&
+                    (st.empty() || ok(hipMemcpyAsync(c->d_step, st.data(), st.size() * 8, hipMemcpyHostToDevice, s),
+                                      "copy")) &&
+                    ok(hipStreamSynchronize(s), "sync");
+  return good || c->fail("upload of evaluation points");
+}
+
 bool columns(pdrchem_cloud *c, bool with_species) {
   ColumnArgs a{};
   a.npts = c->npts;
@@ -505,10 +690,10 @@ bool columns(pdrchem_cloud *c, bool with_species) {
 
 extern "C" {
 
-pdrchem_cloud *pdrchem_cloud_create(int net, int nx, int ny, int nz, double spacing_pc, const double *density,
-                                    int level, double theta_crit, int fieldchoice, const double *gext,
-                                    double dust_temperature, const pdrchem_phys *phys, const int *shield_slots,
-                                    int nelect) {
+static pdrchem_cloud *create_common(int net, int nx, int ny, int nz, double spacing_pc, int npts_pc,
+                                    const double *xyz, const int *etype, int order, const double *density, int level,
+                                    double theta_crit, int fieldchoice, const double *gext, double dust_temperature,
+                                    const pdrchem_phys *phys, const int *shield_slots, int nelect) {
   t_err.clear();
   int nspec = 0, nreac = 0;
   const int shapes[3][2] = {{33, 329}, {215, 2926}, {58, 634}};
@@ -518,10 +703,14 @@ pdrchem_cloud *pdrchem_cloud_create(int net, int nx, int ny, int nz, double spac
   }
   nspec = shapes[net][0];
   nreac = shapes[net][1];
-  if (nx < 2 || ny < 2 || nz < 2 || nx > 128 || ny > 128 || nz > 128 || !(spacing_pc > 0) || !density || !gext ||
-      !phys || level < 0 || level > 3 || !(theta_crit > 0) || theta_crit >= 1.5707963267948966 ||
-      nelect < 1 || nelect > nspec || (fieldchoice != 0 && fieldchoice != 1)) {
-    t_err = "bad argument to pdrchem_cloud_create (grid sides 2..128, HEALPix level 0..3, 0 < theta_crit < pi/2)";
+  const bool grid = xyz == nullptr;
+  if ((grid && (nx < 2 || ny < 2 || nz < 2 || nx > 128 || ny > 128 || nz > 128 || !(spacing_pc > 0))) ||
+      (!grid && (npts_pc < 2 || npts_pc > 200000 ||
+                 (order != PDRCHEM_ORDER_REFERENCE && order != PDRCHEM_ORDER_DISTANCE))) ||
+      !density || !gext || !phys || level < 0 || level > 3 || !(theta_crit > 0) ||
+      theta_crit >= 1.5707963267948966 || nelect < 1 || nelect > nspec || (fieldchoice != 0 && fieldchoice != 1)) {
+    t_err = "bad argument to pdrchem_cloud_create (grid sides 2..128 or 2..200000 points, HEALPix level 0..3, "
+            "0 < theta_crit < pi/2)";
     return nullptr;
   }
   pdrchem_cloud *c = new pdrchem_cloud();
@@ -532,7 +721,7 @@ pdrchem_cloud *pdrchem_cloud_create(int net, int nx, int ny, int nz, double spac
   c->nx = nx;
   c->ny = ny;
   c->nz = nz;
-  c->npts = nx * ny * nz;
+  c->npts = grid ? nx * ny * nz : npts_pc;
   c->level = level;
   c->nrays = 12 << (2 * level);
   c->h = spacing_pc;
@@ -586,7 +775,8 @@ pdrchem_cloud *pdrchem_cloud_create(int net, int nx, int ny, int nz, double spac
   good = c->alloc(&d_gray, nr, "field") &&
          ok(hipMemcpyAsync(d_gray, gray.data(), nr * 8, hipMemcpyHostToDevice, s), "copy") &&
          ok(hipMemcpyAsync(c->d_rho, density, (size_t)n * 8, hipMemcpyHostToDevice, s), "copy") &&
-         ok(hipMemsetAsync(c->d_conv, 0, n, s), "memset") && build_rays(c) && columns(c, false);
+         ok(hipMemsetAsync(c->d_conv, 0, n, s), "memset") &&
+         (grid ? build_rays(c) : upload_chains(c, xyz, etype, order)) && columns(c, false);
   if (good) {
     hipLaunchKernelGGL(uvfield_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, nr, d_gray, c->d_Av,
                        c->uv_fac, c->d_G, c->d_uv, c->d_T);
@@ -608,6 +798,45 @@ pdrchem_cloud *pdrchem_cloud_create(int net, int nx, int ny, int nz, double spac
     return nullptr;
   }
   return c;
+}
+
+pdrchem_cloud *pdrchem_cloud_create(int net, int nx, int ny, int nz, double spacing_pc, const double *density,
+                                    int level, double theta_crit, int fieldchoice, const double *gext,
+                                    double dust_temperature, const pdrchem_phys *phys, const int *shield_slots,
+                                    int nelect) {
+  return create_common(net, nx, ny, nz, spacing_pc, 0, nullptr, nullptr, 0, density, level, theta_crit, fieldchoice,
+                       gext, dust_temperature, phys, shield_slots, nelect);
+}
+
+pdrchem_cloud *pdrchem_cloud_create_points(int net, int npts, const double *xyz, const int *etype, int order,
+                                           const double *density, int level, double theta_crit, int fieldchoice,
+                                           const double *gext, double dust_temperature, const pdrchem_phys *phys,
+                                           const int *shield_slots, int nelect) {
+  if (!xyz) {
+    t_err = "pdrchem_cloud_create_points needs positions";
+    return nullptr;
+  }
+  return create_common(net, 0, 0, 0, 0.0, npts, xyz, etype, order, density, level, theta_crit, fieldchoice, gext,
+                       dust_temperature, phys, shield_slots, nelect);
+}
+
+long long pdrchem_trace_points(int npts, const double *xyz, const int *etype, int level, double theta_crit, int order,
+                               int *count, long long *offsets, int *proj, double *step, long long capacity) {
+  if (npts < 2 || !xyz || level < 0 || level > 3 || !(theta_crit > 0) || theta_crit >= 1.5707963267948966 ||
+      (order != PDRCHEM_ORDER_REFERENCE && order != PDRCHEM_ORDER_DISTANCE))
+    return -1;
+  std::vector<int> cnt, pj;
+  std::vector<long long> of;
+  std::vector<double> st;
+  trace_points_host(npts, xyz, etype, level, theta_crit, order, cnt, of, pj, st);
+  if (count) std::copy(cnt.begin(), cnt.end(), count);
+  if (offsets) std::copy(of.begin(), of.end(), offsets);
+  const long long total = (long long)pj.size();
+  if (capacity >= total) {
+    if (proj) std::copy(pj.begin(), pj.end(), proj);
+    if (step) std::copy(st.begin(), st.end(), step);
+  }
+  return total;
 }
 
 int pdrchem_cloud_set_abundances(pdrchem_cloud *c, const double *abundance) {

@@ -58,6 +58,39 @@ class Cloud:
         if not self._h:
             raise lib.PdrChemError(L.pdrchem_cloud_last_error(None).decode())
 
+    @classmethod
+    def from_points(cls, net: Network, xyz: np.ndarray, density: np.ndarray, gext=(100.0, 0.0, 0.0),
+                    fieldchoice: int = UNI, level: int = 0, theta_crit: float = 1.3, dust_temperature: float = 20.0,
+                    phys: Optional[lib.Phys] = None, order: int = lib.ORDER_DISTANCE,
+                    etype: Optional[np.ndarray] = None) -> "Cloud":
+        """A cloud of arbitrary points (x, y, z in pc; n_H), as the reference's
+        input files hold; evaluation points in `order` (ORDER_REFERENCE: the
+        reference's heapsort order)."""
+        L = lib.load()
+        self = cls.__new__(cls)
+        self.net = net
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+        self.npts = xyz.shape[0]
+        self.nx, self.ny, self.nz = self.npts, 1, 1
+        self.nrays = 12 << (2 * level)
+        self.nspec = net.nspec
+        rho = np.ascontiguousarray(density, dtype=np.float64).reshape(-1)
+        if rho.shape[0] != self.npts:
+            raise lib.PdrChemError("density must have one entry per point")
+        g = np.ascontiguousarray(gext, dtype=np.float64)
+        self._phys = phys or lib.default_phys()
+        slots = shield_slots(net)
+        et = None if etype is None else np.ascontiguousarray(etype, dtype=np.int32)
+        dp, ip = C.POINTER(C.c_double), C.POINTER(C.c_int)
+        self._h = L.pdrchem_cloud_create_points(lib.network_id(net.name), self.npts, xyz.ctypes.data_as(dp),
+                                                None if et is None else et.ctypes.data_as(ip), order,
+                                                rho.ctypes.data_as(dp), level, theta_crit, fieldchoice,
+                                                g.ctypes.data_as(dp), dust_temperature, C.byref(self._phys),
+                                                slots.ctypes.data_as(ip), net.nspec)
+        if not self._h:
+            raise lib.PdrChemError(L.pdrchem_cloud_last_error(None).decode())
+        return self
+
     def _err(self) -> str:
         return lib.load().pdrchem_cloud_last_error(self._h).decode()
 

@@ -27,6 +27,7 @@ EXPORTED_FUNCTIONS = [
     "pdrchem_server_create", "pdrchem_server_submit", "pdrchem_server_progress", "pdrchem_server_poll",
     "pdrchem_server_last_error", "pdrchem_server_destroy", "pdrchem_server_graph_stats",
     "pdrchem_server_submit_device", "pdrchem_server_max_batch", "pdrchem_cloud_set_server",
+    "pdrchem_cloud_create_points", "pdrchem_trace_points",
     "pdrchem_cloud_create", "pdrchem_cloud_set_abundances", "pdrchem_cloud_iterate", "pdrchem_cloud_get",
     "pdrchem_cloud_num_evaluation_points", "pdrchem_cloud_evaluation_points", "pdrchem_cloud_last_error",
     "pdrchem_cloud_destroy", "pdrchem_autotune", "pdrchem_tuned_config",
@@ -107,6 +108,12 @@ def load(path: Optional[str] = None):
     L.pdrchem_server_submit_device.argtypes = [vp, C.c_int, C.c_int, vp, vp, vp, C.c_double, vp, vp]
     L.pdrchem_server_max_batch.argtypes = [vp]
     L.pdrchem_cloud_set_server.argtypes = [vp, vp]
+    L.pdrchem_cloud_create_points.argtypes = [C.c_int, C.c_int, dp, ip, C.c_int, dp, C.c_int, C.c_double, C.c_int,
+                                              dp, C.c_double, C.POINTER(Phys), ip, C.c_int]
+    L.pdrchem_cloud_create_points.restype = vp
+    L.pdrchem_trace_points.argtypes = [C.c_int, dp, ip, C.c_int, C.c_double, C.c_int, ip,
+                                       C.POINTER(C.c_longlong), ip, dp, C.c_longlong]
+    L.pdrchem_trace_points.restype = C.c_longlong
     L.pdrchem_cloud_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, dp, C.c_int, C.c_double,
                                        C.c_int, dp, C.c_double, C.POINTER(Phys), ip, C.c_int]
     L.pdrchem_cloud_create.restype = vp
@@ -252,6 +259,34 @@ def tuned_config(net: int, npts: int, precision=MIXED):
 
 def clear_tuning():
     load().pdrchem_clear_tuning()
+
+
+ORDER_REFERENCE = 0   # PDRCHEM_ORDER_REFERENCE: heapsort.F90's candidate order
+ORDER_DISTANCE = 1    # PDRCHEM_ORDER_DISTANCE: (distance, point index)
+
+
+def trace_points(xyz, level=0, theta_crit=1.3, order=ORDER_DISTANCE, etype=None):
+    """Evaluation points of a point cloud (the library's host tracer; no
+    device use).  Returns (count[n, nrays], offsets[n*nrays+1], proj, step)."""
+    L = load()
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+    n = xyz.shape[0]
+    nrays = 12 << (2 * level)
+    et = None if etype is None else np.ascontiguousarray(etype, dtype=np.int32)
+    etp = None if et is None else et.ctypes.data_as(C.POINTER(C.c_int))
+    count = np.zeros((n, nrays), dtype=np.int32)
+    offs = np.zeros(n * nrays + 1, dtype=np.int64)
+    dp, ip = C.POINTER(C.c_double), C.POINTER(C.c_int)
+    tot = L.pdrchem_trace_points(n, xyz.ctypes.data_as(dp), etp, level, theta_crit, order, count.ctypes.data_as(ip),
+                                 offs.ctypes.data_as(C.POINTER(C.c_longlong)), None, None, 0)
+    if tot < 0:
+        raise PdrChemError("bad argument to pdrchem_trace_points")
+    proj = np.zeros(max(tot, 1), dtype=np.int32)
+    step = np.zeros(max(tot, 1))
+    L.pdrchem_trace_points(n, xyz.ctypes.data_as(dp), etp, level, theta_crit, order, count.ctypes.data_as(ip),
+                           offs.ctypes.data_as(C.POINTER(C.c_longlong)), proj.ctypes.data_as(ip),
+                           step.ctypes.data_as(dp), tot)
+    return count, offs, proj[:tot], step[:tot]
 
 
 def set_tuned_config(net: int, npts: int, systems_per_block: int, blocks_per_cu: int, precision=MIXED):

@@ -231,6 +231,27 @@ pdrchem_cloud *pdrchem_cloud_create(int net, int nx, int ny, int nz, double spac
                                     int level, double theta_crit, int fieldchoice, const double *gext,
                                     double dust_temperature, const pdrchem_phys *phys, const int *shield_slots,
                                     int nelect);
+/* Candidate order of the evaluation-point search among equidistant points */
+#define PDRCHEM_ORDER_REFERENCE 0  /* heapsort.F90's order, as the reference runs it */
+#define PDRCHEM_ORDER_DISTANCE 1   /* by (distance, point index) */
+/* As pdrchem_cloud_create for an arbitrary point cloud (the reference's
+ * input: one x, y, z in pc and one n_H per point, read at 3DPDR.F90:166-169):
+ * xyz[npts*3]; etype[npts] (NULL = all PDR points; 2 = ionised: a ray stops at
+ * it, eval_points.F90:222); order as above.  The evaluation points are found
+ * on the host by a threaded restatement of evaluation_points (O(N^2 log N),
+ * done once; up to 200,000 points); everything after that runs on the device
+ * as for a grid. */
+pdrchem_cloud *pdrchem_cloud_create_points(int net, int npts, const double *xyz, const int *etype, int order,
+                                           const double *density, int level, double theta_crit, int fieldchoice,
+                                           const double *gext, double dust_temperature, const pdrchem_phys *phys,
+                                           const int *shield_slots, int nelect);
+/* The evaluation points of a point cloud alone (host only, no device use):
+ * count[npts*nrays] and offsets[npts*nrays+1] always written, proj and step
+ * (projected point ids, step lengths in pc) when capacity >= the total, which
+ * is returned (-1 on bad arguments). */
+long long pdrchem_trace_points(int npts, const double *xyz, const int *etype, int level, double theta_crit,
+                               int order, int *count, long long *offsets, int *proj, double *step,
+                               long long capacity);
 /* Abundances of every point (host [npts*nspec]); recomputes the columns and
  * marks every point unconverged. */
 int pdrchem_cloud_set_abundances(pdrchem_cloud *cloud, const double *abundance);

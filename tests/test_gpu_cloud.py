@@ -148,6 +148,48 @@ def test_iterations_through_request_loop_equal_direct():
     assert out[0][1] == out[1][1]
 
 
+@pytest.mark.parametrize("order", [lib.ORDER_REFERENCE, lib.ORDER_DISTANCE])
+def test_point_cloud_matches_oracle(order):
+    """A cloud of arbitrary points (a jittered sphere and a log-spaced 1D slab,
+    the shapes of the reference's input files): the device columns, field and
+    temperature from the host tracer's evaluation points equal the oracle's
+    (in the same candidate order), and one iteration is within the parity bar
+    of the FP64 oracle."""
+    print(f"point cloud order={order}", flush=True)
+    rng = np.random.default_rng(17)
+    d = rng.normal(size=(120, 3))
+    sphere = d / np.linalg.norm(d, axis=1)[:, None] * rng.uniform(0, 1, (120, 1)) ** (1 / 3) * 0.5
+    slab = np.zeros((40, 3))
+    slab[1:, 0] = np.logspace(-5, 0, 39)
+    net = load_network("reduced")
+    for xyz in (sphere, slab):
+        n = len(xyz)
+        rho = 1e3 * np.exp(0.5 * np.sin(7.0 * xyz[:, 0]) + 0.2 * rng.normal(size=n))
+        c = CL.Cloud.from_points(net, xyz, rho, gext=(100.0, 0.0, 0.0), order=order)
+        x0 = workloads.initial_abundances(net, n)
+        c.set_abundances(x0)
+        e, p, ep = O.eval_points(xyz, order=O.ORDER_HEAPSORT if order == lib.ORDER_REFERENCE else O.ORDER_STABLE,
+                                 maxpoints=600)
+        slots = CL.shield_slots(net)
+        xs = np.where(slots[None, :] >= 0, x0[:, np.maximum(slots, 0)], 0.0)
+        Av, cols = O.ray_columns(e, p, ep, rho, xsel=xs)
+        G, uv, T = O.uvfield(Av, gext=(100.0, 0.0, 0.0))
+        assert np.allclose(c.get(CL.AV), Av, rtol=1e-14, atol=0)
+        assert np.allclose(c.get(CL.COLUMNS), cols, rtol=1e-14, atol=0)
+        assert np.allclose(c.get(CL.TEMPERATURE), T, rtol=1e-13, atol=0)
+        st = c.iterate(t_end=1e7, conv_tol=1e-3, precision=lib.MIXED)
+        assert st["solved"] == n and st["failed"] == 0
+        got = c.get(CL.ABUNDANCE)
+        rate = O.Oracle("reduced").rates(T, 20.0, G, Av, cols)
+        s_, ref, _ = O.Oracle("reduced").calculate_abundances(x0.copy(), rate, rho, T, mode=O.MODE_GPU)
+        assert s_ == 0
+        m = ref[:, :-1] > 1e-20
+        err = np.max(np.abs(got[:, :-1][m] - ref[:, :-1][m]) / ref[:, :-1][m])
+        print(f"  {n} points: max rel err vs oracle {err:.2e}", flush=True)
+        assert err <= 1e-4
+        c.close()
+
+
 def test_rejects_bad_arguments():
     net = load_network("reduced")
     with pytest.raises(lib.PdrChemError):

@@ -11,6 +11,7 @@ import numpy as np
 import pytest
 
 import oracle as O
+from pdrchem import lib
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 GOLD = np.load(os.path.join(HERE, "golden", "raytrace_ref.npz"))
@@ -132,3 +133,42 @@ def test_live_reference_on_random_cloud():
     vec, e_r, p_r, ep_r = O.ref_eval_points(xyz, maxpoints=MAXP)
     e_o, p_o, ep_o = O.eval_points(xyz, order=O.ORDER_HEAPSORT, maxpoints=MAXP)
     assert np.array_equal(e_r, e_o) and np.array_equal(p_r, p_o) and np.array_equal(ep_r, ep_o)
+
+
+def _library_chains(xyz, order, etype=None, maxp=MAXP):
+    """The library's host tracer in the oracle's padded layout."""
+    count, offs, proj, step = lib.trace_points(xyz, order=order, etype=etype)
+    n, nr = count.shape
+    P = np.zeros((n, nr, maxp + 1), dtype=np.int32)
+    S = np.zeros((n, nr, maxp + 1))
+    for q in range(n):
+        for j in range(nr):
+            o, k = offs[q * nr + j], count[q, j]
+            P[q, j, 0] = q
+            P[q, j, 1:k + 1] = proj[o:o + k]
+            S[q, j, 1:k + 1] = step[o:o + k]
+    return count, P, S
+
+
+@pytest.mark.parametrize("name", ["jitter", "grid", "grid_ion"])
+def test_library_tracer_reference_order_matches_reference_bitwise(name):
+    """The product's host tracer (libpdrchem.so, PDRCHEM_ORDER_REFERENCE) finds
+    the reference's evaluation points, and its step lengths are those of the
+    reference's evaluation-point coordinates, bitwise."""
+    xyz, et = GOLD[f"{name}_xyz"], GOLD[f"{name}_etype"]
+    count, P, S = _library_chains(xyz, lib.ORDER_REFERENCE, etype=et)
+    assert np.array_equal(count, GOLD[f"{name}_epray"])
+    assert np.array_equal(P, GOLD[f"{name}_proj"])
+    ep = GOLD[f"{name}_epoint"]
+    d = ep[:, :, 1:, :] - ep[:, :, :-1, :]
+    ref_step = np.sqrt((d[..., 0] * d[..., 0] + d[..., 1] * d[..., 1]) + d[..., 2] * d[..., 2])
+    mask = np.arange(1, MAXP + 1)[None, None, :] <= count[:, :, None]
+    assert np.array_equal(S[:, :, 1:][mask], ref_step[mask])
+
+
+@pytest.mark.parametrize("name", ["jitter", "grid"])
+def test_library_tracer_distance_order_matches_oracle(name):
+    xyz = GOLD[f"{name}_xyz"]
+    count, P, S = _library_chains(xyz, lib.ORDER_DISTANCE)
+    e, p, ep = O.eval_points(xyz, order=O.ORDER_STABLE, maxpoints=MAXP)
+    assert np.array_equal(count, e) and np.array_equal(P, p)
