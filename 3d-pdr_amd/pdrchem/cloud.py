@@ -47,6 +47,8 @@ class Cloud:
         rho = np.ascontiguousarray(density, dtype=np.float64).reshape(-1)
         if rho.shape[0] != self.npts:
             raise lib.PdrChemError("density must have nx*ny*nz entries")
+        self.density = rho.copy()
+        self.dust_temperature = float(dust_temperature)
         nid = lib.network_id(net.name)
         g = np.ascontiguousarray(gext, dtype=np.float64)
         self._phys = phys or lib.default_phys()
@@ -77,6 +79,8 @@ class Cloud:
         rho = np.ascontiguousarray(density, dtype=np.float64).reshape(-1)
         if rho.shape[0] != self.npts:
             raise lib.PdrChemError("density must have one entry per point")
+        self.density = rho.copy()
+        self.dust_temperature = float(dust_temperature)
         g = np.ascontiguousarray(gext, dtype=np.float64)
         self._phys = phys or lib.default_phys()
         slots = shield_slots(net)
