@@ -95,7 +95,7 @@ __device__ __forceinline__ double ipow(double x, int n) {
 template <class T>
 __device__ __forceinline__ double wrms(T &c, double v, double w) {
   double p = c.own() ? v * w : 0.0;
-  return sqrt(c.sum(p * p) / NET::NEQ);
+  return sqrt(c.sum_own_sq(p * p) / NET::NEQ);
 }
 
 template <class T>
@@ -254,7 +254,6 @@ __device__ __forceinline__ int newton_iter(T &c, BdfU LDSA &u, S &b) {
     double crate = u.crate;
     if (m > 0) crate = fmax(CRDOWN * crate, del / delp);
     const double dcon = del * fmin(1.0, crate) / u.tq[4];
-    const double acn = wrms(c, b.acor, b.ewt);
     const int jcur = u.jcur;
     c.sync();
     if (c.lead()) {
@@ -262,8 +261,12 @@ __device__ __forceinline__ int newton_iter(T &c, BdfU LDSA &u, S &b) {
       u.crate = crate;
     }
     if (dcon <= 1.0) {
+      // the norm of the accumulated correction is needed only here, and only
+      // after more than one iteration (dcon is uniform, so the whole team
+      // takes this branch together)
+      const double acn = (m == 0) ? del : wrms(c, b.acor, b.ewt);
       if (c.lead()) {
-        u.acnrm = (m == 0) ? del : acn;
+        u.acnrm = acn;
         u.jcur = 0;
       }
       c.sync();

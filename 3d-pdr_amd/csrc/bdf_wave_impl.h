@@ -147,6 +147,8 @@ struct Ctx {
   __device__ __forceinline__ bool lead() const { return ln.L == 0; }
   __device__ __forceinline__ bool own() const { return ln.owner; }
   __device__ __forceinline__ double sum(double v) { return wave_sum(v); }
+  // sum of non-negative values held by the owner lanes (0..31) only
+  __device__ __forceinline__ double sum_own_sq(double v) { return wave_sum_lo(v); }
   __device__ __forceinline__ double vmax(double v) { return wave_max(v); }
   __device__ __forceinline__ void sync() { wsync(); }
   __device__ __forceinline__ double rhs(double y, double &xe);

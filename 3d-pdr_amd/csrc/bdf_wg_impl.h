@@ -126,6 +126,7 @@ struct Team {
     red_buf ^= 1;
     return r;
   }
+  __device__ __forceinline__ double sum_own_sq(double v) { return sum(v); }
   __device__ __forceinline__ double vmax(double v) {
     v = wave_max(v);
     if (lane == 0) S->red[red_buf][wave] = v;

@@ -86,6 +86,14 @@ __device__ __forceinline__ double wave_sum(double v) {
   return (rdlane(v, 0) + rdlane(v, 16)) + (rdlane(v, 32) + rdlane(v, 48));
 }
 
+// the same for a value that is zero outside lanes 0..31 and non-negative
+// (the squares of a norm): the two upper rows contribute +0, which leaves the
+// sum unchanged, so they are not read
+__device__ __forceinline__ double wave_sum_lo(double v) {
+  v = row_sum(v);
+  return rdlane(v, 0) + rdlane(v, 16);
+}
+
 // max over all lanes, result uniform
 __device__ __forceinline__ double wave_max(double v) {
   v = fmax(v, dpp<DPP_XOR1>(v));
