@@ -110,7 +110,7 @@ __device__ __forceinline__ double f_eval(T &c, BdfU LDSA &u, double y) {
 template <class T, class S>
 __device__ __forceinline__ void rescale(T &c, BdfU LDSA &u, S &b) {
   const double eta = u.eta;
-  const int q = u.q;
+  const int q = uni(u.q);
   const double h = u.hscale * eta;
   double factor = eta;
   b.hist_in();
@@ -134,7 +134,7 @@ __device__ __forceinline__ void rescale(T &c, BdfU LDSA &u, S &b) {
 template <class T, class S>
 __device__ __forceinline__ void predict(T &c, BdfU LDSA &u, S &b) {
   const double tn = u.tn + u.h;
-  const int q = u.q;
+  const int q = uni(u.q);
   b.hist_in();
 #pragma unroll
   for (int k = 1; k <= QMAX; k++)
@@ -149,7 +149,7 @@ __device__ __forceinline__ void predict(T &c, BdfU LDSA &u, S &b) {
 
 template <class T, class S>
 __device__ __forceinline__ void restore(T &c, BdfU LDSA &u, S &b, double saved_t) {
-  const int q = u.q;
+  const int q = uni(u.q);
   b.hist_in();
 #pragma unroll
   for (int k = 1; k <= QMAX; k++)
@@ -165,7 +165,7 @@ __device__ __forceinline__ void restore(T &c, BdfU LDSA &u, S &b, double saved_t
 // BDF coefficients l[] and error-test constants tq[] (uniform scalar code)
 template <class T>
 __device__ __forceinline__ void set_coeffs(T &c, BdfU LDSA &u) {
-  const int q = u.q;
+  const int q = uni(u.q);
   const double h = u.h;
   double l[QMAX + 1], tau[QMAX + 2], tq[6];
 #pragma unroll
@@ -204,7 +204,7 @@ __device__ __forceinline__ void set_coeffs(T &c, BdfU LDSA &u) {
   double A2 = 1.0 + q * A1;
   tq[2] = fabs(A1 / (alpha0 * A2));
   tq[5] = fabs(A2 * xistar_inv / (lq * xi_inv));
-  if (u.qwait == 1) {
+  if (uni(u.qwait) == 1) {
     if (q > 1) {
       double C = xistar_inv / lq;
       double A3 = alpha0 + 1.0 / q;
@@ -224,7 +224,7 @@ __device__ __forceinline__ void set_coeffs(T &c, BdfU LDSA &u) {
   tq[4] = NLSCOEF / tq[2];
   const double rl1 = 1.0 / l[1];
   const double gamma = h * rl1;
-  const int nst = u.nst;
+  const int nst = uni(u.nst);
   const double gammap = (nst == 0) ? gamma : u.gammap;
   const double gamrat = (nst > 0) ? gamma / gammap : 1.0;
   c.sync();
@@ -254,7 +254,7 @@ __device__ __forceinline__ int newton_iter(T &c, BdfU LDSA &u, S &b) {
     double crate = u.crate;
     if (m > 0) crate = fmax(CRDOWN * crate, del / delp);
     const double dcon = del * fmin(1.0, crate) / u.tq[4];
-    const int jcur = u.jcur;
+    const int jcur = uni(u.jcur);
     c.sync();
     if (c.lead()) {
       u.nni++;
@@ -288,15 +288,15 @@ __device__ __forceinline__ int newton_iter(T &c, BdfU LDSA &u, S &b) {
 template <class T, class S>
 __device__ __forceinline__ int nls(T &c, BdfU LDSA &u, S &b, int nflag) {
   int convfail = ((nflag == FIRST_CALL) || (nflag == PREV_ERR_FAIL)) ? CV_NO_FAILURES : CV_FAIL_OTHER;
-  bool callSetup = (nflag == PREV_CONV_FAIL) || (nflag == PREV_ERR_FAIL) || (u.nst == 0) ||
-                   (u.nst >= u.nstlp + MSBP) || (fabs(u.gamrat - 1.0) > DGMAX);
+  bool callSetup = (nflag == PREV_CONV_FAIL) || (nflag == PREV_ERR_FAIL) || (uni(u.nst) == 0) ||
+                   (uni(u.nst) >= uni(u.nstlp) + MSBP) || (fabs(u.gamrat - 1.0) > DGMAX);
   for (;;) {
     b.ftemp = f_eval(c, u, b.zn[0]);
     if (c.lead()) u.nfe++;
     c.sync();
     if (callSetup) {
       int ier = c.setup(u, b, convfail);
-      const int nst = u.nst;
+      const int nst = uni(u.nst);
       const double gamma = u.gamma;
       c.sync();
       if (c.lead()) {
@@ -321,7 +321,7 @@ __device__ __forceinline__ int nls(T &c, BdfU LDSA &u, S &b, int nflag) {
 
 template <class S>
 __device__ __forceinline__ void adjust_order(BdfU LDSA &u, S &b, int deltaq) {
-  const int q = u.q;
+  const int q = uni(u.q);
   double l[QMAX + 1];
   if ((q == 2) && (deltaq != 1)) return;
   b.hist_in();
@@ -349,7 +349,7 @@ __device__ __forceinline__ void adjust_order(BdfU LDSA &u, S &b, int deltaq) {
     double A1 = (-alpha0 - alpha1) / prod;
     // zn[L] = A1 * zn[QMAX] (saved acor); zn[j] += l[j] * zn[L], j = 2..q
     double zl = A1 * b.zn[QMAX];
-    const int L = u.L;
+    const int L = uni(u.L);
 #pragma unroll
     for (int k = 0; k <= QMAX; k++)
       if (k == L) b.zn[k] = zl;
@@ -388,7 +388,7 @@ __device__ __forceinline__ int do_error_test(T &c, BdfU LDSA &u, S &b, int &nfla
   if (dsm <= 1.0) return 0;
   nef++;
   nflag = PREV_ERR_FAIL;
-  const int q = u.q, L = u.L;
+  const int q = uni(u.q), L = uni(u.L);
   restore(c, u, b, saved_t);
   if (c.lead()) u.netf++;
   if (nef == MXNEF) { c.sync(); return -3; }  // CV_ERR_FAILURE
@@ -431,17 +431,17 @@ __device__ __forceinline__ int do_error_test(T &c, BdfU LDSA &u, S &b, int &nfla
 
 template <class T, class S>
 __device__ __forceinline__ void complete_step(T &c, BdfU LDSA &u, S &b) {
-  const int q = u.q;
+  const int q = uni(u.q);
   const double h = u.h;
   b.hist_in();
 #pragma unroll
   for (int j = 0; j <= QMAX; j++)
     if (j <= q) b.zn[j] += u.l[j] * b.acor;
-  const int qwait = u.qwait - 1;
+  const int qwait = uni(u.qwait) - 1;
   if ((qwait == 1) && (q != QMAX)) b.zn[QMAX] = b.acor;
   b.hist_out();
   const double tq5 = u.tq[5];
-  const int nst = u.nst + 1;
+  const int nst = uni(u.nst) + 1;
   c.sync();
   if (c.lead()) {
     u.nst = nst;
@@ -470,10 +470,10 @@ __device__ __forceinline__ void set_eta_vals(double &eta, double &hprime, double
 
 template <class T, class S>
 __device__ __forceinline__ void prepare_next_step(T &c, BdfU LDSA &u, S &b, double dsm) {
-  const int q = u.q, L = u.L;
+  const int q = uni(u.q), L = uni(u.L);
   const double h = u.h, etamax = u.etamax;
   double eta, hprime;
-  int qprime = q, qwait = u.qwait;
+  int qprime = q, qwait = uni(u.qwait);
   if (etamax == 1.0) {
     qwait = max(qwait, 2);
     hprime = h;
@@ -530,8 +530,8 @@ __device__ __forceinline__ int bdf_step(T &c, BdfU LDSA &u, S &b) {
   const double saved_t = u.tn;
   double dsm = 0.0;
   int ncf = 0, nef = 0, nflag = FIRST_CALL;
-  if ((u.nst > 0) && (u.hprime != u.h)) {
-    const int qp = u.qprime, q = u.q;
+  if ((uni(u.nst) > 0) && (u.hprime != u.h)) {
+    const int qp = uni(u.qprime), q = uni(u.q);
     if (qp != q) {
       adjust_order(u, b, qp - q);
       c.sync();
@@ -562,7 +562,7 @@ __device__ __forceinline__ int bdf_step(T &c, BdfU LDSA &u, S &b) {
   complete_step(c, u, b);
   prepare_next_step(c, u, b, dsm);
   b.acor *= u.tq[2];
-  const int nst = u.nst;
+  const int nst = uni(u.nst);
   c.sync();
   if (c.lead()) u.etamax = (nst <= SMALL_NST) ? ETAMX2 : ETAMX3;
   c.sync();
@@ -622,7 +622,7 @@ __device__ __forceinline__ int hin(T &c, BdfU LDSA &u, S &b, double tout) {
 template <class S>
 __device__ __forceinline__ double get_dky(const BdfU LDSA &u, S &b, double t) {
   const double s = (t - u.tn) / u.h;
-  const int q = u.q;
+  const int q = uni(u.q);
   b.hist_in();
   double v = selz(b.zn, q);
 #pragma unroll
@@ -714,7 +714,7 @@ __device__ __forceinline__ int integrate(T &c, BdfU LDSA &u, S &b, const PointAr
     // register pressure)
     int nstloc = 0, why = 0;  // why: 1 = tout reached, < 0 = failure flag
     while (!done) {
-      if (u.nst > 0) b.ewt = 1.0 / (c.rtol * fabs(b.zn[0]) + c.atol);
+      if (uni(u.nst) > 0) b.ewt = 1.0 / (c.rtol * fabs(b.zn[0]) + c.atol);
       if (nstloc >= a.mxstep) { why = -1; break; }
       double nrm = wrms(c, b.zn[0], b.ewt);
       if (UROUND * nrm > 1.0) { why = -2; break; }
