@@ -268,7 +268,9 @@ int pdrchem_cloud_iterate(pdrchem_cloud *cloud, double t_end_years, double conv_
  * most pdrchem_server_max_batch points; the loop's precision applies and
  * the precision argument of pdrchem_cloud_iterate is ignored); NULL goes back
  * to direct launches on the cloud's stream.  The server must outlive its use
- * here. */
+ * here, and while attached it serves this cloud only: an iteration waits for
+ * the loop to drain and clears its completion list (per-point statuses come
+ * back through the requests' device arrays). */
 int pdrchem_cloud_set_server(pdrchem_cloud *cloud, pdrchem_server *server);
 #define PDRCHEM_CLOUD_ABUNDANCE 0    /* [npts*nspec] */
 #define PDRCHEM_CLOUD_AV 1           /* [npts*nrays] */
