@@ -71,6 +71,25 @@ def test_calculate_abundances_slab_vs_oracle(slab_case, precision):
     assert max_rel(ab[:, -1], ref[:, -1]) <= TOL
 
 
+def test_c1_full_slab(orc, net):
+    """The oracle configuration in full: the 1,000-point slab (n_H 1e3, T 10-300 K,
+    Av 0-10, G0 10, zeta 5e-17) through the drop-in entry point, every point
+    against the FP64 oracle, both precisions."""
+    P = workloads.slab_1d(1000)
+    r, x0 = _inputs(orc, net, P)
+    print("case: slab_1d 1000 pts, oracle on 16 threads", flush=True)
+    st, ref, _ = orc.calculate_abundances(x0.copy(), r, P.n_H, P.T, mode=oracle.MODE_GPU, nthreads=16)
+    assert st == 0
+    lib.set_globals(rtol=1e-8, atol=1e-30, start_time=0.0, end_time=1e7, nelect=33)
+    for precision in (lib.MIXED, lib.FP64):
+        ab = x0.copy()
+        assert lib.calculate_abundances(ab, np.ascontiguousarray(r), P.n_H.copy(), P.T.copy(),
+                                        precision=precision) == 0
+        err = max_rel(ab[:, :-1], ref[:, :-1])
+        print(f"  precision={precision}: max rel err vs oracle over 1000 points {err:.3e}", flush=True)
+        assert err <= TOL
+
+
 def test_vs_radau_truth(orc, net):
     path = os.path.join(GOLD, "truth_reduced.npz")
     if not os.path.exists(path):
