@@ -318,16 +318,15 @@ __device__ __forceinline__ bool gj_invert(const Lane &ln, WaveLDS<MT> LDSA &S, M
     const MT fi = (ln.hh == hk) ? colk : oc;
     used = used || isp;
     invp = isp ? k : invp;
+    // row update a_j <- a_j - f_i (p_j / piv), pivot row p_j / piv, column
+    // k <- -f_i / piv (pivot: 1 / piv): with g the row's multiplier (-f_i/piv,
+    // or 1/piv on the pivot row, which is also the new column-k entry) every
+    // entry is one FMA on g and the pivot row
+    const MT g = isp ? d : -fi * d;
 #pragma unroll
     for (int j = 0; j < 16; j++) {
-      const MT pj = pr[j] * d;
-      const MT nv = isp ? pj : fma(-fi, pj, A[j]);
-      if (j == jk) {
-        const MT kv = isp ? d : -fi * d;
-        A[j] = (ln.hh == hk) ? kv : nv;
-      } else {
-        A[j] = nv;
-      }
+      const MT nv = fma(g, pr[j], isp ? (MT)0 : A[j]);
+      A[j] = (j == jk && ln.hh == hk) ? g : nv;
     }
   }
   return true;
