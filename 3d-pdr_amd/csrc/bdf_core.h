@@ -75,15 +75,33 @@ __device__ __forceinline__ double selz(const double (&a)[NA], int k) {
 // binary exponent is split off exactly (x = m 2^e, e = kn + r), leaving a
 // root of m 2^r in [2^(-n), 2^n) for the hardware log2 / exp2; the library
 // pow(double) costs about 30x the instructions.
+// e / n (C truncation) for the orders' small n: one uniform switch over
+// constant divisors instead of a general integer division
+__device__ __forceinline__ int div_small(int e, int n) {
+  switch (n) {
+    case 1: return e;
+    case 2: return e / 2;
+    case 3: return e / 3;
+    case 4: return e / 4;
+    case 5: return e / 5;
+    case 6: return e / 6;
+    default: return e / n;
+  }
+}
+
 __device__ __forceinline__ double root_n(double x, int n) {
   if (x == 0.0) return 0.0;
   if (!(x > 0.0) || isinf(x)) return (x < 0.0 && n > 1) ? __builtin_nan("") : x;  // as pow
   int e;
   const double m = frexp(x, &e);
-  const int k = e / n, r = e - k * n;
+  const int k = div_small(e, n), r = e - k * n;
   const float f = ((float)r + __builtin_amdgcn_logf((float)m)) * __builtin_amdgcn_rcpf((float)n);
   return ldexp((double)__builtin_amdgcn_exp2f(f), k);
 }
+
+// 1/n for the orders' small n, correctly rounded (the values a division
+// gives), from a table indexed by the uniform n
+__constant__ double kRecip[8] = {0.0, 1.0, 1.0 / 2, 1.0 / 3, 1.0 / 4, 1.0 / 5, 1.0 / 6, 1.0 / 7};
 
 // x^n for a small integer n >= 0
 __device__ __forceinline__ double ipow(double x, int n) {
@@ -190,7 +208,7 @@ __device__ __forceinline__ void set_coeffs(T &c, BdfU LDSA &u) {
           if (i <= j) l[i] += l[i - 1] * xi_inv;
       }
     }
-    alpha0 -= 1.0 / q;
+    alpha0 -= kRecip[q];
     xistar_inv = -l[1] - alpha0;
     hsum += selz(tau, q - 1);
     xi_inv = h / hsum;
@@ -207,7 +225,7 @@ __device__ __forceinline__ void set_coeffs(T &c, BdfU LDSA &u) {
   if (uni(u.qwait) == 1) {
     if (q > 1) {
       double C = xistar_inv / lq;
-      double A3 = alpha0 + 1.0 / q;
+      double A3 = alpha0 + kRecip[q];
       double A4 = alpha0_hat + xi_inv;
       double Cpinv = (1.0 - A4 + A3) / A3;
       tq[1] = fabs(C * Cpinv);
@@ -216,7 +234,7 @@ __device__ __forceinline__ void set_coeffs(T &c, BdfU LDSA &u) {
     }
     hsum += selz(tau, q);
     xi_inv = h / hsum;
-    double A5 = alpha0 - 1.0 / (q + 1);
+    double A5 = alpha0 - kRecip[q + 1];
     double A6 = alpha0_hat - xi_inv;
     double Cppinv = (1.0 - A6 + A5) / A2;
     tq[3] = fabs(Cppinv / (xi_inv * (q + 2) * A5));
