@@ -134,8 +134,9 @@ def main():
     if a.npts is None:
         a.npts = 1_000_000 if a.network == "reduced" else 20_000
     if a.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
-              f"(grid points are independent; see DESIGN.md for how the path shards)", file=sys.stderr)
+        print(f"bench.py: runs on one GPU; --gpus {a.gpus} is not supported "
+              f"(grid points are independent: a multi-GPU run would give each GPU its own range of points)",
+              file=sys.stderr)
         sys.exit(2)
 
     import torch
