@@ -458,6 +458,10 @@ __global__ void fill_kernel(double *x, long long n, double v) {
     x[i] = v;
 }
 __global__ void zero_u32_kernel(unsigned *x) { *x = 0u; }
+__global__ void fill_u8_kernel(unsigned char *x, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = 0;
+}
 __global__ void zero_u64_kernel(unsigned long long *x, int n) {
   if ((int)threadIdx.x < n) x[threadIdx.x] = 0ull;
 }
@@ -537,6 +541,8 @@ struct pdrchem_cloud {
   // compact buffers of the active points
   double *c_T = nullptr, *c_Td = nullptr, *c_G = nullptr, *c_Av = nullptr, *c_cols = nullptr, *c_rho = nullptr;
   double *c_ab = nullptr, *c_rate = nullptr;
+  double *d_zeta = nullptr, *c_zeta = nullptr;  // per-point cosmic-ray rate (optional)
+  double *d_zeta_store = nullptr;
   int *c_status = nullptr, *c_stats = nullptr, *d_neg = nullptr;
   unsigned long long *d_ctr = nullptr;
   unsigned *d_counter = nullptr, *d_nconv = nullptr, *d_nfail = nullptr;
@@ -901,6 +907,9 @@ int pdrchem_cloud_iterate(pdrchem_cloud *c, double t_end_years, double conv_tol,
                        c->c_cols, c->d_cols, c->d_idx, nact, ncol);
     hipLaunchKernelGGL(rows_kernel<false>, dim3(blocks_for((long long)nact * c->nspec, 256)), dim3(256), 0, s,
                        c->c_ab, c->d_ab, c->d_idx, nact, c->nspec);
+    if (c->d_zeta)
+      hipLaunchKernelGGL(rows_kernel<false>, dim3(blocks_for(nact, 256)), dim3(256), 0, s, c->c_zeta, c->d_zeta,
+                         c->d_idx, nact, 1);
     hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(1), 0, s, (unsigned *)c->d_neg);
     hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(1), 0, s, c->d_counter);
     hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(1), 0, s, c->d_nconv);
@@ -910,8 +919,8 @@ int pdrchem_cloud_iterate(pdrchem_cloud *c, double t_end_years, double conv_tol,
       c->fail("gather");
       return -1;
     }
-    if (pdrchem_rates_device(c->net, &c->phys, c->c_T, c->c_Td, nr, c->c_G, c->c_Av, c->c_cols, nact, c->c_rate,
-                             c->d_neg, s) != 0) {
+    if (pdrchem_rates_device_zeta(c->net, &c->phys, c->c_T, c->c_Td, nr, c->c_G, c->c_Av, c->c_cols,
+                                  c->d_zeta ? c->c_zeta : nullptr, nact, c->c_rate, c->d_neg, s) != 0) {
       c->err = std::string("rates: ") + pdrchem_last_error();
       return -1;
     }
@@ -1020,6 +1029,33 @@ int pdrchem_cloud_get(pdrchem_cloud *c, int field, double *out) {
     return -1;
   }
   return 0;
+}
+
+int pdrchem_cloud_set_zeta(pdrchem_cloud *c, const double *zeta) {
+  if (!c) return -1;
+  c->err.clear();
+  t_err.clear();
+  if (!zeta) {
+    c->d_zeta = nullptr;  // back to phys.zeta (the buffers stay allocated)
+    return 0;
+  }
+  if (!c->c_zeta) {
+    double *dz = nullptr, *cz = nullptr;
+    if (!c->alloc(&dz, c->npts, "zeta") || !c->alloc(&cz, c->npts, "zeta")) {
+      c->fail("allocation");
+      return -1;
+    }
+    c->c_zeta = cz;
+    c->d_zeta_store = dz;
+  }
+  if (!ok(hipMemcpyAsync(c->d_zeta_store, zeta, (size_t)c->npts * 8, hipMemcpyHostToDevice, c->stream), "copy") ||
+      !ok(hipStreamSynchronize(c->stream), "sync")) {
+    c->fail("set zeta");
+    return -1;
+  }
+  c->d_zeta = c->d_zeta_store;
+  hipLaunchKernelGGL(fill_u8_kernel, dim3(blocks_for(c->npts, 256)), dim3(256), 0, c->stream, c->d_conv, c->npts);
+  return ok(hipGetLastError(), "flags") ? 0 : -1;
 }
 
 int pdrchem_cloud_set_server(pdrchem_cloud *c, pdrchem_server *server) {

@@ -105,6 +105,20 @@ class Cloud:
         if lib.load().pdrchem_cloud_set_server(self._h, server.h if server is not None else None) != 0:
             raise lib.PdrChemError(self._err())
 
+    def set_zeta(self, zeta: Optional[np.ndarray]) -> None:
+        """Per-point cosmic-ray ionisation rate (units of 1.3e-17 s^-1, like
+        Phys.zeta) for later iterations; None: the global Phys.zeta."""
+        L = lib.load()
+        if zeta is None:
+            rc = L.pdrchem_cloud_set_zeta(self._h, None)
+        else:
+            z = np.ascontiguousarray(zeta, dtype=np.float64).reshape(-1)
+            if z.shape[0] != self.npts:
+                raise lib.PdrChemError("zeta must have one entry per point")
+            rc = L.pdrchem_cloud_set_zeta(self._h, z.ctypes.data_as(C.POINTER(C.c_double)))
+        if rc != 0:
+            raise lib.PdrChemError(self._err())
+
     def set_abundances(self, x: np.ndarray) -> None:
         x = np.ascontiguousarray(x, dtype=np.float64)
         if x.shape != (self.npts, self.nspec):

@@ -27,7 +27,7 @@ EXPORTED_FUNCTIONS = [
     "pdrchem_server_create", "pdrchem_server_submit", "pdrchem_server_progress", "pdrchem_server_poll",
     "pdrchem_server_last_error", "pdrchem_server_destroy", "pdrchem_server_graph_stats",
     "pdrchem_server_submit_device", "pdrchem_server_max_batch", "pdrchem_cloud_set_server",
-    "pdrchem_cloud_create_points", "pdrchem_trace_points",
+    "pdrchem_cloud_create_points", "pdrchem_trace_points", "pdrchem_cloud_set_zeta",
     "pdrchem_cloud_create", "pdrchem_cloud_set_abundances", "pdrchem_cloud_iterate", "pdrchem_cloud_get",
     "pdrchem_cloud_num_evaluation_points", "pdrchem_cloud_evaluation_points", "pdrchem_cloud_last_error",
     "pdrchem_cloud_destroy", "pdrchem_autotune", "pdrchem_tuned_config",
@@ -108,6 +108,7 @@ def load(path: Optional[str] = None):
     L.pdrchem_server_submit_device.argtypes = [vp, C.c_int, C.c_int, vp, vp, vp, C.c_double, vp, vp]
     L.pdrchem_server_max_batch.argtypes = [vp]
     L.pdrchem_cloud_set_server.argtypes = [vp, vp]
+    L.pdrchem_cloud_set_zeta.argtypes = [vp, dp]
     L.pdrchem_cloud_create_points.argtypes = [C.c_int, C.c_int, dp, ip, C.c_int, dp, C.c_int, C.c_double, C.c_int,
                                               dp, C.c_double, C.POINTER(Phys), ip, C.c_int]
     L.pdrchem_cloud_create_points.restype = vp

@@ -264,6 +264,11 @@ int pdrchem_cloud_set_abundances(pdrchem_cloud *cloud, const double *abundance);
  * converged points (3DPDR.F90:628, columns :1822), which it re-solves and discards. */
 int pdrchem_cloud_iterate(pdrchem_cloud *cloud, double t_end_years, double conv_tol, int precision,
                           pdrchem_cloud_stats *stats);
+/* Cosmic-ray ionisation rate per point (host [npts], units of 1.3e-17 s^-1
+ * like phys->zeta; NULL = phys->zeta everywhere) for the rates of later
+ * iterations (pdrchem_rates_device_zeta; the per-point zeta of the reference's
+ * cosmic-ray fork).  Marks every point unconverged. */
+int pdrchem_cloud_set_zeta(pdrchem_cloud *cloud, const double *zeta);
 /* Send the iterations' solves through a request loop (device requests of at
  * most pdrchem_server_max_batch points; the loop's precision applies and
  * the precision argument of pdrchem_cloud_iterate is ignored); NULL goes back

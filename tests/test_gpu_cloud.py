@@ -190,6 +190,50 @@ def test_point_cloud_matches_oracle(order):
         c.close()
 
 
+def test_per_point_zeta_and_pdr_fin(tmp_path):
+    """Per-point cosmic-ray rates in the iteration (the cosmic-ray fork's
+    per-point zeta) against the oracle evaluated with each point's zeta, and
+    the state written as <output>.pdr.fin reads back."""
+    from pdrchem import outputs
+    shape = (4, 4, 3)
+    print(f"cloud per-point zeta {shape}", flush=True)
+    net = load_network("reduced")
+    n = int(np.prod(shape))
+    rho = smooth_density(*shape)
+    c = CL.Cloud(net, shape, H, rho, gext=(100.0, 0.0, 0.0))
+    x0 = workloads.initial_abundances(net, n)
+    c.set_abundances(x0)
+    z0 = lib.default_phys().zeta
+    zeta = z0 * np.array([1.0, 3.0, 10.0])[np.arange(n) % 3]
+    c.set_zeta(zeta)
+    st = c.iterate(t_end=1e7, conv_tol=1e-3, precision=lib.MIXED)
+    assert st["solved"] == n and st["failed"] == 0
+    got = c.get(CL.ABUNDANCE)
+    xyz, e, p, ep = oracle_chains(shape)
+    slots = CL.shield_slots(net)
+    xs = np.where(slots[None, :] >= 0, x0[:, np.maximum(slots, 0)], 0.0)
+    Av, cols = O.ray_columns(e, p, ep, rho, xsel=xs)
+    G, uv, T = O.uvfield(Av, gext=(100.0, 0.0, 0.0))
+    orc = O.Oracle("reduced")
+    rate = np.zeros((n, orc.nreac))
+    for zf in np.unique(zeta):
+        m = zeta == zf
+        rate[m] = orc.rates(T[m], 20.0, G[m], Av[m], cols[m], phys=O.default_phys(zeta=float(zf)))
+    s_, ref, _ = orc.calculate_abundances(x0.copy(), rate, rho, T, mode=O.MODE_GPU)
+    assert s_ == 0
+    mk = ref[:, :-1] > 1e-20
+    err = np.max(np.abs(got[:, :-1][mk] - ref[:, :-1][mk]) / ref[:, :-1][mk])
+    print(f"  max rel err vs oracle {err:.2e}", flush=True)
+    assert err <= 1e-4
+    path = tmp_path / "cloud.pdr.fin"
+    outputs.write_cloud_pdr_fin(str(path), c, xyz)
+    back = np.loadtxt(str(path))
+    assert back.shape == (n, 9 + net.nspec + 12)
+    assert np.allclose(back[:, 9:9 + net.nspec], got, rtol=1e-3, atol=1e-99)
+    assert np.allclose(back[:, 1:4], xyz, rtol=1e-3, atol=0)
+    c.close()
+
+
 def test_rejects_bad_arguments():
     net = load_network("reduced")
     with pytest.raises(lib.PdrChemError):
