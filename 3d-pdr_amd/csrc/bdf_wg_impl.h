@@ -488,10 +488,16 @@ __device__ __forceinline__ int wg_setup(Team<MT> &c, BdfU LDSA &u, BdfWG<MT> &b,
                     [&](uint32_t w) { return Pair<MT>{LU[w & 0x3fff], LU[(w >> 14) & 0x3fff]}; },
                     [&](uint32_t w, Pair<MT> g) {
                       const int kind = w >> 28, tgt = w & 0x3fff;
-                      if (kind == 1) {
-                        LU[tgt] = g.a / g.b;            // multiplier
-                      } else if (kind == 2) {
-                        LU[tgt] = (MT)1.0 / g.b;        // diagonal of the head's U^-1
+                      // FP32 form: the hardware reciprocal (1 ulp) in place of
+                      // an exactly rounded division; the factors only
+                      // precondition the FP64 Newton iteration
+                      if constexpr (sizeof(MT) == 4) {
+                        const MT r = __builtin_amdgcn_rcpf(g.b);
+                        if (kind == 1) LU[tgt] = g.a * r;       // multiplier
+                        else if (kind == 2) LU[tgt] = r;        // diagonal of the head's U^-1
+                      } else {
+                        if (kind == 1) LU[tgt] = g.a / g.b;
+                        else if (kind == 2) LU[tgt] = (MT)1.0 / g.b;
                       }
                     });
     __syncthreads();
