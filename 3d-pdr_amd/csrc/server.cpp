@@ -457,6 +457,13 @@ int pdrchem_server_submit_device(pdrchem_server *S, int net, int npts, double *d
 
 int pdrchem_server_max_batch(pdrchem_server *S) { return S ? (int)S->max_batch_pts : -1; }
 
+int pdrchem_server_set_max_batch(pdrchem_server *S, int npts) {
+  if (!S || npts < 1) return -1;
+  std::lock_guard<std::mutex> lock(S->mu);
+  S->max_batch_pts = (size_t)npts;
+  return 0;
+}
+
 int pdrchem_server_progress(pdrchem_server *S, int wait) {
   if (!S) return -1;
   std::lock_guard<std::mutex> lock(S->mu);

@@ -26,7 +26,8 @@ EXPORTED_FUNCTIONS = [
     "pdrchem_rates_device", "pdrchem_rates_device_zeta", "pdrchem_kernel_info", "pdrchem_set_max_blocks", "pdrchem_last_error",
     "pdrchem_server_create", "pdrchem_server_submit", "pdrchem_server_progress", "pdrchem_server_poll",
     "pdrchem_server_last_error", "pdrchem_server_destroy", "pdrchem_server_graph_stats",
-    "pdrchem_server_submit_device", "pdrchem_server_max_batch", "pdrchem_cloud_set_server",
+    "pdrchem_server_submit_device", "pdrchem_server_max_batch", "pdrchem_server_set_max_batch",
+    "pdrchem_cloud_set_server",
     "pdrchem_cloud_create_points", "pdrchem_trace_points", "pdrchem_cloud_set_zeta",
     "pdrchem_cloud_create", "pdrchem_cloud_set_abundances", "pdrchem_cloud_iterate", "pdrchem_cloud_get",
     "pdrchem_cloud_num_evaluation_points", "pdrchem_cloud_evaluation_points", "pdrchem_cloud_last_error",
@@ -107,6 +108,7 @@ def load(path: Optional[str] = None):
     L.pdrchem_server_graph_stats.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
     L.pdrchem_server_submit_device.argtypes = [vp, C.c_int, C.c_int, vp, vp, vp, C.c_double, vp, vp]
     L.pdrchem_server_max_batch.argtypes = [vp]
+    L.pdrchem_server_set_max_batch.argtypes = [vp, C.c_int]
     L.pdrchem_cloud_set_server.argtypes = [vp, vp]
     L.pdrchem_cloud_set_zeta.argtypes = [vp, dp]
     L.pdrchem_cloud_create_points.argtypes = [C.c_int, C.c_int, dp, ip, C.c_int, dp, C.c_int, C.c_double, C.c_int,
@@ -378,6 +380,10 @@ class Server:
         for t, _, _ in out:
             self.live.pop(t, None)
         return out
+
+    def set_max_batch(self, npts: int) -> None:
+        if self.L.pdrchem_server_set_max_batch(self.h, npts) != 0:
+            raise PdrChemError("bad batch size")
 
     def graph_stats(self):
         """(uses_graphs, graph launches, graph instantiations)."""

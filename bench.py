@@ -119,6 +119,8 @@ def main():
     ap.add_argument("--serve-duration", type=float, default=30.0, help="seconds of arrivals")
     ap.add_argument("--serve-networks", default="reduced,full",
                     help="networks of the requests, chosen with equal odds")
+    ap.add_argument("--serve-batch", type=int, default=65536,
+                    help="largest batch of the request loop (points)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="launch with the default shape instead of the autotuner's choice")
     ap.add_argument("--cloud", action="store_true",
@@ -303,6 +305,7 @@ def serve(a):
                                  torch.tensor(d["nH"][:n], device=dev), precision=prec)
                 tuned[f"{name}:{n}"] = (r["systems_per_block"], r["blocks_per_cu"])
     srv = lib.Server(device_bytes=4 << 30, nslots=4, precision=prec)
+    srv.set_max_batch(a.serve_batch)
     lat, status = {}, {}
     t0 = time.perf_counter()
     i = 0
@@ -334,7 +337,8 @@ def serve(a):
         "config": {"workload": f"serving: Poisson {a.serve_rate} req/s for {a.serve_duration} s, 1-4096 points "
                                f"per request (log-uniform), networks {'/'.join(nets)} with equal odds, "
                                f"t_end logU[1e3,1e7] yr",
-                   "requests": len(R), "points": npts, "precision": a.precision, "slots": 4},
+                   "requests": len(R), "points": npts, "precision": a.precision, "slots": 4,
+                   "max_batch": a.serve_batch},
         "failed_requests": int(sum(1 for v in status.values() if v != 0)),
         "launch_shapes": tuned or None,
     }

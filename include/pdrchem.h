@@ -179,8 +179,11 @@ int pdrchem_server_submit(pdrchem_server *server, int net, int npts, double *abu
 int pdrchem_server_submit_device(pdrchem_server *server, int net, int npts, double *d_abundance,
                                  const double *d_rate, const double *d_density, double t_end_years, int *d_status,
                                  int *d_stats);
-/* Largest number of points of one request (and of one batch). */
+/* Largest number of points of one request (and of one batch); default 65536.
+ * Smaller batches trade throughput for latency (a batch's requests complete
+ * together). */
 int pdrchem_server_max_batch(pdrchem_server *server);
+int pdrchem_server_set_max_batch(pdrchem_server *server, int npts);
 /* Retire finished batches (results copied into the callers' arrays) and
  * start pending ones; wait != 0 blocks until nothing is pending or in
  * flight.  Returns the number of requests pending or in flight, -1 on error. */
