@@ -20,7 +20,12 @@
  *                    restated from its published algorithm (Hindmarsh et al.
  *                    2005, ACM TOMS 31:363; CVODE user guide, "Mathematical
  *                    considerations"); the library itself is absent from the
- *                    reference tree (.MISSING_LARGE_BLOBS:1)
+ *                    reference tree (.MISSING_LARGE_BLOBS:1), so its parity is
+ *                    unpinned by the reference: no reference output or test
+ *                    exercises the integrator.  It is pinned instead by closed
+ *                    forms (linear decay), Robertson's problem, Radau (scipy,
+ *                    rtol 1e-12) solutions of slab points and element/charge
+ *                    conservation (tests/test_oracle.py)
  *   pdr_calculate_abundances   calculate_abundances_ calculate_abundances.c:64-354
  *   pdr_eval_points, pdr_ray_columns, pdr_uvfield, pdr_hp_*
  *                    evaluation points, Av and column densities along the
