@@ -404,3 +404,66 @@ def ref_eval_points(xyz, level=0, theta_crit=1.3, maxpoints=600, etype=None, wor
                 proj[p, j, i] = int(f[0])
                 epoint[p, j, i] = [float(v) for v in f[1:4]]
     return vec, epray, proj, epoint
+
+
+# --------------------------------------------------------------------------
+# heating terms (heating.c) and the reference's CALC_HEATING (oracle/_ref)
+# --------------------------------------------------------------------------
+
+HEAT_SPECIES = ("NH", "NH2", "NC", "NCO", "NHEx", "NH3x", "NH3Ox", "NHCOx", "NH2x", "NELECT")
+HEAT_REACTIONS = ("NRGR", "NRH2", "NRCI")
+HEAT_NET = {"reduced": 0, "full": 1}
+
+
+def heating_indices(net) -> np.ndarray:
+    """0-based slots the heating reads, from READ_SPECIES's labels (1-based)."""
+    return np.array([net.labels[k] - 1 for k in HEAT_SPECIES] + [net.nr_indices[k] - 1 for k in HEAT_REACTIONS],
+                    dtype=np.int32)
+
+
+def heating(name, net, n, T, Td, uv, ab, rate, phys: Optional[Phys] = None) -> np.ndarray:
+    L = lib()
+    if not getattr(L, "_heat_ready", False):
+        L.pdr_heating.argtypes = [C.POINTER(Phys), C.c_int, C.POINTER(C.c_int), C.c_double, C.c_double, C.c_double,
+                                  C.c_double, _dp, _dp, _dp]
+        L._heat_ready = True
+    ph = phys or default_phys()
+    idx = heating_indices(net)
+    ab = np.ascontiguousarray(ab, dtype=np.float64).reshape(len(n), -1)
+    rate = np.ascontiguousarray(rate, dtype=np.float64).reshape(len(n), -1)
+    out = np.zeros((len(n), 12))
+    for p in range(len(n)):
+        h = np.zeros(12)
+        L.pdr_heating(C.byref(ph), HEAT_NET[name], idx.ctypes.data_as(C.POINTER(C.c_int)), float(n[p]), float(T[p]),
+                      float(Td[p]), float(uv[p]), ab[p], rate[p], h)
+        out[p] = h
+    return out
+
+
+def have_ref_heating(name: str) -> bool:
+    return os.path.exists(os.path.join(REF_BIN_DIR, f"ref_heating_{name}"))
+
+
+def ref_heating(name, n, T, Td, uv, ab, rate, phys: Optional[Phys] = None, ref_dir: str = "/root/reference"):
+    """Run the amdflang-built reference CALC_HEATING on each point."""
+    ph = phys or default_phys()
+    o = Oracle(name)
+    lines = [f"{o.nspec} {o.nreac}",
+             " ".join(repr(float(v)) for v in (ph.zeta, ph.v_turb, ph.omega, ph.grain_radius, ph.metallicity,
+                                               ph.g2d)),
+             f"{len(n)}"]
+    for p in range(len(n)):
+        lines.append(" ".join(repr(float(v)) for v in (n[p], T[p], Td[p], uv[p])))
+        lines.append(" ".join(repr(float(v)) for v in ab[p]))
+        lines.append(" ".join(repr(float(v)) for v in rate[p]))
+    r = subprocess.run([os.path.join(REF_BIN_DIR, f"ref_heating_{name}")], input="\n".join(lines) + "\n",
+                       capture_output=True, text=True, cwd=ref_dir, timeout=600)
+    if r.returncode != 0:
+        raise RuntimeError(f"reference heating driver failed: {r.stderr[-2000:]}")
+    vals = []
+    for line in r.stdout.splitlines():
+        try:
+            vals.append(float(line.strip().replace("D", "E")))
+        except ValueError:
+            continue
+    return np.array(vals).reshape(len(n), 12)

@@ -165,6 +165,10 @@ int pdr_integrate_generic(int n, pdr_rhs_fn f, pdr_jac_fn jac, void *ud,
                           const double *tout, double *yout,
                           const pdr_cvode_opts *opts, pdr_cvode_stats *st);
 
+/* heating terms of a point (heating.c): CALC_HEATING, sub_calculate_heating.F90 */
+void pdr_heating(const pdr_phys *ph, int net, const int *idx, double n, double T, double Td, double uv,
+                 const double *ab, const double *rate, double *heat);
+
 /* ray tracer (raytrace.c) */
 #define PDR_ORDER_HEAPSORT 0  /* the reference's heapsort order (ties as it leaves them) */
 #define PDR_ORDER_STABLE 1    /* strict (distance, point index) order */
