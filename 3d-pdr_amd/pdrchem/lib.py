@@ -32,7 +32,7 @@ EXPORTED_FUNCTIONS = [
     "pdrchem_cloud_create", "pdrchem_cloud_set_abundances", "pdrchem_cloud_iterate", "pdrchem_cloud_get",
     "pdrchem_cloud_num_evaluation_points", "pdrchem_cloud_evaluation_points", "pdrchem_cloud_last_error",
     "pdrchem_cloud_destroy", "pdrchem_autotune", "pdrchem_tuned_config",
-    "pdrchem_set_tuned_config", "pdrchem_clear_tuning",
+    "pdrchem_set_tuned_config", "pdrchem_clear_tuning", "pdrchem_heating_device",
 ]
 EXPORTED_GLOBALS = [
     ("chemistry_module_mp_relative_abundance_tolerance_", C.c_double),
@@ -91,6 +91,8 @@ def load(path: Optional[str] = None):
     L.pdrchem_rates_device_zeta.argtypes = [C.c_int, C.POINTER(Phys), vp, vp, C.c_int, vp, vp, vp, vp, C.c_int,
                                             vp, vp, vp]
     L.pdrchem_kernel_info.argtypes = [C.c_int, C.c_int, ip, ip, ip, ip]
+    L.pdrchem_heating_device.argtypes = [C.c_int, C.POINTER(Phys), ip, C.c_int, vp, vp, vp, vp, vp, vp, vp,
+                                         vp]
     L.pdrchem_set_max_blocks.argtypes = [C.c_int]
     L.pdrchem_set_max_blocks.restype = None
     L.pdrchem_set_precision.argtypes = [C.c_int]
@@ -295,6 +297,34 @@ def trace_points(xyz, level=0, theta_crit=1.3, order=ORDER_DISTANCE, etype=None)
 def set_tuned_config(net: int, npts: int, systems_per_block: int, blocks_per_cu: int, precision=MIXED):
     if load().pdrchem_set_tuned_config(net, precision, npts, systems_per_block, blocks_per_cu) != 0:
         raise PdrChemError("bad launch shape")
+
+
+def heating_device(net: int, index, density, T, Td, uv, abundance, rate, phys: Optional[Phys] = None, out=None,
+                   stream=None):
+    """Device heating terms (CALC_HEATING): index = the 13 1-based slots NH,
+    NH2, NC, NCO, NHEx, NH3x, NH3Ox, NHCOx, NH2x, NELECT, NRGR, NRH2, NRCI;
+    density, T, Td, uv [n]; abundance [n, nspec]; rate [n, nreac] (float64
+    CUDA tensors).  Returns heat [n, 12]."""
+    import torch
+    L = load()
+    args = [density, T, Td, uv, abundance, rate]
+    for t in args:
+        if not (t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()):
+            raise PdrChemError("heating_device takes contiguous float64 device tensors")
+    npts = density.shape[0]
+    dims = {0: (33, 329), 1: (215, 2926)}.get(net)
+    if dims is None or any(t.numel() != npts for t in args[1:4]) or abundance.numel() != npts * dims[0] \
+            or rate.numel() != npts * dims[1]:
+        raise PdrChemError("heating_device: unsupported network or shapes that do not match it")
+    if out is None:
+        out = torch.empty((npts, 12), dtype=torch.float64, device=density.device)
+    idx = (C.c_int * 13)(*[int(i) for i in index])
+    ph = phys or default_phys()
+    s = stream if stream is not None else torch.cuda.current_stream(density.device).cuda_stream
+    rc = L.pdrchem_heating_device(net, C.byref(ph), idx, npts, *[t.data_ptr() for t in args], out.data_ptr(), s)
+    if rc != 0:
+        raise PdrChemError("pdrchem_heating_device failed (network, index or arguments)")
+    return out
 
 
 def rates_device(net: int, T, Td, G, Av, cols=None, phys: Optional[Phys] = None, out=None, neg=None,

@@ -110,6 +110,22 @@ int pdrchem_rates_device_zeta(int net, const pdrchem_phys *phys, const double *d
                               int nrays, const double *d_G, const double *d_Av, const double *d_cols,
                               const double *d_zeta, int npts, double *d_rate, int *d_neg, void *stream);
 
+/* Batched CALC_HEATING (sub_calculate_heating.F90:6-386) on device memory,
+ * reduced and full networks (PDRCHEM_NET ids 0 and 1; others return -1).
+ * index[13]: 1-based slots as the Fortran module variables hold them -- NH,
+ * NH2, NC, NCO, NHEx, NH3x, NH3Ox, NHCOx, NH2x, NELECT, then reactions NRGR,
+ * NRH2, NRCI.  Per point: density, T, Td, uv [npts] (uv in Draine units, as
+ * UVfield); abundances [npts*nspec] and rates [npts*nreac] row-major.  Writes
+ * heat [npts*12]: photoelectric, PAH photoelectric, Weingartner & Draine, C
+ * ionization, H2 formation, H2 photodissociation, FUV pumping, cosmic rays,
+ * turbulence, chemical, gas-grain, total -- the order of the reference's
+ * .heat.fin columns (3DPDR.F90 output), the total leaving out the
+ * photoelectric and Weingartner & Draine terms as the reference's does.
+ * Asynchronous on stream; 0 on success, -1 on a bad argument or index. */
+int pdrchem_heating_device(int net, const pdrchem_phys *phys, const int *index, int npts,
+                           const double *d_density, const double *d_T, const double *d_Td, const double *d_uv,
+                           const double *d_abundance, const double *d_rate, double *d_heat, void *stream);
+
 /* Kernel resource usage of the solver for a network and precision
  * (registers per lane, LDS bytes per block, scratch bytes per lane, blocks
  * per CU).  0 = ok. */
