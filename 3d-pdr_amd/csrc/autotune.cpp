@@ -91,12 +91,14 @@ int pdrchem_autotune(int net, int precision, double *d_abundance, const double *
   int rc = -1;
   double best = -1.0;
   LaunchConfig best_cfg{0, 0};
+  bool backed_up = false;
   do {
     if (hipMalloc(&backup, abytes) != hipSuccess || hipMalloc(&status, (size_t)npts * sizeof(int)) != hipSuccess ||
         hipMalloc(&counter, sizeof(unsigned int)) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
         hipEventCreate(&e1) != hipSuccess)
       break;
     if (hipMemcpyAsync(backup, d_abundance, abytes, hipMemcpyDeviceToDevice, s) != hipSuccess) break;
+    backed_up = true;
     SolveRequest r{};
     r.abundance = d_abundance;
     r.rate = d_rate;
@@ -109,7 +111,8 @@ int pdrchem_autotune(int net, int precision, double *d_abundance, const double *
     r.tend = t_end * 3.1556926e7;  // calculate_abundances.c:90
     r.tout0 = 1.0e-4 * 3.1556926e7;
     r.nelect0 = nelect - 1;
-    r.mxstep = 10000000LL;
+    r.mxstep = solver_options().mxstep;  // calculate_abundances.c:111 (1e7 unless set)
+    r.xe_policy = solver_options().xe_policy;
     r.status = status;
     r.counter = counter;
     r.mixed = mixed;
@@ -149,6 +152,14 @@ int pdrchem_autotune(int net, int precision, double *d_abundance, const double *
     }
     rc = 0;
   } while (false);
+  // on a failure after the backup was taken, the caller's abundances may be
+  // partly integrated: put them back before the backup goes away
+  if (rc != 0 && backed_up) {
+    (void)hipMemcpyAsync(d_abundance, backup, abytes, hipMemcpyDeviceToDevice, s);
+    (void)hipStreamSynchronize(s);
+  } else if (rc != 0) {
+    (void)hipStreamSynchronize(s);
+  }
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
   if (counter) (void)hipFree(counter);

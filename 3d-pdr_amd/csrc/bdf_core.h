@@ -37,6 +37,11 @@ constexpr double UROUND = 2.220446049250313e-16;
 constexpr double ETAMX1 = 10000.0, ETAMX2 = 10.0, ETAMX3 = 10.0, ETAMXF = 0.2, ETAMIN = 0.1;
 constexpr double ETACF = 0.25, ADDON = 0.000001, BIAS1 = 6.0, BIAS2 = 6.0, BIAS3 = 10.0;
 constexpr double THRESH = 1.5, CRDOWN = 0.3, DGMAX = 0.3, RDIV = 2.0;
+// DGMAX_J: a setup after a failed Newton iteration evaluates a new Jacobian
+// unless gamma moved by DGMAX_J or more since the last setup; the test is
+// written !(dgamma >= DGMAX_J) so that a non-finite gamma also forces one
+// (the Newton failure then counts as a convergence failure, bounded by
+// MXNCF, instead of retrying the setup with the stale Jacobian forever)
 constexpr double DGMAX_J = 0.2, NLSCOEF = 0.1, HLB_FACTOR = 100.0, HUB_FACTOR = 0.1;
 constexpr double H_BIAS = 0.5;
 constexpr int SMALL_NST = 10, MXNCF = 10, MXNEF = 7, MXNEF1 = 3, SMALL_NEF = 2, LONG_WAIT = 10;
@@ -661,6 +666,7 @@ struct PointArgs {
   const double *tend_pts;  // [nsys] per-point end time, or null: tend for all
   int nelect0;
   long long mxstep;
+  int xe_policy;           // 0: x_e of the last RHS evaluation; 1: charge sum of the returned abundances
   int *status;             // [nsys]
   int *stats;              // [nsys][8] or null
   unsigned int *counter;   // work queue

@@ -31,6 +31,7 @@ static void fill_args(Args &a, const SolveRequest &r) {
   a.tout0 = r.tout0;
   a.nelect0 = r.nelect0;
   a.mxstep = r.mxstep;
+  a.xe_policy = r.xe_policy;
   a.status = r.status;
   a.stats = r.stats;
   a.counter = r.counter;
@@ -38,7 +39,7 @@ static void fill_args(Args &a, const SolveRequest &r) {
 
 template <typename MT, int NW>
 static int launch_persistent(const bdfw_reduced::PointArgs &a, int npts, hipStream_t s, int max_blocks,
-                             int cap_per_cu) {
+                             int cap_per_cu, void *wsp, size_t ws_bytes) {
   auto kern = bdfw_reduced::bdf_wave_kernel<MT, NW>;
   int dev = 0, ncu = 0, per_cu = 0;
   if (hipGetDevice(&dev) != hipSuccess) return -1;
@@ -50,8 +51,13 @@ static int launch_persistent(const bdfw_reduced::PointArgs &a, int npts, hipStre
   long long full = (long long)ncu * per_cu;
   long long nb = want < full ? want : full;
   if (max_blocks > 0 && nb > max_blocks) nb = max_blocks;
-  if (nb < 1) nb = 1;
-  auto *ws = (bdfw_reduced::WaveWS<MT> *)stream_workspace(s, (size_t)nb * NW * sizeof(bdfw_reduced::WaveWS<MT>));
+  constexpr size_t per_block = NW * sizeof(bdfw_reduced::WaveWS<MT>);
+  if (wsp && nb > (long long)(ws_bytes / per_block)) nb = (long long)(ws_bytes / per_block);
+  if (nb < 1) {
+    if (wsp) return -1;  // not even one block's workspace fits the caller's buffer
+    nb = 1;
+  }
+  auto *ws = (bdfw_reduced::WaveWS<MT> *)(wsp ? wsp : stream_workspace(s, (size_t)nb * per_block));
   if (!ws) return -1;
   hipLaunchKernelGGL(kern, dim3((unsigned)nb), dim3(64 * NW), 0, s, a, ws);
   return hipGetLastError() == hipSuccess ? 0 : -1;
@@ -59,11 +65,11 @@ static int launch_persistent(const bdfw_reduced::PointArgs &a, int npts, hipStre
 
 template <typename MT>
 static int launch_shape(const bdfw_reduced::PointArgs &a, int npts, hipStream_t s, int max_blocks,
-                        LaunchConfig cfg) {
+                        LaunchConfig cfg, void *wsp, size_t ws_bytes) {
   switch (cfg.systems_per_block) {
-    case 1: return launch_persistent<MT, 1>(a, npts, s, max_blocks, cfg.blocks_per_cu);
-    case 2: return launch_persistent<MT, 2>(a, npts, s, max_blocks, cfg.blocks_per_cu);
-    default: return launch_persistent<MT, 4>(a, npts, s, max_blocks, cfg.blocks_per_cu);
+    case 1: return launch_persistent<MT, 1>(a, npts, s, max_blocks, cfg.blocks_per_cu, wsp, ws_bytes);
+    case 2: return launch_persistent<MT, 2>(a, npts, s, max_blocks, cfg.blocks_per_cu, wsp, ws_bytes);
+    default: return launch_persistent<MT, 4>(a, npts, s, max_blocks, cfg.blocks_per_cu, wsp, ws_bytes);
   }
 }
 
@@ -74,12 +80,29 @@ int launch_wave_solver(int net, const SolveRequest &r, hipStream_t s) {
       fill_args(a, r);
       LaunchConfig cfg = r.cfg;
       if (cfg.systems_per_block == 0 && cfg.blocks_per_cu == 0) cfg = tuned_config(net, r.mixed, r.npts);
-      if (r.mixed) return launch_shape<float>(a, r.npts, s, r.max_blocks, cfg);
-      return launch_shape<double>(a, r.npts, s, r.max_blocks, cfg);
+      if (r.mixed) return launch_shape<float>(a, r.npts, s, r.max_blocks, cfg, r.workspace, r.workspace_bytes);
+      return launch_shape<double>(a, r.npts, s, r.max_blocks, cfg, r.workspace, r.workspace_bytes);
     }
     default:
       return -2;
   }
+}
+
+int launch_wave_jacobian(int net, const JacRequest &q, hipStream_t s) {
+  if (net != NET_REDUCED) return -2;
+  if (q.mixed) {
+    auto *ws = (bdfw_reduced::WaveWS<float> *)stream_workspace(s, (size_t)q.npts * sizeof(bdfw_reduced::WaveWS<float>));
+    if (!ws) return -1;
+    hipLaunchKernelGGL(bdfw_reduced::wave_jac_kernel<float>, dim3((unsigned)q.npts), dim3(64), 0, s, q.x, q.rate,
+                       q.density, q.nspec, q.nreac, q.out, ws);
+  } else {
+    auto *ws =
+        (bdfw_reduced::WaveWS<double> *)stream_workspace(s, (size_t)q.npts * sizeof(bdfw_reduced::WaveWS<double>));
+    if (!ws) return -1;
+    hipLaunchKernelGGL(bdfw_reduced::wave_jac_kernel<double>, dim3((unsigned)q.npts), dim3(64), 0, s, q.x, q.rate,
+                       q.density, q.nspec, q.nreac, q.out, ws);
+  }
+  return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 // the largest whole-device workspace over the block shapes

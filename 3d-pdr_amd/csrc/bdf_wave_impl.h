@@ -218,11 +218,14 @@ __device__ __forceinline__ double wave_rhs(Ctx<MT> &c, double y, double &xe_out)
 }
 
 // ---------------------------------------------------------------------
-// exact Jacobian at the factor vector currently in S.X: row i, columns
-// hh*16.., including the x_e chain term g_i * charge_j
+// exact Jacobian at the factor vector currently in S.X.  jac_partials
+// writes the flux partials, jac_accumulate runs the generator's term lists:
+// acc[j] = d ydot_i / d x_{hh*16+j} at fixed x_e (j < 16) and acc[16] =
+// d ydot_i / d x_e, for row i of this lane; jacobian folds in the x_e chain
+// term g_i * charge_j and saves the lane's 16 columns.
 // ---------------------------------------------------------------------
 template <typename MT>
-__device__ __forceinline__ void jacobian(Ctx<MT> &c, MT (&jv)[16]) {
+__device__ __forceinline__ void jac_partials(Ctx<MT> &c) {
   const Lane &ln = c.ln;
   WaveLDS<MT> LDSA &S = *c.S;
   MT LDSA *D = (MT LDSA *)S.F;  // partials in the matrix type
@@ -234,13 +237,17 @@ __device__ __forceinline__ void jacobian(Ctx<MT> &c, MT (&jv)[16]) {
     D[2 * s] = (MT)(kp * S.X[(f >> 8) & 0xff]);   // d flux / d X[a]
     D[2 * s + 1] = (MT)(kp * S.X[f & 0xff]);      // d flux / d X[b]
   }
-  wsync();
+}
+
+template <typename MT>
+__device__ __forceinline__ void jac_accumulate(Ctx<MT> &c, double (&acc)[17]) {
+  const Lane &ln = c.ln;
+  const MT LDSA *D = (const MT LDSA *)c.S->F;
   // the term table lives in global memory (L2): its rounds are fetched in
   // groups of G ahead of use so that a lane waits for the table once per
   // group, not once per round; the column of each round is a compile-time
   // constant after unrolling
   constexpr int G = 8;
-  double acc[17];
 #pragma unroll
   for (int j = 0; j < 17; j++) acc[j] = 0.0;
   const uint32_t *jt = NET::jac_term + ln.L;
@@ -262,6 +269,16 @@ __device__ __forceinline__ void jacobian(Ctx<MT> &c, MT (&jv)[16]) {
       }
     }
   }
+}
+
+template <typename MT>
+__device__ __forceinline__ void jacobian(Ctx<MT> &c, MT (&jv)[16]) {
+  const Lane &ln = c.ln;
+  WaveLDS<MT> LDSA &S = *c.S;
+  jac_partials(c);
+  wsync();
+  double acc[17];
+  jac_accumulate(c, acc);
 #pragma unroll
   for (int j = 0; j < 16; j++) {
     jv[j] = (MT)fma(acc[16], (double)NET::charge[ln.hh * 16 + j], acc[j]);
@@ -360,7 +377,7 @@ __device__ __forceinline__ int lsetup(Ctx<MT> &c, BdfU LDSA &u, Bdf<MT> &b, int 
   for (int j = 0; j < 16; j++) b.A[j] = c.W->J[j * 64 + ln.L];
   double dgamma = fabs(u.gamma / u.gammap - 1.0);
   bool jbad = (u.nst == 0) || (u.nst > u.nstlj + MSBJ) ||
-              ((convfail == CV_FAIL_BAD_J) && (dgamma < DGMAX_J)) || (convfail == CV_FAIL_OTHER);
+              ((convfail == CV_FAIL_BAD_J) && !(dgamma >= DGMAX_J)) || (convfail == CV_FAIL_OTHER);
   if (jbad) {
     const int nst = u.nst;
     if (ln.L == 0) {
@@ -417,6 +434,22 @@ __device__ __forceinline__ double Ctx<MT>::solve(const BdfU LDSA &u, const Bdf<M
   return lsolve(*this, u, b, r);
 }
 
+// rates of one point -> the lanes' flux constants (rate, times n_H for
+// two-body slots)
+template <typename MT>
+__device__ __forceinline__ void load_kp(Ctx<MT> &c, const double *rate, double nH) {
+#pragma unroll
+  for (int r = 0; r < NET::ROUNDS; r++) {
+    const int s = r * 64 + c.ln.L;
+    const int re = NET::slot_reac[s];
+    const uint32_t f = c.tb.fab[s];
+    const double k = (re >= 0) ? rate[re] : 0.0;
+    const double kps = ((f >> 16) & 1) ? k * nH : k;
+    if constexpr (kKpInLDS<MT>) c.S->Kp[s] = kps;
+    else c.W->Kp[s] = kps;
+  }
+}
+
 // ---------------------------------------------------------------------
 // one grid point: calculate_abundances.c:132-326 semantics
 // ---------------------------------------------------------------------
@@ -425,19 +458,7 @@ __device__ __forceinline__ void solve_point(Ctx<MT> &c, const PointArgs &a, int 
   Bdf<MT> b;
   const Lane &ln = c.ln;
   BdfU LDSA &u = c.S->u;
-  // rates -> per-lane flux constants
-  const double *rate = a.rate + (size_t)sys * a.rate_stride;
-  const double nH = a.density[sys];
-#pragma unroll
-  for (int r = 0; r < NET::ROUNDS; r++) {
-    int s = r * 64 + ln.L;
-    int re = NET::slot_reac[s];
-    const uint32_t f = c.tb.fab[s];
-    const double k = (re >= 0) ? rate[re] : 0.0;
-    const double kps = ((f >> 16) & 1) ? k * nH : k;
-    if constexpr (kKpInLDS<MT>) c.S->Kp[s] = kps;
-    else c.W->Kp[s] = kps;
-  }
+  load_kp(c, a.rate + (size_t)sys * a.rate_stride, a.density[sys]);
   double *ab = a.abundance + (size_t)sys * a.ab_stride;
   const double y0 = ln.row_ok ? ab[ln.i] : 0.0;
   const double xe0 = ab[a.nelect0];
@@ -464,15 +485,16 @@ __device__ __forceinline__ void solve_point(Ctx<MT> &c, const PointArgs &a, int 
   const double tend = a.tend_pts ? a.tend_pts[sys] : a.tend;
   const int status = integrate(c, u, b, a, tend, yout, flag);
 
-  // write-back with clip (calculate_abundances.c:308-317)
-  if (ln.owner) {
-    double v = yout;
-    if (v < c.atol) v = 0.0;
-    ab[ln.i] = v;
-  }
+  // write-back with clip (calculate_abundances.c:308-317); the electron
+  // slot holds x_e of the last RHS evaluation (:317) or, with xe_policy 1,
+  // the charge sum of the abundances written back
+  double v = yout;
+  if (v < c.atol) v = 0.0;
+  if (ln.owner) ab[ln.i] = v;
+  const double xe_fin = a.xe_policy ? wave_sum(ln.owner ? c.tb.q * v : 0.0) : 0.0;
   wsync();
   if (ln.L == 0) {
-    ab[a.nelect0] = u.xe_last;
+    ab[a.nelect0] = a.xe_policy ? xe_fin : u.xe_last;
     a.status[sys] = status;
     if (a.stats) {
       int *st = a.stats + (size_t)sys * 8;
@@ -481,6 +503,25 @@ __device__ __forceinline__ void solve_point(Ctx<MT> &c, const PointArgs &a, int 
     }
   }
   wsync();
+}
+
+// a wavefront's constant state: lane roles, table pointers, LDS record
+template <typename MT, int NW>
+__device__ __forceinline__ void init_ctx(Ctx<MT> &c, BlockLDS<MT, NW> &blk, WaveWS<MT> *ws, double rtol, double atol) {
+  c.ln.L = lane_id();
+  c.ln.i = c.ln.L & 31;
+  c.ln.hh = c.ln.L >> 5;
+  c.ln.row_ok = c.ln.i < NET::NEQ;
+  c.ln.owner = c.ln.row_ok && c.ln.hh == 0;
+  c.S = (WaveLDS<MT> LDSA *)&blk.w[threadIdx.x >> 6];
+  c.W = ws + (size_t)blockIdx.x * NW + (threadIdx.x >> 6);
+  c.rtol = rtol;
+  c.atol = atol;
+  c.tb.rbase = NET::row_base[c.ln.L];
+  c.tb.rnl = NET::row_nlanes[c.ln.L];
+  c.tb.q = (double)NET::charge[c.ln.i];
+  c.tb.rterm = (const uint32_t LDSA *)blk.rterm;
+  c.tb.fab = (const uint32_t LDSA *)blk.fab;
 }
 
 // persistent kernel: every wavefront pulls grid points from a queue
@@ -492,25 +533,52 @@ bdf_wave_kernel(PointArgs a, WaveWS<MT> *ws) {
   for (int k = threadIdx.x; k < NET::ROUNDS * 64; k += blockDim.x) blk.fab[k] = NET::slot_fab[k];
   __syncthreads();
   Ctx<MT> c;
-  c.ln.L = lane_id();
-  c.ln.i = c.ln.L & 31;
-  c.ln.hh = c.ln.L >> 5;
-  c.ln.row_ok = c.ln.i < NET::NEQ;
-  c.ln.owner = c.ln.row_ok && c.ln.hh == 0;
-  c.S = (WaveLDS<MT> LDSA *)&blk.w[threadIdx.x >> 6];
-  c.W = ws + (size_t)blockIdx.x * NW + (threadIdx.x >> 6);
-  c.rtol = a.rtol;
-  c.atol = a.atol;
-  c.tb.rbase = NET::row_base[c.ln.L];
-  c.tb.rnl = NET::row_nlanes[c.ln.L];
-  c.tb.q = (double)NET::charge[c.ln.i];
-  c.tb.rterm = (const uint32_t LDSA *)blk.rterm;
-  c.tb.fab = (const uint32_t LDSA *)blk.fab;
+  init_ctx<MT, NW>(c, blk, ws, a.rtol, a.atol);
   for (;;) {
     unsigned int sys = 0;
     if (c.ln.L == 0) sys = atomicAdd(a.counter, 1u);
     sys = (unsigned int)uni((int)sys);
     if (sys >= (unsigned int)a.nsys) break;
     solve_point(c, a, (int)sys);
+  }
+}
+
+// The solver's Jacobian at given abundances, one point per block (an
+// introspection kernel, not on the solve path): rows of the bordered form,
+// out[(sys*NEQ + i)*(NEQ+1) + j] = d ydot_i / d x_j at fixed x_e for j < NEQ
+// and d ydot_i / d x_e for j = NEQ, evaluated by the same partials and term
+// lists as the solver's Jacobian (rounded to MT where the solver rounds).
+// x_e is the charge sum of x, as in the solver's RHS.
+template <typename MT>
+__global__ void __launch_bounds__(64) wave_jac_kernel(const double *x, const double *rate, const double *dens,
+                                                      int ab_stride, int rate_stride, double *out, WaveWS<MT> *ws) {
+  __shared__ BlockLDS<MT, 1> blk;
+  for (int k = threadIdx.x; k < NET::TM * 64; k += blockDim.x) blk.rterm[k] = NET::rhs_term[k];
+  for (int k = threadIdx.x; k < NET::ROUNDS * 64; k += blockDim.x) blk.fab[k] = NET::slot_fab[k];
+  __syncthreads();
+  Ctx<MT> c;
+  init_ctx<MT, 1>(c, blk, ws, 0.0, 0.0);
+  const int sys = blockIdx.x;
+  load_kp(c, rate + (size_t)sys * rate_stride, dens[sys]);
+  const double y = c.ln.row_ok ? x[(size_t)sys * ab_stride + c.ln.i] : 0.0;
+  wsync();
+  c.S->X[c.ln.L] = (c.ln.L == NET::NEQ + 1) ? 1.0 : 0.0;
+  wsync();
+  if (c.ln.owner) c.S->X[c.ln.i] = y;
+  const double xe = wave_sum(c.ln.owner ? c.tb.q * y : 0.0);
+  if (c.ln.L == 0) c.S->X[NET::NEQ] = xe;
+  wsync();
+  jac_partials(c);
+  wsync();
+  double acc[17];
+  jac_accumulate(c, acc);
+  if (c.ln.row_ok) {
+    double *row = out + ((size_t)sys * NET::NEQ + c.ln.i) * (NET::NEQ + 1);
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+      const int col = c.ln.hh * 16 + j;
+      if (col < NET::NEQ) row[col] = (double)(MT)acc[j];
+    }
+    if (c.ln.hh == 0) row[NET::NEQ] = (double)(MT)acc[16];
   }
 }

@@ -74,6 +74,7 @@ void fill_args(Args &a, const SolveRequest &r) {
   a.tout0 = r.tout0;
   a.nelect0 = r.nelect0;
   a.mxstep = r.mxstep;
+  a.xe_policy = r.xe_policy;
   a.status = r.status;
   a.stats = r.stats;
   a.counter = r.counter;
@@ -94,8 +95,12 @@ int launch(int net, void (*kern)(Args, unsigned char *), const SolveRequest &r, 
   long long full = (long long)ncu * per_cu;
   long long nb = r.npts < full ? r.npts : full;
   if (r.max_blocks > 0 && nb > r.max_blocks) nb = r.max_blocks;
-  if (nb < 1) nb = 1;
-  auto *ws = (unsigned char *)stream_workspace(s, (size_t)nb * slot_bytes);
+  if (r.workspace && nb > (long long)(r.workspace_bytes / slot_bytes)) nb = (long long)(r.workspace_bytes / slot_bytes);
+  if (nb < 1) {
+    if (r.workspace) return -1;  // not even one workgroup's slot fits the caller's buffer
+    nb = 1;
+  }
+  auto *ws = (unsigned char *)(r.workspace ? r.workspace : stream_workspace(s, (size_t)nb * slot_bytes));
   if (!ws) return -1;
   hipLaunchKernelGGL(kern, dim3((unsigned)nb), dim3(256), 0, s, a, ws);
   return hipGetLastError() == hipSuccess ? 0 : -1;
@@ -125,7 +130,32 @@ int info_of(Kern fn, KernelInfo *info) {
   return 0;
 }
 
+template <typename MT>
+int launch_jac(void (*kern)(const double *, const double *, const double *, int, int, double *, unsigned char *),
+               size_t slot_bytes, const JacRequest &q, hipStream_t s) {
+  auto *ws = (unsigned char *)stream_workspace(s, (size_t)q.npts * slot_bytes);
+  if (!ws) return -1;
+  hipLaunchKernelGGL(kern, dim3((unsigned)q.npts), dim3(256), 0, s, q.x, q.rate, q.density, q.nspec, q.nreac, q.out,
+                     ws);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 }  // namespace
+
+int launch_wg_jacobian(int net, const JacRequest &q, hipStream_t s) {
+  switch (net) {
+    case NET_FULL:
+      return q.mixed ? launch_jac<float>(bdfg_full::wg_jac_kernel<float>, bdfg_full::Slot<float>::bytes, q, s)
+                     : launch_jac<double>(bdfg_full::wg_jac_kernel<double>, bdfg_full::Slot<double>::bytes, q, s);
+    case NET_MYNETWORK:
+      return q.mixed
+                 ? launch_jac<float>(bdfg_mynetwork::wg_jac_kernel<float>, bdfg_mynetwork::Slot<float>::bytes, q, s)
+                 : launch_jac<double>(bdfg_mynetwork::wg_jac_kernel<double>, bdfg_mynetwork::Slot<double>::bytes, q,
+                                      s);
+    default:
+      return -2;
+  }
+}
 
 int launch_wg_solver(int net, const SolveRequest &r, hipStream_t s) {
   switch (net) {

@@ -74,6 +74,7 @@ struct WgLDS {
   MT gjd[2];                 //   reciprocal of the pivot
   int gjp[2];                //   pivot row, -1 if singular
   double red[2][NWAVE];      // team reductions (double-buffered)
+  int badpiv;                // head elimination met a zero or non-finite pivot
   BdfU u;
   // Hv[NHV] | zero | Tl[NT*NT] over F[NSLOT+1];  D[2*NSLOT+1] over all of it
   alignas(16) unsigned char region[RegionBytes<MT>::total];
@@ -412,7 +413,7 @@ __device__ __forceinline__ int wg_setup(Team<MT> &c, BdfU LDSA &u, BdfWG<MT> &b,
   const double dgamma = fabs(gamma / u.gammap - 1.0);
   const int nst = u.nst;
   const bool jbad = (nst == 0) || (nst > u.nstlj + MSBJ) ||
-                    ((convfail == CV_FAIL_BAD_J) && (dgamma < DGMAX_J)) || (convfail == CV_FAIL_OTHER);
+                    ((convfail == CV_FAIL_BAD_J) && !(dgamma >= DGMAX_J)) || (convfail == CV_FAIL_OTHER);
   __syncthreads();
   if (c.tid == 0) {
     if (jbad) {
@@ -441,6 +442,7 @@ __device__ __forceinline__ int wg_setup(Team<MT> &c, BdfU LDSA &u, BdfWG<MT> &b,
     S.Winv[NET::NEQ] = 1.0 / we;
   }
   __syncthreads();
+  if (c.tid == 0) S.badpiv = 0;
   // M = I - gamma J, scaled, in LU storage order.  Position entry d: row,
   // column, source of J in the saved Jacobian; the bordered row holds the
   // charges.  The table entries and the J values or charges of a group of
@@ -482,7 +484,9 @@ __device__ __forceinline__ int wg_setup(Team<MT> &c, BdfU LDSA &u, BdfWG<MT> &b,
     if ((d & 0xff) != 0xff) LU[t0 * BLOCK + c.tid] = (MT)m_entry(d, m_value(d));
   }
   __syncthreads();
-  // head elimination, level by level
+  // head elimination, level by level.  The head has no pivoting: a zero or
+  // non-finite pivot makes the setup fail (CVODE's lsetup failure, as the
+  // tail's singular Gauss-Jordan step does) instead of passing inf/NaN on.
   for (int lv = 0; lv < NET::NLEV; lv++) {
     for_items_gc<4>(opaque(NET::elimA), NET::elimA_off[lv], NET::elimA_n[lv], c.tid,
                     [&](uint32_t w) { return Pair<MT>{LU[w & 0x3fff], LU[(w >> 14) & 0x3fff]}; },
@@ -490,14 +494,18 @@ __device__ __forceinline__ int wg_setup(Team<MT> &c, BdfU LDSA &u, BdfWG<MT> &b,
                       const int kind = w >> 28, tgt = w & 0x3fff;
                       // FP32 form: the hardware reciprocal (1 ulp) in place of
                       // an exactly rounded division; the factors only
-                      // precondition the FP64 Newton iteration
+                      // precondition the FP64 Newton iteration.  A pivot
+                      // whose reciprocal is not finite (zero, or denormal
+                      // and flushed) is rejected.
                       if constexpr (sizeof(MT) == 4) {
                         const MT r = __builtin_amdgcn_rcpf(g.b);
                         if (kind == 1) LU[tgt] = g.a * r;       // multiplier
                         else if (kind == 2) LU[tgt] = r;        // diagonal of the head's U^-1
+                        if ((kind == 1 || kind == 2) && !(isfinite(r) && isfinite(g.b))) S.badpiv = 1;
                       } else {
                         if (kind == 1) LU[tgt] = g.a / g.b;
                         else if (kind == 2) LU[tgt] = (MT)1.0 / g.b;
+                        if ((kind == 1 || kind == 2) && !(g.b != (MT)0 && isfinite(g.b))) S.badpiv = 1;
                       }
                     });
     __syncthreads();
@@ -513,6 +521,7 @@ __device__ __forceinline__ int wg_setup(Team<MT> &c, BdfU LDSA &u, BdfWG<MT> &b,
                     });
     __syncthreads();
   }
+  if (S.badpiv) return 1;  // uniform: read after the level loop's last barrier
   if constexpr (NET::NHEAD > 0) {
     // explicit inverses of the head triangles, level by level:
     // target <- -(sum of src1*src2) * LU[scale]
@@ -641,6 +650,18 @@ __device__ __forceinline__ double Team<MT>::solve(const BdfU LDSA &u, const BdfW
   return wg_solve(*this, u, b, r);
 }
 
+// rates of one point -> the lanes' flux constants (rate, times n_H for
+// two-body slots)
+template <typename MT>
+__device__ __forceinline__ void wg_load_kp(Team<MT> &c, const double *rate, double nH) {
+#pragma unroll
+  for (int r = 0; r < NET::FR; r++) {
+    const int re = NET::slot_reac[r * BLOCK + c.tid];
+    const double k = (re >= 0) ? rate[re] : 0.0;
+    c.Kp[r * BLOCK] = ((c.fab[r * BLOCK] >> 16) & 1) ? k * nH : k;
+  }
+}
+
 // ---------------------------------------------------------------------
 // one grid point: calculate_abundances.c:132-326 semantics
 // ---------------------------------------------------------------------
@@ -649,14 +670,7 @@ __device__ __forceinline__ void wg_solve_point(Team<MT> &c, const PointArgs &a, 
   BdfWG<MT> b;
   WgLDS<MT> LDSA &S = *c.S;
   BdfU LDSA &u = S.u;
-  const double *rate = a.rate + (size_t)sys * a.rate_stride;
-  const double nH = a.density[sys];
-#pragma unroll
-  for (int r = 0; r < NET::FR; r++) {
-    const int re = NET::slot_reac[r * BLOCK + c.tid];
-    const double k = (re >= 0) ? rate[re] : 0.0;
-    c.Kp[r * BLOCK] = ((c.fab[r * BLOCK] >> 16) & 1) ? k * nH : k;
-  }
+  wg_load_kp(c, a.rate + (size_t)sys * a.rate_stride, a.density[sys]);
   double *ab = a.abundance + (size_t)sys * a.ab_stride;
   const double y0 = c.own_ ? ab[c.tid] : 0.0;
   const double xe0 = ab[a.nelect0];
@@ -681,15 +695,16 @@ __device__ __forceinline__ void wg_solve_point(Team<MT> &c, const PointArgs &a, 
   const double tend = a.tend_pts ? a.tend_pts[sys] : a.tend;
   const int status = integrate(c, u, b, a, tend, yout, flag);
 
-  // write-back with clip (calculate_abundances.c:308-317)
-  if (c.own_) {
-    double v = yout;
-    if (v < c.atol) v = 0.0;
-    ab[c.tid] = v;
-  }
+  // write-back with clip (calculate_abundances.c:308-317); the electron
+  // slot holds x_e of the last RHS evaluation (:317) or, with xe_policy 1,
+  // the charge sum of the abundances written back
+  double v = yout;
+  if (v < c.atol) v = 0.0;
+  if (c.own_) ab[c.tid] = v;
+  const double xe_fin = a.xe_policy ? c.sum(c.own_ ? c.q * v : 0.0) : 0.0;
   __syncthreads();
   if (c.tid == 0) {
-    ab[a.nelect0] = u.xe_last;
+    ab[a.nelect0] = a.xe_policy ? xe_fin : u.xe_last;
     a.status[sys] = status;
     if (a.stats) {
       int *st = a.stats + (size_t)sys * 8;
@@ -698,6 +713,26 @@ __device__ __forceinline__ void wg_solve_point(Team<MT> &c, const PointArgs &a, 
     }
   }
   __syncthreads();
+}
+
+// a workgroup's constant state: thread roles, its device slot, tables
+template <typename MT>
+__device__ __forceinline__ void init_team(Team<MT> &c, WgLDS<MT> &lds, unsigned char *ws, double rtol, double atol) {
+  c.tid = threadIdx.x;
+  c.lane = c.tid & 63;
+  c.wave = c.tid >> 6;
+  c.own_ = c.tid < NET::NEQ;
+  c.q = (double)NET::charge[c.tid];
+  c.red_buf = 0;
+  c.S = (WgLDS<MT> LDSA *)&lds;
+  unsigned char *slot = ws + (size_t)blockIdx.x * Slot<MT>::bytes;
+  c.Jg = (MT *)slot;
+  c.Kp = (double *)(slot + Slot<MT>::kp_off) + c.tid;
+  c.rtol = rtol;
+  c.atol = atol;
+  c.fab = NET::slot_fab + c.tid;
+  c.rbase = (c.tid < NET::NEQ) ? NET::row_base[c.tid] : 0;
+  c.rnl = (c.tid < NET::NEQ) ? NET::row_nl[c.tid] : 0;
 }
 
 // persistent kernel: every workgroup pulls grid points from a queue; jac is
@@ -710,21 +745,7 @@ bdf_wg_kernel(PointArgs a, unsigned char *ws) {
   __shared__ WgLDS<MT> lds;
   __shared__ unsigned int next;
   Team<MT> c;
-  c.tid = threadIdx.x;
-  c.lane = c.tid & 63;
-  c.wave = c.tid >> 6;
-  c.own_ = c.tid < NET::NEQ;
-  c.q = (double)NET::charge[c.tid];
-  c.red_buf = 0;
-  c.S = (WgLDS<MT> LDSA *)&lds;
-  unsigned char *slot = ws + (size_t)blockIdx.x * Slot<MT>::bytes;
-  c.Jg = (MT *)slot;
-  c.Kp = (double *)(slot + Slot<MT>::kp_off) + c.tid;
-  c.rtol = a.rtol;
-  c.atol = a.atol;
-  c.fab = NET::slot_fab + c.tid;
-  c.rbase = (c.tid < NET::NEQ) ? NET::row_base[c.tid] : 0;
-  c.rnl = (c.tid < NET::NEQ) ? NET::row_nl[c.tid] : 0;
+  init_team(c, lds, ws, a.rtol, a.atol);
   for (;;) {
     if (c.tid == 0) next = atomicAdd(a.counter, 1u);
     __syncthreads();
@@ -732,5 +753,38 @@ bdf_wg_kernel(PointArgs a, unsigned char *ws) {
     __syncthreads();
     if (sys >= (unsigned int)a.nsys) break;
     wg_solve_point(c, a, (int)sys);
+  }
+}
+
+// The solver's Jacobian at given abundances, one point per workgroup (an
+// introspection kernel, not on the solve path): out[(sys*NEQ + i)*(NEQ+1) +
+// j] = d ydot_i / d x_j at fixed x_e for j < NEQ and d ydot_i / d x_e for
+// j = NEQ, read back from the solver's saved Jacobian (in MT) through the
+// iteration matrix's position table.  out must be zeroed beforehand (only
+// the network's structural entries are written).  x_e is the charge sum of
+// x, as in the solver's RHS.
+template <typename MT>
+__global__ void __launch_bounds__(BLOCK) wg_jac_kernel(const double *x, const double *rate, const double *dens,
+                                                       int ab_stride, int rate_stride, double *out,
+                                                       unsigned char *ws) {
+  __shared__ WgLDS<MT> lds;
+  Team<MT> c;
+  init_team(c, lds, ws, 0.0, 0.0);
+  WgLDS<MT> LDSA &S = *c.S;
+  const int sys = blockIdx.x;
+  wg_load_kp(c, rate + (size_t)sys * rate_stride, dens[sys]);
+  const double y = c.own_ ? x[(size_t)sys * ab_stride + c.tid] : 0.0;
+  if (c.tid <= NET::NB) S.X[c.tid] = (c.tid == NET::NEQ + 1) ? 1.0 : 0.0;
+  __syncthreads();
+  if (c.own_) S.X[c.tid] = y;
+  const double xe = c.sum(c.own_ ? c.q * y : 0.0);
+  if (c.tid == 0) S.X[NET::NEQ] = xe;
+  __syncthreads();
+  wg_jacobian(c);
+  double *o = out + (size_t)sys * NET::NEQ * (NET::NEQ + 1);
+  for (int t = 0; t < NET::MTM; t++) {
+    const uint32_t d = NET::mpos[t * BLOCK + c.tid];
+    const int row = d & 0xff, col = (d >> 8) & 0xff, src = d >> 16;
+    if (row < NET::NEQ && col <= NET::NEQ && src != 0xffff) o[(size_t)row * (NET::NEQ + 1) + col] = (double)c.Jg[src];
   }
 }

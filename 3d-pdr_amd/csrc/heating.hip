@@ -52,6 +52,7 @@ struct HeatArgs {
   int idx[NIDX];  // 0-based
   int npts;
   const double *n, *T, *Td, *uv, *ab, *rate;
+  const double *zeta_pts;  // per-point cosmic-ray ionisation rate (units of ph.zeta) or null
   double *heat;
   ChemTerm chem[13];  // the network's chemical-heating channels, passed by value
 };
@@ -106,7 +107,8 @@ __global__ __launch_bounds__(256) void heating_kernel(HeatArgs a) {
                          (1.6 * s[S_H] * exp(-((400.0 / T) * (400.0 / T))) +
                           1.4 * s[S_H2] * exp(-(18100.0 / (T + 1200.0))));
     const double h_fuv = (sp(2.2f) * kEv) * 9.0 * k_h2 * s[S_H2] * n / (1.0 + ncrit / n);
-    const double h_cr = (sp(9.4f) * kEv) * (1.3e-17 * a.ph.zeta) * n * s[S_H2];
+    const double zeta = a.zeta_pts ? a.zeta_pts[p] : a.ph.zeta;
+    const double h_cr = (sp(9.4f) * kEv) * (1.3e-17 * zeta) * n * s[S_H2];
     const double vt = a.ph.v_turb / 1.0e5;
     const double h_turb = 3.5e-28 * (vt * vt * vt) * (1.0 / 5.0) * n;
     double h_chem = 0.0;
@@ -149,9 +151,10 @@ int launch(Kern kern, const ChemTerm (&tab)[NCHEM], HeatArgs &a, hipStream_t s) 
 }  // namespace
 
 int launch_heating(int net, const PhysParams &ph, const int *index, int npts, const double *n, const double *T,
-                   const double *Td, const double *uv, const double *ab, const double *rate, double *heat,
-                   hipStream_t s) {
+                   const double *Td, const double *uv, const double *ab, const double *rate, const double *zeta,
+                   double *heat, hipStream_t s) {
   HeatArgs a{};
+  a.zeta_pts = zeta;
   a.ph = ph;
   int nspec = net == NET_REDUCED ? pdrnet_reduced::NSPEC : pdrwg_full::NSPEC;
   int nreac = net == NET_REDUCED ? pdrnet_reduced::NREAC : pdrwg_full::NREAC;
@@ -181,17 +184,25 @@ int launch_heating(int net, const PhysParams &ph, const int *index, int npts, co
 
 using namespace pdrchem;
 
-extern "C" int pdrchem_heating_device(int net, const pdrchem_phys *phys, const int *index, int npts,
-                                      const double *d_density, const double *d_T, const double *d_Td,
-                                      const double *d_uv, const double *d_abundance, const double *d_rate,
-                                      double *d_heat, void *stream) {
+extern "C" int pdrchem_heating_device_zeta(int net, const pdrchem_phys *phys, const int *index, int npts,
+                                           const double *d_density, const double *d_T, const double *d_Td,
+                                           const double *d_uv, const double *d_abundance, const double *d_rate,
+                                           const double *d_zeta, double *d_heat, void *stream) {
   if (npts == 0) return 0;
   if ((net != NET_REDUCED && net != NET_FULL) || !phys || !index || npts < 0 || !d_density || !d_T || !d_Td ||
       !d_uv || !d_abundance || !d_rate || !d_heat)
     return -1;
   const PhysParams ph{phys->zeta, phys->v_turb, phys->omega, phys->grain_radius, phys->metallicity, phys->g2d};
-  return launch_heating(net, ph, index, npts, d_density, d_T, d_Td, d_uv, d_abundance, d_rate, d_heat,
+  return launch_heating(net, ph, index, npts, d_density, d_T, d_Td, d_uv, d_abundance, d_rate, d_zeta, d_heat,
                         (hipStream_t)stream) == 0
              ? 0
              : -1;
+}
+
+extern "C" int pdrchem_heating_device(int net, const pdrchem_phys *phys, const int *index, int npts,
+                                      const double *d_density, const double *d_T, const double *d_Td,
+                                      const double *d_uv, const double *d_abundance, const double *d_rate,
+                                      double *d_heat, void *stream) {
+  return pdrchem_heating_device_zeta(net, phys, index, npts, d_density, d_T, d_Td, d_uv, d_abundance, d_rate,
+                                     nullptr, d_heat, stream);
 }

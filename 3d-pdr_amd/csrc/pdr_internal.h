@@ -29,14 +29,28 @@ struct SolveRequest {
   double t0, tend, tout0; // seconds
   const double *tend_pts; // [npts] per-point end time (seconds) or null
   int nelect0;            // 0-based electron slot
-  long long mxstep;
+  long long mxstep;       // steps per CVode call before flag -1 (calculate_abundances.c:111)
+  int xe_policy;          // electron slot on return: 0 = x_e of the last RHS evaluation, 1 = charge sum
   int *status;            // [npts]
   int *stats;             // [npts][8] or null
   unsigned int *counter;  // work-queue counter, zero on entry
   int max_blocks;         // 0 = fill the device
   int mixed;              // 0: FP64 iteration matrix; 1: FP32 matrix, FP64 Newton
   LaunchConfig cfg;       // {0, 0}: tuned or default launch shape
+  // solver workspace supplied by the caller (device memory of
+  // workspace_bytes); null: the stream's workspace (allocated on demand).
+  // With a caller's workspace the launch keeps to the resident blocks whose
+  // workspace fits in it and allocates nothing.
+  void *workspace;
+  size_t workspace_bytes;
 };
+
+// library-wide solver options (pdrchem_set_mxstep, pdrchem_set_electron_policy)
+struct SolverOptions {
+  long long mxstep;
+  int xe_policy;
+};
+SolverOptions solver_options();
 
 struct KernelInfo {
   int num_regs, shared_bytes, local_bytes, max_threads, blocks_per_cu;
@@ -57,6 +71,15 @@ int wg_solver_info(int net, int mixed, KernelInfo *info);
 // every later launch on the stream allocation-free (capturable)
 size_t wave_solver_workspace_bytes(int net, int mixed);
 size_t wg_solver_workspace_bytes(int net, int mixed);
+// the solver's Jacobian at given abundances (introspection; one point per
+// block): out[npts][nspec-1][nspec] in the bordered layout of the kernels
+struct JacRequest {
+  const double *x, *rate, *density;  // [npts][nspec], [npts][nreac], [npts]
+  int npts, nspec, nreac, mixed;
+  double *out;
+};
+int launch_wave_jacobian(int net, const JacRequest &q, hipStream_t s);
+int launch_wg_jacobian(int net, const JacRequest &q, hipStream_t s);
 // one unsigned int set to zero by a kernel (capturable, unlike a memset node)
 int launch_zero_u32(unsigned int *p, hipStream_t s);
 

@@ -37,6 +37,8 @@ constexpr double SECONDS_IN_YEAR = 3.1556926e7;  // calculate_abundances.c:90
 thread_local std::string g_err;
 int g_max_blocks = 0;
 int g_precision = -1;  // -1: not set, read PDRCHEM_PRECISION on first use
+long long g_mxstep = 10000000LL;  // calculate_abundances.c:111
+int g_xe_policy = PDRCHEM_XE_LAST_RHS;
 
 int host_precision() {
   if (g_precision < 0) {
@@ -110,24 +112,92 @@ struct HostPathBuffers {
 };
 HostPathBuffers g_buf;
 
-// cvode.log entries for failed points (calculate_abundances.c:108, 235-298)
+// cvode.log entries for failed points, in the reference's layout
+// (calculate_abundances.c:108, 235-298): the fifth-failure block for points
+// that ran out of retries on -1 or -4, the plain failure block for any other
+// flag.  The per-call retry lines and the time reached at the failure are not
+// reported by the device, so "After" gives the flag only.
 void log_failures(const std::vector<int> &st, double start_time, double end_time) {
   FILE *fh = nullptr;
   for (size_t n = 0; n < st.size(); n++) {
     if (st[n] == 0) continue;
     if (!fh) fh = std::fopen("cvode.log", "a");
     if (!fh) return;
-    std::fprintf(fh, "ERROR! Integration failed: particle = %zu, flag = %d, t = 0 .. %8.2le yr\n\n", n,
-                 st[n], end_time - start_time);
+    if (st[n] == -1 || st[n] == -4)
+      std::fprintf(fh, "ERROR! Fifth failure - aborting integration\n\n");
+    else
+      std::fprintf(fh, "ERROR! CVODE failed with the following parameters:\n\n");
+    std::fprintf(fh, "-----------------------------------------------\n");
+    std::fprintf(fh, "CVODE Parameters:\n\n");
+    std::fprintf(fh, "Particle %zu\n\n", n);
+    std::fprintf(fh, "Before:\n");
+    std::fprintf(fh, "flag   = %d\n", 0);
+    std::fprintf(fh, "t      = %8.2le yr\n", start_time);
+    std::fprintf(fh, "tout   = %8.2le yr\n", end_time);
+    std::fprintf(fh, "dt     = %8.2le yr\n\n", end_time - start_time);
+    std::fprintf(fh, "After:\n");
+    std::fprintf(fh, "flag   = %d\n", st[n]);
+    std::fprintf(fh, "-----------------------------------------------\n\n");
   }
   if (fh) std::fclose(fh);
 }
 
 }  // namespace
 
+namespace pdrchem {
+SolverOptions solver_options() { return SolverOptions{g_mxstep, g_xe_policy}; }
+}  // namespace pdrchem
+
 extern "C" {
 
 const char *pdrchem_last_error(void) { return g_err.c_str(); }
+
+int pdrchem_set_mxstep(long long mxstep) {
+  if (mxstep < 1) {
+    set_err("mxstep must be positive");
+    return -1;
+  }
+  g_mxstep = mxstep;
+  return 0;
+}
+
+long long pdrchem_mxstep(void) { return g_mxstep; }
+
+int pdrchem_set_electron_policy(int policy) {
+  if (policy != PDRCHEM_XE_LAST_RHS && policy != PDRCHEM_XE_CHARGE_SUM) {
+    set_err("unknown electron policy");
+    return -1;
+  }
+  g_xe_policy = policy;
+  return 0;
+}
+
+int pdrchem_electron_policy(void) { return g_xe_policy; }
+
+int pdrchem_jacobian_device(int net, int precision, const double *d_abundance, const double *d_rate,
+                            const double *d_density, int npts, double *d_jac, void *stream) {
+  if (net < 0 || net >= NET_COUNT || !kCompiled[net]) {
+    set_err("unsupported network id");
+    return -1;
+  }
+  if (npts <= 0) return 0;
+  if (!d_abundance || !d_rate || !d_density || !d_jac) {
+    set_err("null device pointer");
+    return -1;
+  }
+  const int neq = kNets[net].nspec - 1;
+  hipStream_t s = (hipStream_t)stream;
+  if (!hip_ok(hipMemsetAsync(d_jac, 0, (size_t)npts * neq * (neq + 1) * sizeof(double), s), "hipMemsetAsync"))
+    return -1;
+  JacRequest q{d_abundance, d_rate, d_density, npts, kNets[net].nspec, kNets[net].nreac,
+               precision == PDRCHEM_MIXED ? 1 : 0, d_jac};
+  const int rc = net == NET_REDUCED ? launch_wave_jacobian(net, q, s) : launch_wg_jacobian(net, q, s);
+  if (rc != 0) {
+    set_err("Jacobian kernel launch failed");
+    return -1;
+  }
+  return 0;
+}
 
 void pdrchem_set_max_blocks(int max_blocks) { g_max_blocks = max_blocks < 0 ? 0 : max_blocks; }
 
@@ -182,7 +252,9 @@ int pdrchem_solve_device(int net, double *d_abundance, const double *d_rate, con
   r.tend = t_end * SECONDS_IN_YEAR;
   r.tout0 = 1.0e-4 * SECONDS_IN_YEAR;  // calculate_abundances.c:214
   r.nelect0 = nelect - 1;
-  r.mxstep = 10000000LL;               // calculate_abundances.c:111
+  const SolverOptions opt = solver_options();
+  r.mxstep = opt.mxstep;               // calculate_abundances.c:111 (1e7 unless set)
+  r.xe_policy = opt.xe_policy;
   r.status = d_status;
   r.stats = d_stats;
   r.counter = d_counter;

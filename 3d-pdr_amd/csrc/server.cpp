@@ -218,7 +218,8 @@ bool launch_batch(pdrchem_server *S, Slot &sl) {
   q.tend_pts = (const double *)(dev + L.tend);
   q.tout0 = 1.0e-4 * kYear;  // calculate_abundances.c:214
   q.nelect0 = s.nspec - 1;   // electron last (read_species.F90:164)
-  q.mxstep = 10000000LL;     // calculate_abundances.c:111
+  q.mxstep = solver_options().mxstep;  // calculate_abundances.c:111 (1e7 unless set)
+  q.xe_policy = solver_options().xe_policy;
   q.status = (int *)(dev + L.status);
   q.stats = sl.device ? (int *)(dev + L.stats) : nullptr;
   q.counter = (unsigned int *)(dev + L.counter);
@@ -339,7 +340,9 @@ bool admit(pdrchem_server *S) {
     std::vector<Req> take;
     size_t n = 0;
     for (auto it = S->pending.begin(); it != S->pending.end();) {
-      if (it->net == net && it->device == device && n + it->npts <= S->max_batch_pts) {
+      // the oldest request always goes, even if the cap was lowered below its
+      // size after it was queued (it then forms a batch of its own)
+      if (it->net == net && it->device == device && (take.empty() || n + it->npts <= S->max_batch_pts)) {
         n += it->npts;
         take.push_back(*it);
         it = S->pending.erase(it);

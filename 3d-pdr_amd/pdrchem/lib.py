@@ -33,6 +33,8 @@ EXPORTED_FUNCTIONS = [
     "pdrchem_cloud_num_evaluation_points", "pdrchem_cloud_evaluation_points", "pdrchem_cloud_last_error",
     "pdrchem_cloud_destroy", "pdrchem_autotune", "pdrchem_tuned_config",
     "pdrchem_set_tuned_config", "pdrchem_clear_tuning", "pdrchem_heating_device",
+    "pdrchem_heating_device_zeta", "pdrchem_set_mxstep", "pdrchem_mxstep", "pdrchem_set_electron_policy",
+    "pdrchem_electron_policy", "pdrchem_jacobian_device",
 ]
 EXPORTED_GLOBALS = [
     ("chemistry_module_mp_relative_abundance_tolerance_", C.c_double),
@@ -93,6 +95,12 @@ def load(path: Optional[str] = None):
     L.pdrchem_kernel_info.argtypes = [C.c_int, C.c_int, ip, ip, ip, ip]
     L.pdrchem_heating_device.argtypes = [C.c_int, C.POINTER(Phys), ip, C.c_int, vp, vp, vp, vp, vp, vp, vp,
                                          vp]
+    L.pdrchem_heating_device_zeta.argtypes = [C.c_int, C.POINTER(Phys), ip, C.c_int, vp, vp, vp, vp, vp, vp,
+                                              vp, vp, vp]
+    L.pdrchem_set_mxstep.argtypes = [C.c_longlong]
+    L.pdrchem_mxstep.restype = C.c_longlong
+    L.pdrchem_set_electron_policy.argtypes = [C.c_int]
+    L.pdrchem_jacobian_device.argtypes = [C.c_int, C.c_int, vp, vp, vp, C.c_int, vp, vp]
     L.pdrchem_set_max_blocks.argtypes = [C.c_int]
     L.pdrchem_set_max_blocks.restype = None
     L.pdrchem_set_precision.argtypes = [C.c_int]
@@ -299,8 +307,48 @@ def set_tuned_config(net: int, npts: int, systems_per_block: int, blocks_per_cu:
         raise PdrChemError("bad launch shape")
 
 
+XE_LAST_RHS = 0     # PDRCHEM_XE_LAST_RHS: x_e of the last RHS evaluation (calculate_abundances.c:317)
+XE_CHARGE_SUM = 1   # PDRCHEM_XE_CHARGE_SUM: charge sum of the abundances written back
+
+
+def set_mxstep(mxstep: int) -> None:
+    if load().pdrchem_set_mxstep(int(mxstep)) != 0:
+        raise PdrChemError(last_error())
+
+
+def mxstep() -> int:
+    return int(load().pdrchem_mxstep())
+
+
+def set_electron_policy(policy: int) -> None:
+    if load().pdrchem_set_electron_policy(int(policy)) != 0:
+        raise PdrChemError(last_error())
+
+
+def electron_policy() -> int:
+    return int(load().pdrchem_electron_policy())
+
+
+def jacobian_device(net: int, abundance, rate, density, precision=FP64, stream=None):
+    """The solver's Jacobian at the given abundances (torch device tensors as
+    for solve_device).  Returns [n, neq, neq+1]: d ydot / d x at fixed x_e
+    and, in the last column, d ydot / d x_e."""
+    import torch
+    L = load()
+    n, nspec = abundance.shape
+    for t in (abundance, rate, density):
+        if t.dtype != torch.float64 or not t.is_contiguous() or not t.is_cuda:
+            raise PdrChemError("device tensors must be contiguous float64")
+    out = torch.empty((n, nspec - 1, nspec), dtype=torch.float64, device=abundance.device)
+    s = stream if stream is not None else torch.cuda.current_stream(abundance.device)
+    if L.pdrchem_jacobian_device(net, precision, _ptr(abundance), _ptr(rate), _ptr(density), n, _ptr(out),
+                                 C.c_void_p(s.cuda_stream)) != 0:
+        raise PdrChemError(last_error())
+    return out
+
+
 def heating_device(net: int, index, density, T, Td, uv, abundance, rate, phys: Optional[Phys] = None, out=None,
-                   stream=None):
+                   stream=None, zeta=None):
     """Device heating terms (CALC_HEATING): index = the 13 1-based slots NH,
     NH2, NC, NCO, NHEx, NH3x, NH3Ox, NHCOx, NH2x, NELECT, NRGR, NRH2, NRCI;
     density, T, Td, uv [n]; abundance [n, nspec]; rate [n, nreac] (float64
@@ -321,7 +369,10 @@ def heating_device(net: int, index, density, T, Td, uv, abundance, rate, phys: O
     idx = (C.c_int * 13)(*[int(i) for i in index])
     ph = phys or default_phys()
     s = stream if stream is not None else torch.cuda.current_stream(density.device).cuda_stream
-    rc = L.pdrchem_heating_device(net, C.byref(ph), idx, npts, *[t.data_ptr() for t in args], out.data_ptr(), s)
+    if zeta is not None and not (zeta.is_cuda and zeta.dtype == torch.float64 and zeta.numel() == npts):
+        raise PdrChemError("heating_device: zeta must be a float64 device tensor of one value per point")
+    rc = L.pdrchem_heating_device_zeta(net, C.byref(ph), idx, npts, *[t.data_ptr() for t in args],
+                                       zeta.data_ptr() if zeta is not None else None, out.data_ptr(), s)
     if rc != 0:
         raise PdrChemError("pdrchem_heating_device failed (network, index or arguments)")
     return out

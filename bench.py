@@ -81,6 +81,135 @@ def oracle_mode(name):
     return oracle.MODE_GPU if name == "reduced" else oracle.MODE_GPU_WG
 
 
+def host_cpus():
+    """Host CPUs as the box presents them: os.cpu_count(), the CPUs this
+    process may run on, the per-GPU CPU share (OMP_NUM_THREADS where the
+    environment sets it) and the model name."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or aff
+    return {"os_cpu_count": os.cpu_count(), "affinity": aff, "share": min(share, aff), "model": model}
+
+
+def electron_check(name, net, nid, prec, x0, rate, nH, got_charge_sum, idx):
+    """The electron slot under the reference's semantics (x_e of the last RHS
+    evaluation, calculate_abundances.c:317) on the checked points: a second
+    device solve of those points with that policy, against the oracle with
+    the same policy, held to the bound the species bar and both sides'
+    measured overshoot set (|x_e - q.y| of each side)."""
+    import torch
+    import oracle
+    o = oracle.Oracle(name)
+    lib_ = __import__("pdrchem.lib", fromlist=["lib"])
+    dev = rate.device
+    ab = torch.tensor(x0[idx], device=dev)
+    lib_.set_electron_policy(lib_.XE_LAST_RHS)
+    lib_.solve_device(nid, ab, rate[idx].contiguous(), nH[idx].contiguous(), precision=prec)
+    torch.cuda.synchronize()
+    got = ab.cpu().numpy()
+    _, ref, _ = o.calculate_abundances(x0[idx].copy(), rate[idx].cpu().numpy(), nH[idx].cpu().numpy(),
+                                       nH[idx].cpu().numpy(), mode=oracle_mode(name),
+                                       nthreads=host_cpus()["share"], xe_policy=0)
+    q = np.array(net.charges(), dtype=np.float64)
+    qg, qo = got[:, :-1] @ q, ref[:, :-1] @ q
+    diff = np.abs(got[:, -1] - ref[:, -1])
+    bound = 1e-4 * qo + np.abs(got[:, -1] - qg) + np.abs(ref[:, -1] - qo)
+    same = bool(np.array_equal(got[:, :-1], got_charge_sum[:, :-1]))
+    return {"rule": "x_e of the last RHS evaluation (calculate_abundances.c:317), the reference's",
+            "max_rel_err_vs_oracle": float(np.max(diff / ref[:, -1])),
+            "max_diff_over_measured_bound": float(np.max(diff / bound)),
+            "bound": "1e-4*q.y_oracle + |x_e,gpu - q.y_gpu| + |x_e,oracle - q.y_oracle| per point",
+            "species_unchanged_by_policy": same, "points": int(len(idx))}
+
+
+def reference_config_record(name, pts, x0, rates, seconds, threads, mxstep=20000):
+    """The reference's own integrator configuration (CVODE's difference-
+    quotient Jacobian, plain dense LU: oracle MODE_REFERENCE) on the first
+    points of the workload, bounded: mxstep per CVode call (so at most five
+    times that many steps per point) and a time budget.  Reported as a
+    record, not as the baseline: at atol 1e-30 it fails most points."""
+    import oracle
+    o = oracle.Oracle(name)
+    t0 = time.perf_counter()
+    done = ok = 0
+    nst = ncf = 0
+    flags = []
+    while time.perf_counter() - t0 < seconds and done < pts.n:
+        sl = slice(done, min(done + threads, pts.n))
+        st, _, so = o.calculate_abundances(x0[sl].copy(), rates[sl], pts.n_H[sl], pts.T[sl], mode=oracle.MODE_REFERENCE,
+                                           nthreads=threads, mxstep=mxstep)
+        ok += int(np.sum(so["flag"] == 0))
+        nst += int(so["nst"].sum())
+        ncf += int(so["ncfn"].sum())
+        flags += [int(f) for f in so["flag"]]
+        done = sl.stop
+    el = time.perf_counter() - t0
+    return {"value": ok / el, "unit": "solves/s", "cores": threads, "kind": "port",
+            "attempted": done, "completed": ok, "seconds": el, "mean_steps": nst / max(done, 1),
+            "mean_conv_failures": ncf / max(done, 1), "flags": flags[:64],
+            "sample": f"first {done} points, CPU oracle in the reference's configuration (difference-quotient "
+                      f"Jacobian, plain dense LU), mxstep {mxstep} per CVode call"}
+
+
+def full_subset_ratios(npts=8192, cpu_seconds=8.0):
+    """215-species ratios on a fixed seeded subset of the full-network
+    workload (the first npts points of its random-point stream, to 1e7 yr):
+    the mixed form served through the request loop (one device request,
+    graph launch), the plain FP64 form by a direct launch, and the CPU
+    oracle on a bounded sample."""
+    import torch
+    from pdrchem import lib, workloads
+    from pdrchem.netgen import load_network
+    dev = torch.device("cuda:0")
+    name = "full"
+    net = load_network(name)
+    nid = lib.network_id(name)
+    P = workloads.random_points(npts, columns=True)
+    x0 = workloads.initial_abundances(net, npts)
+    rate, _ = lib.rates_device(nid, *[torch.tensor(v, device=dev) for v in (P.T, P.Td, P.G, P.Av, P.cols)])
+    nH = torch.tensor(P.n_H, device=dev)
+    x0d = torch.tensor(x0, device=dev)
+    torch.cuda.synchronize()
+    # warm-up of both kernels on a few points
+    for prec in (lib.MIXED, lib.FP64):
+        w = x0d[:64].clone()
+        lib.solve_device(nid, w, rate[:64].contiguous(), nH[:64].contiguous(), precision=prec)
+    torch.cuda.synchronize()
+    srv = lib.Server(device_bytes=2 << 30, nslots=4, precision=lib.MIXED)
+    try:
+        ab = x0d.clone()
+        st = torch.zeros(npts, dtype=torch.int32, device=dev)
+        t0 = time.perf_counter()
+        srv.submit_device(nid, ab, rate, nH, 1e7, status=st)
+        srv.progress(wait=True)
+        t_served = time.perf_counter() - t0
+        res_served = srv.poll()
+    finally:
+        srv.close()
+    ab64 = x0d.clone()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s64, _ = lib.solve_device(nid, ab64, rate, nH, precision=lib.FP64)
+    torch.cuda.synchronize()
+    t_fp64 = time.perf_counter() - t0
+    cpu = cpu_baseline(name, P, x0, rate.cpu().numpy(), cpu_seconds, host_cpus()["share"])
+    v_mixed, v_fp64 = npts / t_served, npts / t_fp64
+    return {"subset": f"first {npts} points of the full-network (215 species) workload, to 1e7 yr",
+            "mixed_served_solves_s": v_mixed, "fp64_direct_solves_s": v_fp64,
+            "mixed_over_fp64": v_mixed / v_fp64, "cpu_solves_s": cpu["value"], "cpu_cores": cpu["cores"],
+            "mixed_over_cpu": v_mixed / cpu["value"],
+            "failed": int((st != 0).sum().item()) + int((s64 != 0).sum().item()),
+            "served_requests": len(res_served)}
+
+
 def cpu_baseline(name, pts, x0, rates, seconds, threads):
     """Oracle (CPU, same integrator as the GPU path) on a bounded sample of
     the same workload: chunks of points until `seconds` have elapsed."""
@@ -112,7 +241,14 @@ def main():
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--check", type=int, default=64, help="points checked against the oracle")
+    ap.add_argument("--check", type=int, default=None,
+                    help="points checked against the oracle, evenly spaced over the batch "
+                         "(default 1000 reduced, 200 others)")
+    ap.add_argument("--electron", choices=["charge-sum", "last-rhs"], default="charge-sum",
+                    help="electron slot of the timed solve: charge sum of the returned abundances, or x_e "
+                         "of the last RHS evaluation (the reference's); the other is checked on the side")
+    ap.add_argument("--no-full-subset", action="store_true",
+                    help="skip the 215-species subset ratios of the default run")
     ap.add_argument("--serve", action="store_true",
                     help="serving workload: Poisson stream of requests through the request loop")
     ap.add_argument("--serve-rate", type=float, default=6.0, help="requests per second")
@@ -135,6 +271,8 @@ def main():
     a = ap.parse_args()
     if a.npts is None:
         a.npts = 1_000_000 if a.network == "reduced" else 20_000
+    if a.check is None:
+        a.check = 1000 if a.network == "reduced" else 200
     if a.gpus != 1:
         print(f"bench.py: runs on one GPU; --gpus {a.gpus} is not supported "
               f"(grid points are independent: a multi-GPU run would give each GPU its own range of points)",
@@ -172,6 +310,7 @@ def main():
     stats = torch.zeros((a.npts, 8), dtype=torch.int32, device=dev)
     counter = torch.zeros(1, dtype=torch.int32, device=dev)
     stream = torch.cuda.current_stream(dev)
+    lib.set_electron_policy(lib.XE_CHARGE_SUM if a.electron == "charge-sum" else lib.XE_LAST_RHS)
 
     def step(ev=None):
         neg.zero_()
@@ -241,6 +380,8 @@ def main():
                                "log10 G0~U[0,5], to 1e7 yr",
                    "points": a.npts, "precision": a.precision, "rtol": 1e-8, "atol": 1e-30},
         "failed_points": nfail,
+        "electron_policy": ("charge sum of the returned abundances (x_e at t_end)" if a.electron == "charge-sum"
+                            else "x_e of the last RHS evaluation (calculate_abundances.c:317)"),
         "launch_shape": shape,
         "mean_steps_per_point": float(st_np[:, 0].mean()),
         "solver_kernel_ms": kt * 1e3,
@@ -250,22 +391,45 @@ def main():
                      "note": "work W of CVODE's algorithm (dense LU) for this run's integrator counters; "
                              "peak blends FP64 78.6 and (mixed form, factorisation) FP32 157.3 TFLOP/s"},
     }
+    cpus = host_cpus()
+    res["host_cpus"] = cpus
     if a.check > 0:
         import oracle
         o = oracle.Oracle(name)
         k = min(a.check, a.npts)
-        got = ab[:k].cpu().numpy()
-        _, ref, _ = o.calculate_abundances(x0_np[:k].copy(), rate[:k].cpu().numpy(), pts.n_H[:k], pts.T[:k],
-                                           mode=oracle_mode(name), nthreads=min(16, os.cpu_count() or 1))
-        m = ref[:, :-1] > 1e-20
-        res["max_rel_err_vs_oracle"] = float(np.max(np.abs(got[:, :-1][m] - ref[:, :-1][m]) / ref[:, :-1][m]))
-        res["checked_points"] = k
+        idx = np.unique(np.linspace(0, a.npts - 1, k).astype(np.int64))
+        got = ab[idx].cpu().numpy()
+        xp = 1 if a.electron == "charge-sum" else 0
+        _, ref, _ = o.calculate_abundances(x0_np[idx].copy(), rate[idx].cpu().numpy(), pts.n_H[idx], pts.T[idx],
+                                           mode=oracle_mode(name), nthreads=cpus["share"], xe_policy=xp)
+        slots = slice(None) if xp == 1 else slice(0, -1)
+        g, r_ = got[:, slots], ref[:, slots]
+        m = r_ > 1e-20
+        res["max_rel_err_vs_oracle"] = float(np.max(np.abs(g[m] - r_[m]) / r_[m]))
+        res["max_rel_err_slots"] = ("all species including the electron (charge-sum policy)" if xp == 1 else
+                                    "all species except the electron slot (reported under electron_check)")
+        res["checked_points"] = int(len(idx))
+        res["checked_points_rule"] = f"{len(idx)} points evenly spaced over the {a.npts}-point batch"
+        res["points_over_bar"] = int(np.sum(np.any(np.where(m, np.abs(g - r_) / np.where(m, r_, 1.0), 0.0) > 1e-4,
+                                                   axis=1)))
         res["per_species_max_rel_err"] = per_species_errors(net, got, ref)
+        if xp == 1:
+            sel = idx[: min(len(idx), 256)]
+            res["electron_check"] = electron_check(name, net, nid, prec, x0_np, rate, nH, got[: len(sel)], sel)
+            lib.set_electron_policy(lib.XE_CHARGE_SUM)
     if not a.no_cpu:
-        thr = min(16, os.cpu_count() or 1)
+        thr = cpus["share"]
         k = min(a.npts, 200000)
         res["cpu_baseline"] = cpu_baseline(name, pts.subset(np.arange(k)), x0_np[:k], rate[:k].cpu().numpy(),
                                            a.cpu_seconds, thr)
+        res["cpu_baseline"]["cores_note"] = (f"{thr} threads = the CPU share of this GPU (OMP_NUM_THREADS / "
+                                             f"affinity); os.cpu_count() {cpus['os_cpu_count']}")
+        if name == "reduced":
+            res["cpu_baseline_reference"] = reference_config_record(name, pts.subset(np.arange(k)), x0_np[:k],
+                                                                    rate[:k].cpu().numpy(), 10.0, thr)
+    if name == "reduced" and not a.no_full_subset and prec == lib.MIXED:
+        res["full_network_subset"] = full_subset_ratios()
+    lib.set_electron_policy(lib.XE_LAST_RHS)
     print(json.dumps(res))
 
 
@@ -438,8 +602,8 @@ def cloud_bench(a):
     }
     # iteration 1 checked point by point against the FP64 oracle fed with the
     # same inputs (device T, G, Av, columns; oracle rates)
-    k = min(a.check, side ** 3) if a.check > 0 else 0
-    thr = min(16, os.cpu_count() or 1)
+    k = min(a.check, 256, side ** 3) if a.check > 0 else 0
+    thr = host_cpus()["share"]
     if k:
         sel = np.linspace(0, side ** 3 - 1, k).astype(int)
         o = oracle.Oracle(name)

@@ -126,6 +126,49 @@ int pdrchem_heating_device(int net, const pdrchem_phys *phys, const int *index, 
                            const double *d_density, const double *d_T, const double *d_Td, const double *d_uv,
                            const double *d_abundance, const double *d_rate, double *d_heat, void *stream);
 
+/* As pdrchem_heating_device, with a cosmic-ray ionisation rate per point:
+ * d_zeta[npts] (units of 1.3e-17 s^-1, like phys->zeta; NULL = phys->zeta
+ * everywhere) in the cosmic-ray heating term (sub_calculate_heating.F90:201),
+ * which the reference evaluates with its one global zeta.  For clouds whose
+ * rates use a per-point zeta (pdrchem_rates_device_zeta), so that heating and
+ * chemistry see the same rate. */
+int pdrchem_heating_device_zeta(int net, const pdrchem_phys *phys, const int *index, int npts,
+                                const double *d_density, const double *d_T, const double *d_Td, const double *d_uv,
+                                const double *d_abundance, const double *d_rate, const double *d_zeta,
+                                double *d_heat, void *stream);
+
+/* Solver options (no counterpart in the reference's interface, where they
+ * are locals of calculate_abundances.c).  mxstep: internal steps allowed per
+ * CVode call before it returns -1 (calculate_abundances.c:111 sets 1e7, the
+ * default here); after five such returns a point fails with status -1, and
+ * its partial result is written back (:216-305).  Applies to every later
+ * solve, request-loop batch and cloud iteration.  Returns 0, -1 if mxstep < 1. */
+int pdrchem_set_mxstep(long long mxstep);
+long long pdrchem_mxstep(void);
+
+/* What the electron slot holds on return.  PDRCHEM_XE_LAST_RHS (default):
+ * x_e of the integrator's last RHS evaluation, as the reference writes it
+ * (calculate_abundances.c:317; the charge sum at a Newton iterate of the last
+ * step, which may lie past t_end).  PDRCHEM_XE_CHARGE_SUM: the charge sum of
+ * the abundances written back, i.e. x_e at t_end.  Returns 0, -1 if unknown. */
+#define PDRCHEM_XE_LAST_RHS 0
+#define PDRCHEM_XE_CHARGE_SUM 1
+int pdrchem_set_electron_policy(int policy);
+int pdrchem_electron_policy(void);
+
+/* The solver's Jacobian at given abundances (introspection; no counterpart in
+ * the reference, whose analytic jacobian_*.c is not registered,
+ * calculate_abundances.c:210-212).  d_abundance [npts*nspec], d_rate
+ * [npts*nreac], d_density [npts] as for pdrchem_solve_device; writes d_jac
+ * [npts][nspec-1][nspec]: entry (i, j) = d ydot_i / d x_j at fixed x_e for
+ * j < nspec-1 and d ydot_i / d x_e for j = nspec-1 (x_e = charge sum of the
+ * abundances), evaluated by the device code the solver runs, in the matrix
+ * precision of `precision` (PDRCHEM_MIXED rounds each entry to FP32 as the
+ * solver stores it).  The Jacobian of CVODE's iteration matrix is
+ * J = J0 + g q^T with q the charges.  Asynchronous on stream; 0 = queued. */
+int pdrchem_jacobian_device(int net, int precision, const double *d_abundance, const double *d_rate,
+                            const double *d_density, int npts, double *d_jac, void *stream);
+
 /* Kernel resource usage of the solver for a network and precision
  * (registers per lane, LDS bytes per block, scratch bytes per lane, blocks
  * per CU).  0 = ok. */

@@ -318,9 +318,14 @@ static int hybrid_factor(int nb, double *a, const int *order, int nhead, double 
   for (int h = 0; h < nhead; h++) {
     const int k = order[h];
     const double d = a[(size_t)k * nb + k];
-    if (d == 0.0) return 1;
+    /* no pivoting in the head: a zero or non-finite pivot (in FP32, one whose
+     * reciprocal is not a finite normal float) fails the setup */
+    if (d == 0.0 || !isfinite(d)) return 1;
     double inv = 1.0 / d;
-    if (f32) inv = (float)inv;
+    if (f32) {
+      inv = (float)inv;
+      if (!isfinite(inv) || fabs(d) < FLT_MIN) return 1;
+    }
     a[(size_t)k * nb + k] = inv;
     for (int hh = h + 1; hh < nb; hh++) {
       const int i = order[hh];
@@ -397,7 +402,7 @@ static int lsetup_hybrid(bdf_mem *m, int convfail, const double *ypred, int *jcu
   const size_t nn = (size_t)nb * nb;
   double dgamma = fabs(m->gamma / m->gammap - 1.0);
   int jbad = (m->nst == 0) || (m->nst > m->nstlj + MSBJ) ||
-             ((convfail == CV_FAIL_BAD_J) && (dgamma < DGMAX_J)) || (convfail == CV_FAIL_OTHER);
+             ((convfail == CV_FAIL_BAD_J) && !(dgamma >= DGMAX_J)) || (convfail == CV_FAIL_OTHER);
   if (!jbad) {
     *jcur = 0;
   } else {
@@ -431,7 +436,7 @@ static int lsetup(bdf_mem *m, int convfail, const double *ypred, const double *f
   if (m->mode & PDR_MODE_HYBRID) return lsetup_hybrid(m, convfail, ypred, jcur);
   double dgamma = fabs(m->gamma / m->gammap - 1.0);
   int jbad = (m->nst == 0) || (m->nst > m->nstlj + MSBJ) ||
-             ((convfail == CV_FAIL_BAD_J) && (dgamma < DGMAX_J)) || (convfail == CV_FAIL_OTHER);
+             ((convfail == CV_FAIL_BAD_J) && !(dgamma >= DGMAX_J)) || (convfail == CV_FAIL_OTHER);
   if (!jbad) {
     *jcur = 0;
     memcpy(m->M, m->savedJ, sizeof(double) * n * n);
@@ -988,4 +993,16 @@ int bdf_solve(bdf_mem *m, double tout, double *yout, double *tret) {
       return 0;
     }
   }
+}
+
+/* The bordered factorisation's status for one matrix (0 = factorised, 1 =
+ * setup failure), for tests of its pivot rule; a[nb*nb] is overwritten. */
+int pdr_test_hybrid_factor(int nb, double *a, const int *order, int nhead, int f32) {
+  const int t = nb - nhead;
+  double *tail = malloc(sizeof(double) * (size_t)(t > 0 ? t * t : 1));
+  int *tpiv = malloc(sizeof(int) * (size_t)(t > 0 ? t : 1));
+  int rc = (tail && tpiv) ? hybrid_factor(nb, a, order, nhead, tail, tpiv, f32) : -1;
+  free(tail);
+  free(tpiv);
+  return rc;
 }

@@ -17,6 +17,16 @@
 
 #define SECONDS_IN_YEAR 3.1556926e7
 
+/* steps per CVode call (calculate_abundances.c:111) and what the electron
+ * slot holds on return: 0 = x_e of the last RHS evaluation (:317), 1 = the
+ * charge sum of the written-back abundances.  Set before a call, never
+ * during one. */
+static long g_mxstep = 10000000L;
+static int g_xe_policy = 0;
+
+void pdr_set_mxstep(long mxstep) { g_mxstep = mxstep > 0 ? mxstep : 10000000L; }
+void pdr_set_xe_policy(int policy) { g_xe_policy = policy ? 1 : 0; }
+
 typedef struct {
   const pdr_net *net;
   const double *rate;
@@ -58,7 +68,7 @@ static void fill_stats(pdr_cvode_stats *st, const bdf_mem *m, int flag) {
   if (!st) return;
   st->nst = m->nst; st->nfe = m->nfe; st->nfeDQ = m->nfeDQ; st->nsetups = m->nsetups;
   st->nje = m->nje; st->nni = m->nni; st->ncfn = m->ncfn; st->netf = m->netf;
-  st->qlast = m->qu; st->hlast = m->hu; st->flag = flag;
+  st->qlast = m->qu; st->hlast = m->hu; st->flag = flag; st->nfails = 0;
 }
 
 /* one grid point: returns the status contribution of calculate_abundances.c */
@@ -66,7 +76,7 @@ static int solve_point(const pdr_net *net, double *ab, const double *rate, doubl
                        double reltol, double abstol, double start_time, double end_time,
                        int nelect0, int mode, pdr_cvode_stats *st) {
   const int neq = net->neq;
-  const long mxstep = 10000000L;
+  const long mxstep = g_mxstep;
   chem_ud ud = {net, rate, n_H, ab[nelect0], 0};
   double *qbuf = malloc(sizeof(double) * (neq + 1));
   bdf_mem *m = qbuf ? create_for_net(net, &ud, reltol, abstol, mxstep, mode, qbuf) : NULL;
@@ -95,12 +105,15 @@ static int solve_point(const pdr_net *net, double *ab, const double *rate, doubl
     tout = tout * 10;
     if (tout > tend) tout = tend;
   } while (t < tout && nfails < 5);
+  double xe_sum = 0.0;
   for (int i = 0; i < neq; i++) {
     ab[i] = y[i];
     if (ab[i] < abstol) ab[i] = 0;
+    xe_sum += net->charge[i] * ab[i];
   }
-  ab[nelect0] = ud.x_e;
+  ab[nelect0] = g_xe_policy ? xe_sum : ud.x_e;
   fill_stats(st, m, flag);
+  if (st) st->nfails = nfails;
   free(y);
   bdf_free(m);
   free(qbuf);

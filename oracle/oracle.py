@@ -53,12 +53,12 @@ class Opts(C.Structure):
 class Stats(C.Structure):
     _fields_ = [("nst", C.c_long), ("nfe", C.c_long), ("nfeDQ", C.c_long), ("nsetups", C.c_long),
                 ("nje", C.c_long), ("nni", C.c_long), ("ncfn", C.c_long), ("netf", C.c_long),
-                ("qlast", C.c_int), ("hlast", C.c_double), ("flag", C.c_int)]
+                ("qlast", C.c_int), ("hlast", C.c_double), ("flag", C.c_int), ("nfails", C.c_int)]
 
 
 STATS_DTYPE = np.dtype([("nst", "<i8"), ("nfe", "<i8"), ("nfeDQ", "<i8"), ("nsetups", "<i8"),
                         ("nje", "<i8"), ("nni", "<i8"), ("ncfn", "<i8"), ("netf", "<i8"),
-                        ("qlast", "<i4"), ("hlast", "<f8"), ("flag", "<i4")], align=True)
+                        ("qlast", "<i4"), ("hlast", "<f8"), ("flag", "<i4"), ("nfails", "<i4")], align=True)
 
 
 def default_phys(**kw) -> Phys:
@@ -97,6 +97,11 @@ def lib():
                                                C.c_double, C.c_int, C.c_int, C.c_int, C.c_void_p]
         L.pdr_integrate_point.argtypes = [C.c_void_p, _dp, C.c_double, _dp, C.c_double, C.c_int,
                                           _dp, _dp, C.POINTER(Opts), C.POINTER(Stats)]
+        L.pdr_test_hybrid_factor.argtypes = [C.c_int, _dp, C.POINTER(C.c_int), C.c_int, C.c_int]
+        L.pdr_set_mxstep.argtypes = [C.c_long]
+        L.pdr_set_mxstep.restype = None
+        L.pdr_set_xe_policy.argtypes = [C.c_int]
+        L.pdr_set_xe_policy.restype = None
         L.pdr_integrate_generic.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, _dp,
                                             C.c_double, C.c_int, _dp, _dp, C.POINTER(Opts),
                                             C.POINTER(Stats)]
@@ -169,8 +174,10 @@ class Oracle:
 
     def calculate_abundances(self, abundance, rate, density, temperature=None, rtol=1e-8,
                              atol=1e-30, start_time=0.0, end_time=1e7, mode=0,
-                             nthreads=0, nelect=None):
-        """calculate_abundances_ semantics; returns (status, abundance, stats)."""
+                             nthreads=0, nelect=None, mxstep=10_000_000, xe_policy=0):
+        """calculate_abundances_ semantics; returns (status, abundance, stats).
+        mxstep: steps per CVode call; xe_policy 1: electron slot = charge sum
+        of the returned abundances (0: x_e of the last RHS evaluation)."""
         ab = np.array(abundance, dtype=np.float64, order="C", copy=True)
         npart = ab.shape[0]
         rate = np.ascontiguousarray(rate, dtype=np.float64)
@@ -179,9 +186,13 @@ class Oracle:
                                                     (npart,)), dtype=np.float64)
         stats = np.zeros(npart, dtype=STATS_DTYPE)
         ne = nelect if nelect is not None else self.nspec
+        lib().pdr_set_mxstep(int(mxstep))
+        lib().pdr_set_xe_policy(int(xe_policy))
         st = lib().pdr_calculate_abundances(ab.ravel(), rate.ravel(), density, temp, npart, self.nspec,
                                             self.nreac, self.h, rtol, atol, start_time, end_time, ne,
                                             mode, nthreads, stats.ctypes.data)
+        lib().pdr_set_mxstep(10_000_000)
+        lib().pdr_set_xe_policy(0)
         return st, ab, stats
 
     def integrate(self, rate, n_H, y0, touts, t0=0.0, rtol=1e-8, atol=1e-30, mode=0,

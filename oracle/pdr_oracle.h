@@ -137,6 +137,7 @@ typedef struct {
   int qlast;
   double hlast;
   int flag;             /* last CVode return flag */
+  int nfails;           /* CVode calls of pdr_calculate_abundances that failed (retried or final) */
 } pdr_cvode_stats;
 
 /* calculate_abundances_ (calculate_abundances.c:64-354) on the CPU:
@@ -149,6 +150,19 @@ int pdr_calculate_abundances(double *abundance, const double *rate,
                              const pdr_net *net, double reltol, double abstol,
                              double start_time, double end_time, int nelect,
                              int mode, int nthreads, pdr_cvode_stats *stats);
+
+/* Options of pdr_calculate_abundances (set before a call, not during one):
+ * steps allowed per CVode call (calculate_abundances.c:111, default 1e7),
+ * and the electron slot on return: 0 = x_e of the last RHS evaluation
+ * (:317, default), 1 = the charge sum of the written-back abundances. */
+void pdr_set_mxstep(long mxstep);
+void pdr_set_xe_policy(int policy);
+
+/* Status of the bordered head/tail factorisation of one nb x nb matrix
+ * (PDR_MODE_HYBRID's setup): 0, or 1 if a head pivot is zero or not finite
+ * (in FP32, f32 != 0, also if its reciprocal is not a finite normal float)
+ * or the tail is singular.  a is overwritten.  For tests of the rule. */
+int pdr_test_hybrid_factor(int nb, double *a, const int *order, int nhead, int f32);
 
 /* Single integration of one point to a list of output times (seconds);
  * writes y at each time into yout[ntout*neq].  For integrator unit tests. */

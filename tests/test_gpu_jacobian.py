@@ -1,0 +1,108 @@
+"""The device Jacobian (the exact J that replaces the unregistered
+jacobian_*.c, calculate_abundances.c:210-212) read back through
+pdrchem_jacobian_device and compared entry by entry with the oracle's
+pdr_jac_bordered (itself pinned by central differences of the RHS), for all
+three networks and both matrix precisions; and the integrator counters of the
+plain FP64 form against the oracle running the same algorithm.  A wrong
+device Jacobian changes step counts, not answers, so the solve parity tests
+alone cannot see it.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from pdrchem import lib, workloads
+from pdrchem.netgen import load_network
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _states(net, n, seed):
+    """abundances log-uniform in [1e-20, 1e-2] (the electron slot is unused:
+    the kernels take x_e as the charge sum)"""
+    rng = np.random.default_rng(seed)
+    x = 10.0 ** rng.uniform(-20.0, -2.0, size=(n, net.nspec))
+    x[:, -1] = 0.0
+    return x
+
+
+@pytest.mark.parametrize("name", ["reduced", "full", "mynetwork"])
+def test_device_jacobian_vs_oracle(name):
+    import torch
+    net = load_network(name)
+    orc = oracle.Oracle(name)
+    d = np.load(os.path.join(GOLD, f"rates_{name}.npz"))
+    npt = 6
+    rates = np.ascontiguousarray(d["rates"][:npt])
+    nH = np.array([1e2, 1e3, 1e4, 1e5, 1e6, 3e3])[:npt]
+    x = _states(net, npt, seed=11)
+    dev = torch.device("cuda:0")
+    nid = lib.network_id(name)
+    q = np.array(net.charges(), dtype=np.float64)
+    for prec, rtol in ((lib.FP64, 1e-12), (lib.MIXED, 2e-6)):
+        print(f"case: device Jacobian {name} {npt} points precision={prec}", flush=True)
+        J = lib.jacobian_device(nid, torch.tensor(x, device=dev), torch.tensor(rates, device=dev),
+                                torch.tensor(nH, device=dev), precision=prec)
+        torch.cuda.synchronize()
+        J = J.cpu().numpy()
+        worst = 0.0
+        for k in range(npt):
+            ref = orc.jac_bordered(rates[k], nH[k], x[k])[: net.neq, :]
+            got = J[k]
+            scale = np.max(np.abs(ref), axis=1, keepdims=True)
+            err = np.abs(got - ref) / (np.abs(ref) + 1e-3 * scale + 1e-300)
+            worst = max(worst, float(err.max()))
+            # same structure: nothing where the oracle's entry is zero (up to
+            # the rounding of terms that cancel exactly in the oracle's order)
+            assert np.all(np.abs(got[ref == 0]) <= 1e-12 * np.broadcast_to(scale, ref.shape)[ref == 0]), name
+            # the iteration matrix's J = J0 + g q^T is the oracle's full J
+            if prec == lib.FP64:
+                dense = got[:, : net.neq] + np.outer(got[:, net.neq], q)
+                jref = orc.jac(rates[k], nH[k], x[k])
+                jscale = np.max(np.abs(jref), axis=1, keepdims=True)
+                assert np.all(np.abs(dense - jref) <= 1e-12 * np.abs(jref) + 1e-13 * jscale + 1e-300)
+        print(f"  max relative entry error {worst:.2e}")
+        assert worst <= rtol
+
+
+@pytest.mark.parametrize("name,mode,npt", [("reduced", oracle.MODE_GPU, 40),
+                                          ("mynetwork", oracle.MODE_GPU_WG, 16),
+                                          ("full", oracle.MODE_GPU_WG, 6)])
+def test_fp64_counters_match_oracle(name, mode, npt):
+    """The plain FP64 form runs the oracle's algorithm: its step, Jacobian,
+    setup and Newton counts agree with the oracle's on average within 2%, and
+    per point within a few percent.  Per-point counts drift apart (median
+    ~3% measured) because the device takes the step-size roots with the
+    hardware log/exp (~1e-7 relative), which eventually changes a step-size
+    decision; a wrong Jacobian would instead inflate the Newton iterations,
+    setups and steps of every point."""
+    import torch
+    net = load_network(name)
+    orc = oracle.Oracle(name)
+    d = np.load(os.path.join(GOLD, f"converged_{name}.npz"))
+    x0 = np.ascontiguousarray(d["x0"][:npt])
+    r = np.ascontiguousarray(d["rates"][:npt])
+    nH = np.ascontiguousarray(d["n_H"][:npt])
+    st, _, so = orc.calculate_abundances(x0.copy(), r, nH, nH, mode=mode, nthreads=16)
+    assert st == 0
+    dev = torch.device("cuda:0")
+    ab = torch.tensor(x0, device=dev)
+    stats = torch.zeros((npt, 8), dtype=torch.int32, device=dev)
+    print(f"case: counters {name} {npt} points FP64", flush=True)
+    status, _ = lib.solve_device(lib.network_id(name), ab, torch.tensor(r, device=dev), torch.tensor(nH, device=dev),
+                                 stats=stats, precision=lib.FP64)
+    torch.cuda.synchronize()
+    assert int(status.abs().sum().item()) == 0
+    g = stats.cpu().numpy().astype(np.float64)
+    for col, key in ((0, "nst"), (2, "nje"), (3, "nsetups"), (4, "nni")):
+        o = so[key].astype(np.float64)
+        rel = np.abs(g[:, col] - o) / np.maximum(o, 1.0)
+        print(f"  {key}: device mean {g[:, col].mean():.1f} oracle mean {o.mean():.1f}, "
+              f"median rel diff {np.median(rel):.3f}, max {rel.max():.3f}")
+        assert abs(g[:, col].mean() - o.mean()) <= 0.02 * o.mean(), key
+        assert np.median(rel) <= 0.05, key
+        assert rel.max() <= 0.15, key

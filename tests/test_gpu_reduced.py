@@ -14,6 +14,7 @@ import numpy as np
 import pytest
 
 import oracle
+from parity import charge_sum, electron_bound
 from pdrchem import lib, workloads
 from pdrchem.netgen import load_network
 
@@ -144,14 +145,32 @@ def test_random_points_device_path(orc, net):
     rate = torch.tensor(d["rates"], device=dev)
     oerr = np.array([max_rel(ref[k, :-1], conv[k, :-1]) for k in range(n)])
     good = oerr <= 2.5e-5
-    for prec in (lib.FP64, lib.MIXED):
+    q = net.charges()
+    for prec, pol in ((lib.FP64, lib.XE_LAST_RHS), (lib.MIXED, lib.XE_LAST_RHS), (lib.MIXED, lib.XE_CHARGE_SUM)):
+        print(f"case: reduced random points precision={prec} electron policy={pol}", flush=True)
+        lib.set_electron_policy(pol)
         ab = torch.tensor(x0, device=dev)
         stats = torch.zeros((n, 8), dtype=torch.int32, device=dev)
-        status, _ = lib.solve_device(nid, ab, rate, torch.tensor(d["n_H"], device=dev), stats=stats,
-                                     precision=prec)
-        torch.cuda.synchronize()
+        try:
+            status, _ = lib.solve_device(nid, ab, rate, torch.tensor(d["n_H"], device=dev), stats=stats,
+                                         precision=prec)
+            torch.cuda.synchronize()
+        finally:
+            lib.set_electron_policy(lib.XE_LAST_RHS)
         assert int(status.abs().sum().item()) == 0
         got = ab.cpu().numpy()
+        if pol == lib.XE_LAST_RHS:
+            ratio, xe_rel = electron_bound(got, ref, q)
+            print(f"  x_e (last RHS): max rel diff {xe_rel.max():.2e}, max diff / measured bound {ratio.max():.3f}")
+            assert ratio.max() <= 1.0
+        else:
+            # the charge sum of the returned abundances, held to the species bar
+            xe = charge_sum(ref, q)
+            xe_rel = np.abs(got[:, -1] - xe) / xe
+            print(f"  x_e (charge sum): max rel diff vs the oracle's charge sum on well-conditioned points "
+                  f"{xe_rel[good].max():.2e}")
+            assert xe_rel[good].max() <= TOL
+            assert np.allclose(got[:, -1], charge_sum(got, q), rtol=1e-13, atol=0)
         gerr_o = np.array([max_rel(got[k, :-1], ref[k, :-1]) for k in range(n)])
         gerr_c = np.array([max_rel(got[k, :-1], conv[k, :-1]) for k in range(n)])
         print(f"precision {prec}: {good.sum()}/{n} well-conditioned points, max err vs oracle there "

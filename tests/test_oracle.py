@@ -179,3 +179,96 @@ def test_calculate_abundances_semantics():
     q = np.array(net.charges(), dtype=float)
     assert np.allclose(out[:, -1], out[:, :-1] @ q, rtol=1e-6)
     assert np.all(stats["nst"] > 0)
+
+
+# ---------------------------------------------------------------------------
+# failure policy (calculate_abundances.c:216-317) and the electron slot
+# ---------------------------------------------------------------------------
+
+@pytest.fixture(scope="module")
+def slab8():
+    net = load_network("reduced")
+    o = oracle.Oracle("reduced")
+    P = workloads.slab_1d(1000).subset(np.arange(0, 1000, 125))
+    r = o.rates(P.T, P.Td, P.G, P.Av, P.cols)
+    return net, o, P, r, workloads.initial_abundances(net, P.n)
+
+
+def test_retries_continue_the_same_integration(slab8):
+    """A CVode call that runs out of steps (-1) is called again with the same
+    budget: a point that needs at most four such retries ends bitwise where
+    an unlimited run ends; the fifth failure ends the point with -1 and
+    writes back the state reached, short of t_end."""
+    net, o, P, r, x0 = slab8
+    st0, ref0, s0 = o.calculate_abundances(x0.copy(), r, P.n_H, P.T, mode=oracle.MODE_GPU, nthreads=8)
+    assert st0 == 0 and np.all(s0["nfails"] == 0)
+    st, ab, s = o.calculate_abundances(x0.copy(), r, P.n_H, P.T, mode=oracle.MODE_GPU, nthreads=8, mxstep=300)
+    assert st == -1
+    rec = (s["flag"] == 0) & (s["nfails"] > 0)
+    exh = s["flag"] == -1
+    assert rec.any() and exh.any()
+    assert np.all(s["nfails"][exh] == 5) and np.all(s["nfails"][~exh] < 5)
+    for k in np.where(~exh)[0]:
+        assert np.array_equal(ab[k], ref0[k])
+    for k in np.where(exh)[0]:
+        assert s["nst"][k] < s0["nst"][k]
+        assert not np.array_equal(ab[k], ref0[k])
+        assert np.all(np.isfinite(ab[k]))
+
+
+def test_nan_rate_fails_its_point_only(slab8):
+    net, o, P, r, x0 = slab8
+    rn = r.copy()
+    rn[3, 10] = np.nan
+    _, ref0, _ = o.calculate_abundances(x0.copy(), r, P.n_H, P.T, mode=oracle.MODE_GPU, nthreads=8, mxstep=5000)
+    st, ab, s = o.calculate_abundances(x0.copy(), rn, P.n_H, P.T, mode=oracle.MODE_GPU, nthreads=8, mxstep=5000)
+    assert st == -4 and s["flag"][3] == -4 and s["nst"][3] == 0 and s["nfails"][3] == 5
+    others = np.arange(P.n) != 3
+    assert np.array_equal(ab[others], ref0[others])
+
+
+def test_electron_policy(slab8):
+    """policy 1 writes the charge sum of the returned abundances; policy 0
+    (the reference's) the last RHS evaluation's x_e, which differs from it
+    by the last step's overshoot past t_end."""
+    net, o, P, r, x0 = slab8
+    q = np.array(net.charges(), dtype=np.float64)
+    _, a0, _ = o.calculate_abundances(x0.copy(), r, P.n_H, P.T, mode=oracle.MODE_GPU, nthreads=8)
+    _, a1, _ = o.calculate_abundances(x0.copy(), r, P.n_H, P.T, mode=oracle.MODE_GPU, nthreads=8, xe_policy=1)
+    assert np.array_equal(a0[:, :-1], a1[:, :-1])
+    assert np.allclose(a1[:, -1], a1[:, :-1] @ q, rtol=1e-14, atol=0)
+    assert not np.array_equal(a0[:, -1], a1[:, -1])
+    assert np.all(np.abs(a0[:, -1] - a1[:, -1]) <= 1e-2 * a1[:, -1])
+
+
+def test_hybrid_head_pivot_rule():
+    """The bordered factorisation eliminates its head without pivoting: a
+    zero or non-finite head pivot is a setup failure (the device kernel's
+    rule), in FP64 and in FP32 (where a pivot below the smallest normal
+    float also fails); a regular matrix factorises."""
+    L = oracle.lib()
+    nb = 4
+    order = np.array([0, 1, 2, 3], dtype=np.int32)
+    ip = order.ctypes.data_as(oracle.C.POINTER(oracle.C.c_int))
+    base = np.array([[4.0, 1.0, 0.0, 0.5], [1.0, 3.0, 1.0, 0.0], [0.0, 1.0, 2.0, 1.0], [0.5, 0.0, 1.0, 5.0]])
+
+    def status(a, f32=0):
+        a = np.ascontiguousarray(a, dtype=np.float64).ravel().copy()
+        return L.pdr_test_hybrid_factor(nb, a, ip, 2, f32)
+
+    assert status(base) == 0 and status(base, 1) == 0
+    z = base.copy()
+    z[0, 0] = 0.0
+    assert status(z) == 1 and status(z, 1) == 1
+    z = base.copy()
+    z[1, 1] = 0.25  # second head pivot 0.25 - 1/4 * 1 = 0 after the first elimination
+    assert status(z) == 1
+    z = base.copy()
+    z[0, 0] = np.nan
+    assert status(z) == 1
+    z = base.copy()
+    z[0, 0] = np.inf
+    assert status(z) == 1
+    z = base.copy()
+    z[0, 0] = 1e-40  # a normal double, a denormal float
+    assert status(z) == 0 and status(z, 1) == 1
