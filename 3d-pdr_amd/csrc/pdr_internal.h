@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <string>
+
 namespace pdrchem {
 
 enum NetId { NET_REDUCED = 0, NET_FULL = 1, NET_MYNETWORK = 2, NET_COUNT = 3 };
@@ -51,6 +53,8 @@ struct SolverOptions {
   int xe_policy;
 };
 SolverOptions solver_options();
+// text returned by pdrchem_last_error on this thread
+void set_last_error(const std::string &msg);
 
 struct KernelInfo {
   int num_regs, shared_bytes, local_bytes, max_threads, blocks_per_cu;

@@ -146,6 +146,7 @@ void log_failures(const std::vector<int> &st, double start_time, double end_time
 
 namespace pdrchem {
 SolverOptions solver_options() { return SolverOptions{g_mxstep, g_xe_policy}; }
+void set_last_error(const std::string &msg) { set_err(msg); }
 }  // namespace pdrchem
 
 extern "C" {

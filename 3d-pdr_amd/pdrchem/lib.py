@@ -34,7 +34,7 @@ EXPORTED_FUNCTIONS = [
     "pdrchem_cloud_destroy", "pdrchem_autotune", "pdrchem_tuned_config",
     "pdrchem_set_tuned_config", "pdrchem_clear_tuning", "pdrchem_heating_device",
     "pdrchem_heating_device_zeta", "pdrchem_set_mxstep", "pdrchem_mxstep", "pdrchem_set_electron_policy",
-    "pdrchem_electron_policy", "pdrchem_jacobian_device",
+    "pdrchem_electron_policy", "pdrchem_jacobian_device", "pdrchem_solve_device_noalloc",
 ]
 EXPORTED_GLOBALS = [
     ("chemistry_module_mp_relative_abundance_tolerance_", C.c_double),
@@ -101,6 +101,8 @@ def load(path: Optional[str] = None):
     L.pdrchem_mxstep.restype = C.c_longlong
     L.pdrchem_set_electron_policy.argtypes = [C.c_int]
     L.pdrchem_jacobian_device.argtypes = [C.c_int, C.c_int, vp, vp, vp, C.c_int, vp, vp]
+    L.pdrchem_solve_device_noalloc.argtypes = [C.c_int, vp, vp, vp, C.c_int, C.c_double, C.c_double, C.c_double,
+                                               C.c_double, C.c_int, vp, vp, C.c_int, vp]
     L.pdrchem_set_max_blocks.argtypes = [C.c_int]
     L.pdrchem_set_max_blocks.restype = None
     L.pdrchem_set_precision.argtypes = [C.c_int]
@@ -241,6 +243,29 @@ def solve_device(net: int, abundance, rate, density, rtol=1e-8, atol=1e-30, t_st
     rc = L.pdrchem_solve_device(net, _ptr(abundance), _ptr(rate), _ptr(density), n, rtol, atol, t_start, t_end,
                                 nelect, _ptr(status), _ptr(stats), _ptr(counter), precision,
                                 C.c_void_p(s.cuda_stream))
+    if rc != 0:
+        raise PdrChemError(last_error())
+    return status, stats
+
+
+def solve_device_noalloc(net: int, abundance, rate, density, rtol=1e-8, atol=1e-30, t_start=0.0, t_end=1e7,
+                         nelect=None, status=None, stats=None, precision=MIXED, stream=None):
+    """As solve_device through the allocation-free entry point (the library's
+    fixed scratch array instead of a per-stream workspace; no counter)."""
+    import torch
+    L = load()
+    n = abundance.shape[0]
+    if status is None:
+        status = torch.empty(n, dtype=torch.int32, device=abundance.device)
+    if nelect is None:
+        nelect = abundance.shape[1]
+    for t in (abundance, rate, density):
+        if t.dtype != torch.float64 or not t.is_contiguous():
+            raise PdrChemError("device tensors must be contiguous float64")
+    s = stream if stream is not None else torch.cuda.current_stream(abundance.device)
+    rc = L.pdrchem_solve_device_noalloc(net, _ptr(abundance), _ptr(rate), _ptr(density), n, rtol, atol, t_start,
+                                        t_end, nelect, _ptr(status), _ptr(stats), precision,
+                                        C.c_void_p(s.cuda_stream))
     if rc != 0:
         raise PdrChemError(last_error())
     return status, stats

@@ -80,6 +80,27 @@ int pdrchem_solve_device(int net, double *d_abundance, const double *d_rate, con
                          int npts, double rtol, double atol, double t_start, double t_end, int nelect,
                          int *d_status, int *d_stats, unsigned int *d_counter, int precision, void *stream);
 
+/* As pdrchem_solve_device, allocating nothing -- no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, hosts that forbid hipMalloc).  The
+ * caller passes inputs and outputs as for pdrchem_solve_device but no work
+ * counter: the solver's scratch memory (work-queue counter and per-block
+ * workspace of saved Jacobians and rate constants) is a fixed 48 MiB device
+ * array in the library, present on every GPU the library runs on until the
+ * process ends, which holds the workspace of a whole-device launch of every
+ * network and precision on MI355X.  The point count is not limited by it
+ * (the persistent kernel reuses each block's workspace for every point it
+ * takes); a launch shape whose single block would not fit is rejected with an
+ * error (pdrchem_last_error).  Every call on one GPU shares that GPU's array
+ * and overwrites what the last call left there: calls that overlap on one GPU
+ * (two streams, two threads) race and must be serialised by the caller.  The
+ * counter is cleared by a kernel (no memset node), so the call may be
+ * captured into a HIP graph.  Results are those of pdrchem_solve_device.
+ * Returns 0 if the launch was queued. */
+int pdrchem_solve_device_noalloc(int net, double *d_abundance, const double *d_rate, const double *d_density,
+                                 int npts, double rtol, double atol, double t_start, double t_end, int nelect,
+                                 int *d_status, int *d_stats, int precision, void *stream);
+
 /* Precision used by calculate_abundances_ (default PDRCHEM_MIXED; the
  * environment variable PDRCHEM_PRECISION=fp64 selects PDRCHEM_FP64). */
 void pdrchem_set_precision(int precision);
