@@ -478,3 +478,172 @@ def ref_heating(name, n, T, Td, uv, ab, rate, phys: Optional[Phys] = None, ref_d
         except ValueError:
             continue
     return np.array(vals).reshape(len(n), 12)
+
+
+# --------------------------------------------------------------------------
+# level populations, line cooling, dust temperature (thermal.c) and the
+# reference's own routines (oracle/_ref/ref_lvg_{1d,3d})
+# --------------------------------------------------------------------------
+
+class Coolant(C.Structure):
+    _fields_ = [("nlev", C.c_int), ("ntemp", C.c_int), ("E", C.c_void_p), ("g", C.c_void_p), ("A", C.c_void_p),
+                ("B", C.c_void_p), ("F", C.c_void_p), ("temps", C.c_void_p), ("col", C.c_void_p)]
+
+
+def _thermlib():
+    L = lib()
+    if not getattr(L, "_therm_ready", False):
+        L.pdr_find_ccoeff.argtypes = [C.POINTER(Coolant), C.c_double, _dp, _dp]
+        L.pdr_escape_probability.argtypes = [C.POINTER(Coolant), C.c_double, C.c_double, C.c_double, C.c_double,
+                                             C.c_double, C.c_double, C.c_int, C.c_int, C.POINTER(C.c_int),
+                                             C.POINTER(C.c_longlong), _dp, C.POINTER(C.c_longlong), _dp, _dp, _dp,
+                                             _dp, _dp, _dp, _dp]
+        L.pdr_escape_probability.restype = C.c_double
+        L.pdr_solvlevpop.argtypes = [C.c_int, _dp, C.c_double, _dp]
+        L.pdr_lte_populations.argtypes = [C.POINTER(Coolant), C.c_double, C.c_double, _dp]
+        L.pdr_dust_temperature.argtypes = [C.c_double, C.c_int, _dp]
+        L.pdr_dust_temperature.restype = C.c_double
+        L._therm_ready = True
+    return L
+
+
+class CoolantTables:
+    """a pdrchem.lamda.Coolant held in the C layout the oracle reads"""
+
+    def __init__(self, cool):
+        self.cool = cool
+        self.arrays = cool.fortran_arrays()
+        E, g, A, B, F, TT, col = self.arrays
+        self.s = Coolant(cool.nlev, cool.ntemp, *[a.ctypes.data for a in (E, g, A, B, F, TT, col)])
+
+
+def find_ccoeff(tab: CoolantTables, T, dens):
+    n = tab.cool.nlev
+    out = np.zeros(n * n)
+    _thermlib().pdr_find_ccoeff(C.byref(tab.s), float(T), np.ascontiguousarray(dens, dtype=np.float64), out)
+    return out.reshape(n, n, order="F")
+
+
+def lvg_point(tab: CoolantTables, T, Td, rho, metallicity, v_turb, g2d, dens, xspec, pop, chains, pseudo_ray=-1):
+    """Level populations of one point and coolant.  chains: per ray a tuple
+    (pops [m+1, nlev] with row 0 the point's own, steps [m] in pc).  Returns a
+    dict with C, transition, cooling, line, tau, beta (nlev x nlev [x nrays],
+    pairs i > j filled) and solution."""
+    L = _thermlib()
+    n = tab.cool.nlev
+    nrays = len(chains)
+    epray = np.array([len(s) for _, s in chains], dtype=np.int32)
+    pop_off = np.zeros(nrays, dtype=np.int64)
+    step_off = np.zeros(nrays, dtype=np.int64)
+    for j in range(1, nrays):
+        pop_off[j] = pop_off[j - 1] + epray[j - 1] + 1
+        step_off[j] = step_off[j - 1] + epray[j - 1]
+    cp = np.ascontiguousarray(np.concatenate([np.asarray(p, dtype=np.float64).reshape(-1, n) for p, _ in chains]))
+    st = np.ascontiguousarray(np.concatenate([np.asarray(s, dtype=np.float64) for _, s in chains] + [np.zeros(1)]))
+    Cc = find_ccoeff(tab, T, dens)
+    Cf = np.ascontiguousarray(Cc.ravel(order="F"))
+    tr = np.zeros(n * n)
+    line = np.zeros(n * n)
+    tau = np.zeros(n * n * nrays)
+    beta = np.zeros(n * n * nrays)
+    popv = np.ascontiguousarray(pop, dtype=np.float64)
+    cool = L.pdr_escape_probability(C.byref(tab.s), float(Td), float(T), float(v_turb), float(rho), float(metallicity),
+                                    float(g2d), nrays, int(pseudo_ray), epray.ctypes.data_as(C.POINTER(C.c_int)),
+                                    pop_off.ctypes.data_as(C.POINTER(C.c_longlong)), cp,
+                                    step_off.ctypes.data_as(C.POINTER(C.c_longlong)), st, popv, Cf, tr, line, tau,
+                                    beta)
+    sol = np.zeros(n)
+    L.pdr_solvlevpop(n, tr, float(xspec * rho), sol)
+    F = lambda a, *sh: a.reshape(*sh, order="F")
+    return {"C": Cc, "transition": F(tr, n, n), "cooling": cool, "line": F(line, n, n),
+            "tau": F(tau, n, n, nrays), "beta": F(beta, n, n, nrays), "solution": sol}
+
+
+def lte_populations(tab: CoolantTables, T, density):
+    out = np.zeros(tab.cool.nlev)
+    _thermlib().pdr_lte_populations(C.byref(tab.s), float(T), float(density), out)
+    return out
+
+
+def dust_temperature(uv, rad_surface):
+    rs = np.ascontiguousarray(rad_surface, dtype=np.float64)
+    return float(_thermlib().pdr_dust_temperature(float(uv), len(rs), rs))
+
+
+def have_ref_lvg() -> bool:
+    return all(os.path.exists(os.path.join(REF_BIN_DIR, f"ref_lvg_{k}")) for k in ("1d", "3d"))
+
+
+def _ref_lvg_run(variant, text):
+    r = subprocess.run([os.path.join(REF_BIN_DIR, f"ref_lvg_{variant}")], input=text, capture_output=True, text=True,
+                       timeout=600, cwd="/tmp")
+    if r.returncode != 0:
+        raise RuntimeError(f"reference level-population driver failed: {r.stdout[-1000:]} {r.stderr[-1000:]}")
+    vals = []
+    for line in r.stdout.splitlines():
+        try:
+            vals.append(float(line.strip().replace("D", "E")))
+        except ValueError:
+            continue
+    return np.array(vals)
+
+
+def ref_coolant_tables(path, nlev, ntemp):
+    """READINPUT of the reference on a coolant file: (E, g, A, B, F, TT,
+    col[7]) with col in the partner order H2, pH2, oH2, e, H, He, H+"""
+    v = _ref_lvg_run("3d", f"1\n{path}\n{nlev} {ntemp}\n")
+    k = 0
+
+    def take(m, *sh):
+        nonlocal k
+        a = v[k:k + m].reshape(*sh, order="F") if sh else v[k:k + m]
+        k += m
+        return a
+    n2, n3 = nlev * nlev, nlev * nlev * ntemp
+    E, g = take(nlev), take(nlev)
+    A, B, F = take(n2, nlev, nlev), take(n2, nlev, nlev), take(n2, nlev, nlev)
+    TT = take(7 * ntemp, 7, ntemp)
+    col = np.stack([take(n3, nlev, nlev, ntemp) for _ in range(7)])
+    return E, g, A, B, F, TT, col
+
+
+def ref_lvg(path, nlev, ntemp, coolant_id, cases, pseudo_1d):
+    """The reference's FIND_CCOEFF, escape_probability and solvlevpop on
+    cases: dicts with T, Td, rho, metallicity, v_turb, g2d, dens (5 partner
+    abundances, fractional), xspec, pop, and rays: per ray (epoints [m+1, 3],
+    pops [m+1, nlev]) with row 0 the point itself.  Returns per case the dict
+    lvg_point returns."""
+    lines = ["2", str(path), f"{nlev} {ntemp} {coolant_id} {len(cases)}"]
+    for c in cases:
+        nrays = len(c["rays"])
+        lines.append(" ".join(repr(float(x)) for x in (c["T"], c["Td"], c["rho"], c["metallicity"], c["v_turb"],
+                                                        c["g2d"])) + f" {nrays}")
+        lines.append(" ".join(repr(float(x)) for x in list(c["dens"]) + [c["xspec"]]))
+        lines.append(" ".join(repr(float(x)) for x in c["pop"]))
+        for ept, pops in c["rays"]:
+            lines.append(str(len(ept) - 1))
+            for i in range(len(ept)):
+                lines.append(" ".join(repr(float(x)) for x in list(ept[i]) + list(pops[i])))
+    v = _ref_lvg_run("1d" if pseudo_1d else "3d", "\n".join(lines) + "\n")
+    out, k, n2 = [], 0, nlev * nlev
+    for c in cases:
+        nrays = len(c["rays"])
+        d = {}
+        d["C"] = v[k:k + n2].reshape(nlev, nlev, order="F"); k += n2
+        d["transition"] = v[k:k + n2].reshape(nlev, nlev, order="F"); k += n2
+        d["cooling"] = v[k]; k += 1
+        d["line"] = v[k:k + n2].reshape(nlev, nlev, order="F"); k += n2
+        d["tau"] = v[k:k + n2 * nrays].reshape(nlev, nlev, nrays, order="F"); k += n2 * nrays
+        d["beta"] = v[k:k + n2 * nrays].reshape(nlev, nlev, nrays, order="F"); k += n2 * nrays
+        d["solution"] = v[k:k + nlev]; k += nlev
+        out.append(d)
+    return out
+
+
+def ref_dust_temperatures(uv, rad_surface, av_fac=AV_FAC):
+    rad_surface = np.asarray(rad_surface, dtype=np.float64)
+    n, nrays = rad_surface.shape
+    lines = ["3", f"{n} {nrays} {av_fac!r}"]
+    for p in range(n):
+        lines.append(" ".join(repr(float(x)) for x in [uv[p]] + list(rad_surface[p])))
+    return _ref_lvg_run("3d", "\n".join(lines) + "\n")

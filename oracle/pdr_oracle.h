@@ -183,6 +183,31 @@ int pdr_integrate_generic(int n, pdr_rhs_fn f, pdr_jac_fn jac, void *ud,
 void pdr_heating(const pdr_phys *ph, int net, const int *idx, double n, double T, double Td, double uv,
                  const double *ab, const double *rate, double *heat);
 
+/* level populations and line cooling (thermal.c).  A coolant's tables in
+ * the reference's column-major layout (read_input.F90): X(i, j) of an
+ * nlev x nlev matrix at [i + j*nlev] (0-based), TEMPERATURES(p, k) at
+ * [p + 7k], partner p's rate COL(i, j, k) at col[p*nlev*nlev*ntemp + i +
+ * j*nlev + k*nlev*nlev]; partners H2, pH2, oH2, e, H, He, H+. */
+typedef struct {
+  int nlev, ntemp;
+  const double *E, *g, *A, *B, *F;
+  const double *temps, *col;
+} pdr_coolant;
+/* FIND_CCOEFF (find_Ccoeff.F90): dens = n(H), n(H+), n(e), n(He), n(H2) */
+void pdr_find_ccoeff(const pdr_coolant *c, double T, const double *dens, double *C);
+/* escape_probability (escape_probability.F90); see thermal.c */
+double pdr_escape_probability(const pdr_coolant *c, double Td, double Tg, double v_turb, double density,
+                              double metallicity, double g2d, int nrays, int pseudo_ray, const int *epray,
+                              const long long *pop_off, const double *chain_pop, const long long *step_off,
+                              const double *step, const double *pop, const double *C, double *transition,
+                              double *line, double *tau, double *beta);
+/* solvlevpop (solvlevpop.F90); returns 1 on a singular matrix */
+int pdr_solvlevpop(int n, const double *transition, double density, double *solution);
+/* LTE populations (3DPDR.F90:1848-1898) */
+void pdr_lte_populations(const pdr_coolant *c, double T, double density, double *pop);
+/* CALCULATE_DUST_TEMPERATURES for one point (dust_t.F90:44-97) */
+double pdr_dust_temperature(double uv, int nrays, const double *rad_surface);
+
 /* ray tracer (raytrace.c) */
 #define PDR_ORDER_HEAPSORT 0  /* the reference's heapsort order (ties as it leaves them) */
 #define PDR_ORDER_STABLE 1    /* strict (distance, point index) order */
