@@ -41,6 +41,7 @@
 #include <vector>
 
 #include "../../include/pdrchem.h"
+#include "pdr_internal.h"
 
 #pragma clang fp contract(off)
 
@@ -521,6 +522,30 @@ inline unsigned blocks_for(long long n, int bs) { return (unsigned)((n + bs - 1)
 }  // namespace
 
 // ---------------------------------------------------------------------------
+// thermal-balance state of a cloud (pdrchem_cloud_thermal_*)
+// ---------------------------------------------------------------------------
+struct CloudThermal {
+  pdrchem_coolant_tables *tables = nullptr;
+  pdrchem::LvgTables tab{};
+  pdrchem_thermal_params prm{};
+  int heat_index[13] = {};
+  int nlevtot = 0;
+  double *d_pop = nullptr, *d_sol = nullptr, *d_S = nullptr;  // [npts][nlevtot], S [npts][nrays][nlevtot]
+  double *d_cool = nullptr, *d_total = nullptr;               // [npts][4], [npts]
+  double *d_lines = nullptr, *d_heat = nullptr;               // [npts][17], [npts][12]
+  double *d_fmean = nullptr, *d_fratio = nullptr;             // [npts]
+  double *c_heat = nullptr, *c_uv = nullptr;                  // compact heating terms, field
+  pdrchem::ThermalPoint *d_state = nullptr;
+  unsigned char *d_lc = nullptr, *d_cconv = nullptr, *d_skip = nullptr;
+  unsigned *d_counts = nullptr;                               // level-converged, per coolant x4, thermally converged
+  // the reference's loop variables (3DPDR.F90:584-600)
+  int iteration = 0, levpop_iteration = 0;
+  bool level_conv = false, first_time = true, started = false, done = false;
+  int pct[4] = {0, 0, 0, 0};                                  // CII, CI, OI, CO percentages as integers
+  ~CloudThermal() { pdrchem_coolant_tables_destroy(tables); }
+};
+
+// ---------------------------------------------------------------------------
 // the cloud object
 // ---------------------------------------------------------------------------
 struct pdrchem_cloud {
@@ -552,6 +577,7 @@ struct pdrchem_cloud {
   double av_fac = 0.0, uv_fac = 0.0;
   bool have_ab = false;
   pdrchem_server *server = nullptr;  // when set, solves go through this request loop
+  CloudThermal *th = nullptr;        // thermal balance, when initialised
   std::string err;
   std::vector<void *> allocs;
 
@@ -565,6 +591,7 @@ struct pdrchem_cloud {
   }
   ~pdrchem_cloud() {
     if (stream) hipStreamSynchronize(stream);
+    delete th;
     for (void *q : allocs) hipFree(q);
     if (ev0) hipEventDestroy(ev0);
     if (ev1) hipEventDestroy(ev1);
@@ -1006,6 +1033,53 @@ int pdrchem_cloud_iterate(pdrchem_cloud *c, double t_end_years, double conv_tol,
   return 0;
 }
 
+// thermal-balance fields of pdrchem_cloud_get
+static int thermal_get(pdrchem_cloud *c, int field, double *out) {
+  CloudThermal *th = c->th;
+  if (!th) {
+    c->err = "thermal field requested before pdrchem_cloud_thermal_init";
+    return -1;
+  }
+  const size_t n = c->npts;
+  hipStream_t s = c->stream;
+  bool good = true;
+  switch (field) {
+    case PDRCHEM_CLOUD_HEATING:
+      good = ok(hipMemcpyAsync(out, th->d_heat, n * 12 * 8, hipMemcpyDeviceToHost, s), "copy");
+      break;
+    case PDRCHEM_CLOUD_LINES:
+      good = ok(hipMemcpyAsync(out, th->d_lines, n * pdrchem::LvgArgs::kLines * 8, hipMemcpyDeviceToHost, s), "copy");
+      break;
+    case PDRCHEM_CLOUD_POPULATIONS:
+      good = ok(hipMemcpyAsync(out, th->d_pop, n * th->nlevtot * 8, hipMemcpyDeviceToHost, s), "copy");
+      break;
+    case PDRCHEM_CLOUD_COOLING: {
+      std::vector<double> cool(n * 4), tot(n);
+      good = ok(hipMemcpyAsync(cool.data(), th->d_cool, n * 4 * 8, hipMemcpyDeviceToHost, s), "copy") &&
+             ok(hipMemcpyAsync(tot.data(), th->d_total, n * 8, hipMemcpyDeviceToHost, s), "copy") &&
+             ok(hipStreamSynchronize(s), "sync");
+      for (size_t p = 0; good && p < n; p++) {
+        for (int k = 0; k < 4; k++) out[p * 5 + k] = cool[p * 4 + k];
+        out[p * 5 + 4] = tot[p];
+      }
+      break;
+    }
+    default: {
+      std::vector<pdrchem::ThermalPoint> st(n);
+      good = ok(hipMemcpyAsync(st.data(), th->d_state, n * sizeof(pdrchem::ThermalPoint), hipMemcpyDeviceToHost, s),
+                "copy") &&
+             ok(hipStreamSynchronize(s), "sync");
+      for (size_t p = 0; good && p < n; p++)
+        out[p] = field == PDRCHEM_CLOUD_PREV_TEMPERATURE ? st[p].Tprev : (double)st[p].converged;
+    }
+  }
+  if (!good || !ok(hipStreamSynchronize(s), "sync")) {
+    c->fail("get");
+    return -1;
+  }
+  return 0;
+}
+
 int pdrchem_cloud_get(pdrchem_cloud *c, int field, double *out) {
   if (!c || !out) return -1;
   c->err.clear();
@@ -1019,6 +1093,14 @@ int pdrchem_cloud_get(pdrchem_cloud *c, int field, double *out) {
     case PDRCHEM_CLOUD_COLUMNS: src = c->d_cols; cnt = n * nr * 5; break;
     case PDRCHEM_CLOUD_TEMPERATURE: src = c->d_T; cnt = n; break;
     case PDRCHEM_CLOUD_UVFIELD: src = c->d_uv; cnt = n; break;
+    case PDRCHEM_CLOUD_DUST_TEMPERATURE: src = c->d_Td; cnt = n; break;
+    case PDRCHEM_CLOUD_HEATING:
+    case PDRCHEM_CLOUD_LINES:
+    case PDRCHEM_CLOUD_POPULATIONS:
+    case PDRCHEM_CLOUD_COOLING:
+    case PDRCHEM_CLOUD_PREV_TEMPERATURE:
+    case PDRCHEM_CLOUD_THERMAL_CONVERGED:
+      return thermal_get(c, field, out);
     default:
       c->err = "unknown field";
       return -1;
@@ -1080,6 +1162,263 @@ int pdrchem_cloud_evaluation_points(pdrchem_cloud *c, int *count, long long *off
     c->fail("evaluation points");
     return -1;
   }
+  return 0;
+}
+
+int pdrchem_cloud_thermal_init(pdrchem_cloud *c, const pdrchem_coolant *coolants, const int *slots,
+                               const int *heat_index, const pdrchem_thermal_params *prm) {
+  using namespace pdrchem;
+  if (!c) return -1;
+  c->err.clear();
+  t_err.clear();
+  if (!coolants || !slots || !heat_index || !prm || (c->net != 0 && c->net != 1)) {
+    c->err = "bad argument to pdrchem_cloud_thermal_init (the reduced or full network, four coolants, 9 slots, "
+             "13 heating indices)";
+    return -1;
+  }
+  for (int k = 0; k < 9; k++)
+    if (slots[k] < 0 || slots[k] >= c->nspec) {
+      c->err = "abundance slot out of range";
+      return -1;
+    }
+  if (prm->pseudo_ray < -1 || prm->pseudo_ray >= c->nrays || !(prm->Tmin > 0) || !(prm->Tmax > prm->Tmin) || !(prm->Fcrit > 0)) {
+    c->err = "bad thermal parameters";
+    return -1;
+  }
+  delete c->th;
+  c->th = new CloudThermal();
+  CloudThermal *th = c->th;
+  th->prm = *prm;
+  for (int k = 0; k < 13; k++) th->heat_index[k] = heat_index[k];
+  th->tables = pdrchem_coolant_tables_create(coolants, slots);
+  if (!th->tables) {
+    c->err = pdrchem_last_error();
+    delete c->th;
+    c->th = nullptr;
+    return -1;
+  }
+  th->tab = *coolant_tables_dev(th->tables);
+  const int off = th->tab.nlevtot;
+  th->nlevtot = off;
+  hipStream_t s = c->stream;
+  const size_t n = c->npts;
+  if (!c->alloc(&th->d_pop, n * off, "populations") || !c->alloc(&th->d_sol, n * off, "populations") ||
+      !c->alloc(&th->d_S, n * c->nrays * off, "level columns") || !c->alloc(&th->d_cool, n * 4, "cooling") ||
+      !c->alloc(&th->d_total, n, "cooling") || !c->alloc(&th->d_lines, n * LvgArgs::kLines, "lines") ||
+      !c->alloc(&th->d_heat, n * 12, "heating") || !c->alloc(&th->d_fmean, n, "balance") ||
+      !c->alloc(&th->d_fratio, n, "balance") || !c->alloc(&th->c_heat, n * 12, "heating") ||
+      !c->alloc(&th->c_uv, n, "field") || !c->alloc(&th->d_state, n, "thermal state") ||
+      !c->alloc(&th->d_lc, n, "flags") || !c->alloc(&th->d_cconv, n * 4, "flags") ||
+      !c->alloc(&th->d_skip, n, "flags") || !c->alloc(&th->d_counts, 8, "counts") ||
+      !ok(hipMemsetAsync(th->d_cool, 0, n * 4 * 8, s), "memset") ||
+      !ok(hipMemsetAsync(th->d_total, 0, n * 8, s), "memset") ||
+      !ok(hipMemsetAsync(th->d_lines, 0, n * LvgArgs::kLines * 8, s), "memset") ||
+      !ok(hipMemsetAsync(th->d_heat, 0, n * 12 * 8, s), "memset") ||
+      !ok(hipMemsetAsync(th->d_lc, 0, n, s), "memset")) {
+    c->fail("thermal state");
+    return -1;
+  }
+  // dust temperatures from the field (CALCULATE_DUST_TEMPERATURES), the
+  // temperature bracket of GUESS_TEMP (3DPDR.F90:1702-1716)
+  if ((prm->dust && launch_dust_temperature(c->npts, c->nrays, c->d_G, c->d_uv, c->d_Td, s) != 0) ||
+      launch_init_thermal_state(c->npts, c->d_T, prm->Tmin, prm->Tmax, th->d_state, s) != 0 ||
+      !ok(hipStreamSynchronize(s), "sync")) {
+    c->fail("thermal set-up");
+    return -1;
+  }
+  return 0;
+}
+
+// one ITERATION of the reference's outer loop (3DPDR.F90:590-1241)
+int pdrchem_cloud_thermal_iterate(pdrchem_cloud *c, double t_end_years, int precision, pdrchem_thermal_stats *out) {
+  using namespace pdrchem;
+  if (!c) return -1;
+  c->err.clear();
+  t_err.clear();
+  CloudThermal *th = c->th;
+  if (!th || !c->have_ab) {
+    c->err = "pdrchem_cloud_thermal_iterate needs pdrchem_cloud_thermal_init and abundances";
+    return -1;
+  }
+  if (!(t_end_years > 0) || (precision != PDRCHEM_FP64 && precision != PDRCHEM_MIXED)) {
+    c->err = "bad argument to pdrchem_cloud_thermal_iterate";
+    return -1;
+  }
+  pdrchem_thermal_stats st{};
+  if (th->done) {
+    st.done = 1;
+    st.iteration = th->iteration;
+    if (out) *out = st;
+    return 0;
+  }
+  const auto t0 = std::chrono::steady_clock::now();
+  hipStream_t s = c->stream;
+  const int n = c->npts;
+  const LvgTables &t = th->tab;
+  auto fail = [&](const char *what) {
+    c->fail(what);
+    return -1;
+  };
+  if (!th->started) {
+    // LTE populations of every point before the first iteration (3DPDR.F90:520-552)
+    if (launch_lte(t, n, c->d_T, c->d_ab, c->nspec, c->d_rho, nullptr, th->d_pop, s) != 0) return fail("LTE");
+    th->started = true;
+  }
+  th->iteration++;
+  th->levpop_iteration++;
+  st.iteration = th->iteration;
+  if (th->iteration > 1 && th->levpop_iteration == 1) {
+    // three chemistry iterations of the points not thermally converged,
+    // then LTE populations of those not converged in levels either
+    // (3DPDR.F90:602-686)
+    for (int k = 0; k < 3; k++) {
+      if (launch_skip_flags(n, th->d_state, th->d_lc, 0, c->d_conv, s) != 0) return fail("flags");
+      pdrchem_cloud_stats cs{};
+      if (pdrchem_cloud_iterate(c, t_end_years, 0.0, precision, &cs) != 0) return -1;
+      st.chemistry_solves += cs.solved;
+      st.failed += cs.failed;
+      st.nst += cs.nst;
+      st.nfe += cs.nfe;
+      st.nje += cs.nje;
+      st.nsetups += cs.nsetups;
+      st.nni += cs.nni;
+    }
+    if (launch_skip_flags(n, th->d_state, th->d_lc, 1, th->d_skip, s) != 0 ||
+        launch_lte(t, n, c->d_T, c->d_ab, c->nspec, c->d_rho, th->d_skip, th->d_pop, s) != 0)
+      return fail("LTE");
+  } else {
+    // level populations by escape probability (3DPDR.F90:688-850)
+    const int p1 = th->prm.pseudo_ray;
+    if (launch_skip_flags(n, th->d_state, th->d_lc, 1, th->d_skip, s) != 0 ||
+        launch_level_columns(n, c->nrays, p1 >= 0 ? p1 : 0, p1 >= 0 ? 1 : c->nrays, c->d_offs, c->d_count,
+                             c->d_proj, c->d_step, th->d_pop, th->nlevtot, th->d_S, s) != 0)
+      return fail("level columns");
+    LvgArgs a{};
+    a.npts = n;
+    a.nspec = c->nspec;
+    a.nrays = c->nrays;
+    a.pseudo_ray = p1;
+    a.T = c->d_T;
+    a.Td = c->d_Td;
+    a.rho = c->d_rho;
+    a.ab = c->d_ab;
+    a.pop = th->d_pop;
+    a.S = th->d_S;
+    a.skip = th->d_skip;
+    a.v_turb = c->phys.v_turb;
+    a.metallicity = c->phys.metallicity;
+    a.g2d = c->phys.g2d;
+    a.co_fix = th->prm.co_fix;
+    a.co_average_co = th->pct[0] == 100 && th->pct[1] == 100 && th->pct[2] == 100;
+    a.levpop_iteration = th->levpop_iteration;
+    a.solution = th->d_sol;
+    a.cool = th->d_cool;
+    a.lines = th->d_lines;
+    for (int k = 0; k < 4; k++)
+      if (launch_lvg(t, a, k, s) != 0) return fail("level populations");
+    if (launch_total_cool(n, th->d_cool, th->d_total, s) != 0) return fail("cooling");
+    st.lvg = 1;
+  }
+  // heating and the temperature update of the points not converged
+  // (3DPDR.F90:852-1057)
+  {
+    if (launch_skip_flags(n, th->d_state, th->d_lc, 0, th->d_skip, s) != 0) return fail("flags");
+    hipLaunchKernelGGL(activity_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, th->d_skip, c->d_active);
+    hipcub::CountingInputIterator<int> it(0);
+    int nact = 0;
+    if (!ok(hipGetLastError(), "flags") ||
+        !ok(hipcub::DeviceSelect::Flagged(c->d_cub, c->cub_bytes, it, c->d_active, c->d_idx, c->d_nsel, n, s),
+            "select") ||
+        !ok(hipMemcpyAsync(&nact, c->d_nsel, sizeof(int), hipMemcpyDeviceToHost, s), "copy") ||
+        !ok(hipStreamSynchronize(s), "sync"))
+      return fail("active set");
+    st.thermal_points = nact;
+    if (nact > 0) {
+      const int nr = c->nrays, ncol = nr * 5;
+      auto gather = [&](double *dst, const double *src, int w) {
+        hipLaunchKernelGGL(rows_kernel<false>, dim3(blocks_for((long long)nact * w, 256)), dim3(256), 0, s, dst, src,
+                           c->d_idx, nact, w);
+      };
+      gather(c->c_T, c->d_T, 1);
+      gather(c->c_Td, c->d_Td, 1);
+      gather(c->c_rho, c->d_rho, 1);
+      gather(th->c_uv, c->d_uv, 1);
+      gather(c->c_G, c->d_G, nr);
+      gather(c->c_Av, c->d_Av, nr);
+      gather(c->c_cols, c->d_cols, ncol);
+      gather(c->c_ab, c->d_ab, c->nspec);
+      if (c->d_zeta) gather(c->c_zeta, c->d_zeta, 1);
+      hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(1), 0, s, (unsigned *)c->d_neg);
+      if (!ok(hipGetLastError(), "gather") ||
+          pdrchem_rates_device_zeta(c->net, &c->phys, c->c_T, c->c_Td, nr, c->c_G, c->c_Av, c->c_cols,
+                                    c->d_zeta ? c->c_zeta : nullptr, nact, c->c_rate, c->d_neg, s) != 0)
+        return fail("rates");
+      const PhysParams ph{c->phys.zeta, c->phys.v_turb, c->phys.omega, c->phys.grain_radius, c->phys.metallicity,
+                          c->phys.g2d};
+      if (launch_heating(c->net, ph, th->heat_index, nact, c->c_rho, c->c_T, c->c_Td, th->c_uv, c->c_ab, c->c_rate,
+                         c->d_zeta ? c->c_zeta : nullptr, th->c_heat, s) != 0)
+        return fail("heating");
+      ThermalArgs ta{};
+      ta.nact = nact;
+      ta.idx = c->d_idx;
+      ta.heat_c = th->c_heat;
+      ta.heat = th->d_heat;
+      ta.total_cool = th->d_total;
+      ta.fmean = th->d_fmean;
+      ta.fratio = th->d_fratio;
+      ta.level_conv = th->level_conv;
+      ta.first_time = th->first_time;
+      ta.temp_fix = th->prm.temp_fix;
+      ta.Tmin = th->prm.Tmin;
+      ta.Tmax = th->prm.Tmax;
+      ta.Fcrit = th->prm.Fcrit;
+      ta.Tdiff = th->prm.Tdiff;
+      ta.state = th->d_state;
+      ta.T = c->d_T;
+      ta.level_converged = th->d_lc;
+      if (launch_thermal_update(ta, s) != 0) return fail("temperature update");
+    }
+  }
+  // convergence (3DPDR.F90:1064-1099)
+  if (th->level_conv && th->first_time) th->first_time = false;
+  unsigned counts[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, s, (unsigned long long *)th->d_counts, 4);
+  if (!ok(hipGetLastError(), "counts") || launch_count_converged(n, th->d_state, th->d_counts + 5, s) != 0 ||
+      !ok(hipMemcpyAsync(&counts[5], th->d_counts + 5, 4, hipMemcpyDeviceToHost, s), "copy") ||
+      !ok(hipStreamSynchronize(s), "sync"))
+    return fail("convergence count");
+  st.thermally_converged = (int)counts[5];
+  if (th->level_conv) th->level_conv = false;
+  if ((int)counts[5] == n) {
+    th->done = true;
+    st.done = 1;
+  } else {
+    // level populations: relative change of every coolant's populations
+    // (3DPDR.F90:1086-1161); then the counts of label 1 and the update of the
+    // populations (3DPDR.F90:1175-1228)
+    if (launch_relch(t, n, c->d_ab, c->nspec, th->d_sol, th->d_pop, th->d_lc, th->d_cconv, th->d_counts, s) != 0 ||
+        !ok(hipMemcpyAsync(counts, th->d_counts, 5 * 4, hipMemcpyDeviceToHost, s), "copy") ||
+        !ok(hipStreamSynchronize(s), "sync"))
+      return fail("level convergence");
+    st.level_converged = (int)counts[0];
+    for (int k = 0; k < 4; k++) st.coolant_converged[k] = (int)counts[1 + k];
+    if ((int)counts[0] == n) {
+      th->levpop_iteration = 0;
+      th->level_conv = true;
+    }
+    for (int k = 0; k < 4; k++) th->pct[k] = (int)(100.0 * (double)counts[1 + k] / (double)n);
+    const double levpop_pct = 100.0 * (double)counts[0] / (double)n;
+    if ((int)levpop_pct >= 100)
+      hipLaunchKernelGGL(fill_u8_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, th->d_lc, n);
+    if (!ok(hipGetLastError(), "flags") ||
+        launch_copy((long long)n * th->nlevtot, th->d_sol, th->d_pop, s) != 0 || !ok(hipStreamSynchronize(s), "sync"))
+      return fail("population update");
+  }
+  st.levpop_iteration = th->levpop_iteration;
+  st.level_conv = th->level_conv ? 1 : 0;
+  st.first_time = th->first_time ? 1 : 0;
+  st.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  if (out) *out = st;
   return 0;
 }
 

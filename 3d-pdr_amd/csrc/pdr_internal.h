@@ -5,6 +5,8 @@
 
 #include <string>
 
+struct pdrchem_coolant_tables;
+
 namespace pdrchem {
 
 enum NetId { NET_REDUCED = 0, NET_FULL = 1, NET_MYNETWORK = 2, NET_COUNT = 3 };
@@ -102,5 +104,95 @@ struct RateRequest {
   int *neg;               // count of points with a negative rate (device int)
 };
 int launch_rates(int net, const RateRequest &r, hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// thermal balance (thermal.hip): coolant tables, level populations, the
+// temperature update
+// ---------------------------------------------------------------------------
+// one coolant's tables on the device, in the reference's column-major layout
+// (read_input.F90): X(i, j) at [i + j*nlev], TEMPERATURES(p, k) at
+// [p + 7k], partner p's COL(i, j, k) at col[p*nlev*nlev*ntemp + i + j*nlev +
+// k*nlev*nlev]; partners H2, pH2, oH2, e, H, He, H+
+struct CoolantDev {
+  int nlev, ntemp;
+  int off;  // first level of this coolant in the per-point population rows
+  const double *E, *g, *A, *B, *F, *temps, *col;
+};
+struct LvgTables {
+  static constexpr int kMaxLev = 41;
+  int ncool, nlevtot;
+  CoolantDev c[4];   // CII, CI, OI, CO
+  int species[4];    // 0-based abundance slots of C+, C, O, CO
+  int partner[5];    // 0-based abundance slots of H, H+, e-, He, H2
+};
+struct LvgArgs {
+  static constexpr int kLines = 17;  // the lines of the reference's .line.fin (3DPDR.F90:1407-1416)
+  int npts, nspec, nrays, pseudo_ray;  // pseudo_ray >= 0: PSEUDO_1D, optical depth along that ray only
+  const double *T, *Td, *rho, *ab;
+  const double *pop;                   // [npts][nlevtot] current populations
+  const double *S;                     // [npts][nrays][nlevtot] level columns
+  const unsigned char *skip;           // points not solved (their outputs are left)
+  double v_turb, metallicity, g2d;
+  int co_fix, co_average_co, levpop_iteration;
+  double *solution;                    // [npts][nlevtot]
+  double *cool;                        // [npts][4]
+  double *lines;                       // [npts][kLines]
+  // index of (upper, lower) of coolant ci among the output lines, or -1:
+  // CII 2-1; CI 2-1, 3-1, 3-2; OI 2-1, 3-1, 3-2; CO J+1 -> J for J = 0..9
+  __host__ __device__ static int line_slot(int ci, int il, int jl) {
+    if (ci == 0) return (il == 1 && jl == 0) ? 0 : -1;
+    if (ci == 1 || ci == 2) {
+      const int b = ci == 1 ? 1 : 4;
+      if (il == 1 && jl == 0) return b;
+      if (il == 2 && jl == 0) return b + 1;
+      if (il == 2 && jl == 1) return b + 2;
+      return -1;
+    }
+    return (il == jl + 1 && jl < 10) ? 7 + jl : -1;
+  }
+};
+// per-point state of the temperature search (3DPDR.F90 arrays of the same
+// names): previous temperature, bracket, last change ('H' heated, 'C'
+// cooled, 0 none), flags
+struct ThermalPoint {
+  double Tprev, Tlow, Thigh;
+  char prev;
+  unsigned char binarychop, converged, doleveltmin;
+};
+struct ThermalArgs {
+  int nact;
+  const int *idx;              // compact index -> point
+  const double *heat_c;        // [nact][12] heating terms of the compact points
+  double *heat;                // [npts][12]
+  const double *total_cool;    // [npts]
+  double *fmean, *fratio;      // [npts]
+  int level_conv, first_time, temp_fix;
+  double Tmin, Tmax, Fcrit, Tdiff;
+  ThermalPoint *state;         // [npts]
+  double *T;                   // [npts] gas temperature
+  unsigned char *level_converged;  // [npts]
+};
+int launch_dust_temperature(int npts, int nrays, const double *G, const double *uv, double *Td, hipStream_t s);
+int launch_lte(const LvgTables &t, int npts, const double *T, const double *ab, int nspec, const double *rho,
+               const unsigned char *skip, double *pop, hipStream_t s);
+int launch_level_columns(int npts, int nrays, int ray0, int nray_sel, const long long *offs, const int *count,
+                         const int *proj, const double *step, const double *pop, int nlevtot, double *S,
+                         hipStream_t s);
+int launch_lvg(const LvgTables &t, const LvgArgs &a, int coolant, hipStream_t s);
+int launch_total_cool(int npts, const double *cool, double *total, hipStream_t s);
+int launch_thermal_update(const ThermalArgs &a, hipStream_t s);
+int launch_relch(const LvgTables &t, int npts, const double *ab, int nspec, const double *sol, const double *pop,
+                 unsigned char *level_converged, unsigned char *cconv, unsigned *counts, hipStream_t s);
+int launch_count_converged(int npts, const ThermalPoint *st, unsigned *count, hipStream_t s);
+int launch_copy(long long n, const double *src, double *dst, hipStream_t s);
+int launch_init_thermal_state(int npts, const double *T, double Tmin, double Tmax, ThermalPoint *st, hipStream_t s);
+int launch_skip_flags(int npts, const ThermalPoint *st, const unsigned char *lc, int with_levels,
+                      unsigned char *skip, hipStream_t s);
+// the device tables behind a pdrchem_coolant_tables handle (thermal.hip)
+const LvgTables *coolant_tables_dev(const ::pdrchem_coolant_tables *t);
+// heating terms (heating.hip): see pdrchem_heating_device
+int launch_heating(int net, const PhysParams &ph, const int *index, int npts, const double *n, const double *T,
+                   const double *Td, const double *uv, const double *ab, const double *rate, const double *zeta,
+                   double *heat, hipStream_t s);
 
 }  // namespace pdrchem

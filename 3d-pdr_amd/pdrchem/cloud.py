@@ -20,6 +20,8 @@ from . import lib
 from .netgen import Network
 
 ABUNDANCE, AV, G, COLUMNS, TEMPERATURE, UVFIELD = range(6)
+(DUST_TEMPERATURE, HEATING, COOLING, LINES, POPULATIONS, PREV_TEMPERATURE,
+ THERMAL_CONVERGED) = range(6, 13)
 SHIELD_SPECIES = ("H2", "HD", "CO", "C", "S")   # column order expected by the rate kernel
 ISO, UNI = 0, 1
 
@@ -132,9 +134,40 @@ class Cloud:
             raise lib.PdrChemError(self._err())
         return {k: getattr(st, k) for k, _ in st._fields_}
 
+    def thermal_init(self, coolants=None, params: Optional[lib.ThermalParams] = None) -> None:
+        """Turn on the thermal balance (THERMALBALANCE, GUESS_TEMP): the four
+        coolants (lamda.Coolant; default the stored CII, CI, OI, CO tables)
+        and the parameters (default params.dat's and the reference build's)."""
+        from . import lamda
+        L = lib.load()
+        cools = coolants if coolants is not None else lamda.load_coolants()
+        arr, keep = lib.coolant_structs(cools)
+        slots = lib.thermal_slots(self.net)
+        hidx = lib.heating_index(self.net)
+        self._tparams = params or lib.default_thermal_params()
+        ip = C.POINTER(C.c_int)
+        rc = L.pdrchem_cloud_thermal_init(self._h, arr, slots.ctypes.data_as(ip), hidx.ctypes.data_as(ip),
+                                          C.byref(self._tparams))
+        del keep
+        if rc != 0:
+            raise lib.PdrChemError(self._err())
+        self.nlevtot = sum(c.nlev for c in cools)
+
+    def thermal_iterate(self, t_end: float = 1e7, precision: int = lib.MIXED) -> dict:
+        """One iteration of the reference's outer loop (3DPDR.F90:590-1241)."""
+        st = lib.ThermalStats()
+        if lib.load().pdrchem_cloud_thermal_iterate(self._h, t_end, precision, C.byref(st)) != 0:
+            raise lib.PdrChemError(self._err())
+        out = {k: getattr(st, k) for k, _ in st._fields_}
+        out["coolant_converged"] = list(st.coolant_converged)
+        return out
+
     def get(self, field: int) -> np.ndarray:
         shapes = {ABUNDANCE: (self.npts, self.nspec), AV: (self.npts, self.nrays), G: (self.npts, self.nrays),
-                  COLUMNS: (self.npts, self.nrays, 5), TEMPERATURE: (self.npts,), UVFIELD: (self.npts,)}
+                  COLUMNS: (self.npts, self.nrays, 5), TEMPERATURE: (self.npts,), UVFIELD: (self.npts,),
+                  DUST_TEMPERATURE: (self.npts,), HEATING: (self.npts, 12), COOLING: (self.npts, 5),
+                  LINES: (self.npts, 17), POPULATIONS: (self.npts, getattr(self, "nlevtot", 0)),
+                  PREV_TEMPERATURE: (self.npts,), THERMAL_CONVERGED: (self.npts,)}
         out = np.zeros(shapes[field])
         if lib.load().pdrchem_cloud_get(self._h, field, out.ctypes.data_as(C.POINTER(C.c_double))) != 0:
             raise lib.PdrChemError(self._err())

@@ -35,6 +35,9 @@ EXPORTED_FUNCTIONS = [
     "pdrchem_set_tuned_config", "pdrchem_clear_tuning", "pdrchem_heating_device",
     "pdrchem_heating_device_zeta", "pdrchem_set_mxstep", "pdrchem_mxstep", "pdrchem_set_electron_policy",
     "pdrchem_electron_policy", "pdrchem_jacobian_device", "pdrchem_solve_device_noalloc",
+    "pdrchem_coolant_tables_create", "pdrchem_coolant_tables_destroy", "pdrchem_coolant_tables_levels",
+    "pdrchem_level_populations_device", "pdrchem_lte_populations_device", "pdrchem_dust_temperature_device",
+    "pdrchem_cloud_thermal_init", "pdrchem_cloud_thermal_iterate",
 ]
 EXPORTED_GLOBALS = [
     ("chemistry_module_mp_relative_abundance_tolerance_", C.c_double),
@@ -64,6 +67,34 @@ class CloudStats(C.Structure):
                 ("negative_rates", C.c_int), ("rates_ms", C.c_double), ("solve_ms", C.c_double),
                 ("nst", C.c_longlong), ("nfe", C.c_longlong), ("nje", C.c_longlong), ("nsetups", C.c_longlong),
                 ("nni", C.c_longlong)]
+
+
+class CoolantStruct(C.Structure):
+    """pdrchem_coolant: one coolant's tables (pointers into arrays the caller
+    keeps alive for the call)"""
+    _fields_ = [("nlev", C.c_int), ("ntemp", C.c_int)] + \
+        [(k, C.POINTER(C.c_double)) for k in ("energies", "weights", "A", "B", "freq", "temperatures", "col")]
+
+
+class ThermalParams(C.Structure):
+    _fields_ = [("Tmin", C.c_double), ("Tmax", C.c_double), ("Fcrit", C.c_double), ("Tdiff", C.c_double),
+                ("pseudo_ray", C.c_int), ("dust", C.c_int), ("co_fix", C.c_int), ("temp_fix", C.c_int)]
+
+
+def default_thermal_params(**kw) -> ThermalParams:
+    """params.dat's thermal-balance values (params.dat:29-33) and the
+    reference makefile's build options (PSEUDO_1D along ray 6, DUST,
+    CO_FIX, TEMP_FIX)."""
+    d = dict(Tmin=2.7, Tmax=30000.0, Fcrit=0.005, Tdiff=0.01, pseudo_ray=6, dust=1, co_fix=1, temp_fix=1)
+    d.update(kw)
+    return ThermalParams(**d)
+
+
+class ThermalStats(C.Structure):
+    _fields_ = [(k, C.c_int) for k in ("iteration", "levpop_iteration", "level_conv", "first_time", "done", "lvg",
+                                      "thermal_points", "thermally_converged", "level_converged")] + \
+        [("coolant_converged", C.c_int * 4), ("chemistry_solves", C.c_int), ("failed", C.c_int)] + \
+        [(k, C.c_longlong) for k in ("nst", "nfe", "nje", "nsetups", "nni")] + [("ms", C.c_double)]
 
 
 class PdrChemError(RuntimeError):
@@ -103,6 +134,18 @@ def load(path: Optional[str] = None):
     L.pdrchem_jacobian_device.argtypes = [C.c_int, C.c_int, vp, vp, vp, C.c_int, vp, vp]
     L.pdrchem_solve_device_noalloc.argtypes = [C.c_int, vp, vp, vp, C.c_int, C.c_double, C.c_double, C.c_double,
                                                C.c_double, C.c_int, vp, vp, C.c_int, vp]
+    L.pdrchem_coolant_tables_create.argtypes = [C.POINTER(CoolantStruct), ip]
+    L.pdrchem_coolant_tables_create.restype = vp
+    L.pdrchem_coolant_tables_destroy.argtypes = [vp]
+    L.pdrchem_coolant_tables_destroy.restype = None
+    L.pdrchem_coolant_tables_levels.argtypes = [vp]
+    L.pdrchem_level_populations_device.argtypes = [vp, C.c_int, C.POINTER(Phys), C.c_int, C.c_int, C.c_int, C.c_int,
+                                                   vp, vp, vp, vp, vp, vp, vp, vp, vp, C.c_int, C.c_int, C.c_int,
+                                                   vp, vp, vp, vp, vp, vp]
+    L.pdrchem_lte_populations_device.argtypes = [vp, C.c_int, C.c_int, vp, vp, vp, vp, vp, vp]
+    L.pdrchem_dust_temperature_device.argtypes = [C.c_int, C.c_int, vp, vp, vp, vp]
+    L.pdrchem_cloud_thermal_init.argtypes = [vp, C.POINTER(CoolantStruct), ip, ip, C.POINTER(ThermalParams)]
+    L.pdrchem_cloud_thermal_iterate.argtypes = [vp, C.c_double, C.c_int, C.POINTER(ThermalStats)]
     L.pdrchem_set_max_blocks.argtypes = [C.c_int]
     L.pdrchem_set_max_blocks.restype = None
     L.pdrchem_set_precision.argtypes = [C.c_int]
@@ -507,3 +550,117 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+# --------------------------------------------------------------------------
+# thermal balance: coolant tables, level populations, dust temperature
+# --------------------------------------------------------------------------
+
+THERMAL_SPECIES = ("NCx", "NC", "NO", "NCO", "NH", "NPROTON", "NELECT", "NHE", "NH2")
+
+
+def thermal_slots(net) -> np.ndarray:
+    """0-based abundance slots of C+, C, O, CO (the coolants) and H, H+, e-,
+    He, H2 (the collision partners), from READ_SPECIES's labels"""
+    return np.array([net.labels[k] - 1 for k in THERMAL_SPECIES], dtype=np.int32)
+
+
+def heating_index(net) -> np.ndarray:
+    """the 13 1-based slots of pdrchem_heating_device: NH, NH2, NC, NCO,
+    NHEx, NH3x, NH3Ox, NHCOx, NH2x, NELECT, NRGR, NRH2, NRCI"""
+    return np.array([net.labels[k] for k in ("NH", "NH2", "NC", "NCO", "NHEx", "NH3x", "NH3Ox", "NHCOx", "NH2x",
+                                             "NELECT")] + [net.nr_indices[k] for k in ("NRGR", "NRH2", "NRCI")],
+                    dtype=np.int32)
+
+
+def coolant_structs(coolants):
+    """(pdrchem_coolant[4], arrays to keep alive) from four lamda.Coolant"""
+    if len(coolants) != 4:
+        raise PdrChemError("four coolants (CII, CI, OI, CO) expected")
+    arr = (CoolantStruct * 4)()
+    keep = []
+    dp = C.POINTER(C.c_double)
+    for k, c in enumerate(coolants):
+        fa = [np.ascontiguousarray(a, dtype=np.float64) for a in c.fortran_arrays()]
+        keep.extend(fa)
+        arr[k] = CoolantStruct(c.nlev, c.ntemp, *[a.ctypes.data_as(dp) for a in fa])
+    return arr, keep
+
+
+class CoolantTables:
+    """Device copies of the four coolants (pdrchem_coolant_tables)."""
+
+    def __init__(self, coolants, slots):
+        L = load()
+        arr, keep = coolant_structs(coolants)
+        sl = np.ascontiguousarray(slots, dtype=np.int32)
+        if sl.shape != (9,):
+            raise PdrChemError("9 abundance slots expected")
+        self.h = L.pdrchem_coolant_tables_create(arr, sl.ctypes.data_as(C.POINTER(C.c_int)))
+        del keep
+        if not self.h:
+            raise PdrChemError(L.pdrchem_last_error().decode())
+        self.nlevtot = L.pdrchem_coolant_tables_levels(self.h)
+        self.offsets = np.cumsum([0] + [c.nlev for c in coolants])
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().pdrchem_coolant_tables_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+def level_populations_device(tables: CoolantTables, coolant: int, T, Td, rho, ab, pop, count, offs, proj, step,
+                             nrays: int, pseudo_ray: int = -1, phys: Optional[Phys] = None, co_fix: int = 0,
+                             co_average_co: int = 0, levpop_iteration: int = 0, skip=None, solution=None,
+                             cool=None, lines=None, stream=None):
+    """Escape-probability level populations of one coolant (-1: all four) on
+    device tensors (float64; count/proj int32, offs int64, skip uint8):
+    T, Td, rho [n]; ab [n, nspec]; pop [nrows, nlevtot]; the evaluation
+    points of ray j of point p are rows proj[offs[p*nrays+j] + i] of pop,
+    i < count[p*nrays+j], at steps `step` (pc).  Returns (solution [n,
+    nlevtot], cool [n, 4], lines [n, 17]); outputs may be passed in."""
+    import torch
+    n = T.shape[0]
+    dev = T.device
+    f64 = dict(dtype=torch.float64, device=dev)
+    solution = solution if solution is not None else torch.zeros(n, tables.nlevtot, **f64)
+    cool = cool if cool is not None else torch.zeros(n, 4, **f64)
+    lines = lines if lines is not None else torch.zeros(n, 17, **f64)
+    S = torch.empty(max(n * nrays * tables.nlevtot, 1), **f64)
+    ph = phys or default_phys()
+    s = stream if stream is not None else torch.cuda.current_stream(dev).cuda_stream
+    ptr = lambda t: None if t is None else t.data_ptr()
+    rc = load().pdrchem_level_populations_device(
+        tables.h, coolant, C.byref(ph), n, ab.shape[1], nrays, pseudo_ray, ptr(T), ptr(Td), ptr(rho), ptr(ab),
+        ptr(pop), ptr(count), ptr(offs), ptr(proj), ptr(step), co_fix, co_average_co, levpop_iteration, ptr(skip),
+        S.data_ptr(), solution.data_ptr(), cool.data_ptr(), lines.data_ptr(), s)
+    if rc != 0:
+        raise PdrChemError(load().pdrchem_last_error().decode())
+    return solution, cool, lines
+
+
+def lte_populations_device(tables: CoolantTables, T, ab, rho, skip=None, pop=None, stream=None):
+    """LTE populations of every coolant [n, nlevtot] (device tensors)."""
+    import torch
+    n = T.shape[0]
+    pop = pop if pop is not None else torch.zeros(n, tables.nlevtot, dtype=torch.float64, device=T.device)
+    s = stream if stream is not None else torch.cuda.current_stream(T.device).cuda_stream
+    rc = load().pdrchem_lte_populations_device(tables.h, n, ab.shape[1], T.data_ptr(), ab.data_ptr(), rho.data_ptr(),
+                                               None if skip is None else skip.data_ptr(), pop.data_ptr(), s)
+    if rc != 0:
+        raise PdrChemError(load().pdrchem_last_error().decode())
+    return pop
+
+
+def dust_temperature_device(G, uv, out=None, stream=None):
+    """CALCULATE_DUST_TEMPERATURES on device tensors: G [n, nrays], uv [n]."""
+    import torch
+    n, nrays = G.shape
+    out = out if out is not None else torch.zeros(n, dtype=torch.float64, device=G.device)
+    s = stream if stream is not None else torch.cuda.current_stream(G.device).cuda_stream
+    if load().pdrchem_dust_temperature_device(n, nrays, G.data_ptr(), uv.data_ptr(), out.data_ptr(), s) != 0:
+        raise PdrChemError(load().pdrchem_last_error().decode())
+    return out

@@ -366,6 +366,14 @@ int pdrchem_cloud_set_server(pdrchem_cloud *cloud, pdrchem_server *server);
 #define PDRCHEM_CLOUD_COLUMNS 3      /* [npts*nrays*5] H2, HD, CO, C, S (cm^-2) */
 #define PDRCHEM_CLOUD_TEMPERATURE 4  /* [npts] */
 #define PDRCHEM_CLOUD_UVFIELD 5      /* [npts] attenuated field */
+/* fields of the thermal balance (after pdrchem_cloud_thermal_init) */
+#define PDRCHEM_CLOUD_DUST_TEMPERATURE 6    /* [npts] (K) */
+#define PDRCHEM_CLOUD_HEATING 7             /* [npts*12] the .heat.fin columns (pdrchem_heating_device) */
+#define PDRCHEM_CLOUD_COOLING 8             /* [npts*5] CII, CI, OI, CO line cooling and their sum */
+#define PDRCHEM_CLOUD_LINES 9               /* [npts*17] the .line.fin lines (pdrchem_level_populations_device) */
+#define PDRCHEM_CLOUD_POPULATIONS 10        /* [npts*nlevtot] CII, CI, OI, CO level populations (cm^-3) */
+#define PDRCHEM_CLOUD_PREV_TEMPERATURE 11   /* [npts] previousgastemperature: the reported temperature */
+#define PDRCHEM_CLOUD_THERMAL_CONVERGED 12  /* [npts] 1 where the point's thermal balance converged */
 int pdrchem_cloud_get(pdrchem_cloud *cloud, int field, double *out);
 /* Evaluation points: total count, then per (point, ray) the count and CSR
  * offset (offsets[npts*nrays+1]) and per evaluation point the projected
@@ -373,6 +381,122 @@ int pdrchem_cloud_get(pdrchem_cloud *cloud, int field, double *out);
  * NULL). */
 long long pdrchem_cloud_num_evaluation_points(const pdrchem_cloud *cloud);
 int pdrchem_cloud_evaluation_points(pdrchem_cloud *cloud, int *count, long long *offsets, int *proj, double *step);
+
+/* ---- thermal balance ---------------------------------------------------- */
+/* One coolant's tables as READINPUT leaves them (read_input.F90:6-259), in
+ * the reference's column-major layout: X(i, j) of an nlev x nlev matrix at
+ * [i + j*nlev]; TEMPERATURES(p, k) at [p + 7k]; col: 7 blocks of
+ * nlev*nlev*ntemp, COL(i, j, k) of partner p at [p*nlev*nlev*ntemp + i +
+ * j*nlev + k*nlev*nlev].  Partners in the order H2, para-H2, ortho-H2, e-,
+ * H, He, H+; zero temperatures mark data a partner lacks.  The coolants
+ * are always given as four, CII (5 levels), CI (5), OI (5), CO (up to 41):
+ * the reference's order (3DPDR.F90 cooling columns). */
+typedef struct {
+  int nlev, ntemp;
+  const double *energies;      /* [nlev] erg */
+  const double *weights;       /* [nlev] statistical weights */
+  const double *A, *B, *freq;  /* [nlev*nlev] s^-1, Einstein B, Hz */
+  const double *temperatures;  /* [7*ntemp] K */
+  const double *col;           /* [7*nlev*nlev*ntemp] cm^3 s^-1 */
+} pdrchem_coolant;
+
+/* Device copies of the four coolants' tables and the abundance slots the
+ * level populations read: slots[9], 0-based, of C+, C, O, CO (the coolants'
+ * species) and of H, H+, e-, He, H2 (the collision partners).  NULL on
+ * failure (pdrchem_last_error says why); the level-population calls below
+ * report their errors there too. */
+typedef struct pdrchem_coolant_tables pdrchem_coolant_tables;
+pdrchem_coolant_tables *pdrchem_coolant_tables_create(const pdrchem_coolant *coolants, const int *slots);
+void pdrchem_coolant_tables_destroy(pdrchem_coolant_tables *tables);
+/* Levels of all four coolants (the row width of population arrays). */
+int pdrchem_coolant_tables_levels(const pdrchem_coolant_tables *tables);
+
+/* Batched escape-probability level populations of one coolant (0 CII, 1 CI,
+ * 2 OI, 3 CO) on device memory: per point FIND_CCOEFF (find_Ccoeff.F90),
+ * escape_probability (escape_probability.F90:6-140) with the optical depth
+ * along each of nrays rays summed over the ray's evaluation points, and
+ * solvlevpop (solvlevpop.F90:6-56), as the reference's level-population
+ * step does (3DPDR.F90:688-850).
+ *   d_T, d_Td, d_rho [npts]; d_ab [npts*nspec];
+ *   d_pop [nrows*nlevtot] (nrows >= npts): the populations of every level of
+ *     every coolant, row p for point p; the evaluation points of ray j of
+ *     point p are rows d_proj[d_offs[p*nrays+j] + i], i < d_count[p*nrays+j],
+ *     at steps d_step (pc) from the previous one (the first from point p);
+ *   pseudo_ray: -1 for 3D (beta the mean over the rays), else the ray of
+ *     the PSEUDO_1D build (the other rays optically thick);
+ *   phys: v_turb, metallicity, g2d used;
+ *   co_fix / co_average_co / levpop_iteration: the CO_FIX averaging with the
+ *     old populations (3DPDR.F90:771-846; co_average_co = 1 where the CO
+ *     populations are averaged too); co_fix 0 = none;
+ *   d_skip [npts] (NULL = none): points left untouched;
+ *   d_S: scratch of npts*nrays*nlevtot doubles (the level columns);
+ * writes d_solution[npts*nlevtot] (this coolant's levels), d_cool[npts*4]
+ * (this coolant's column: line cooling, erg cm^-3 s^-1) and d_lines[npts*17]
+ * (this coolant's lines among CII 158um; CI 2-1, 3-1, 3-2; OI 2-1, 3-1, 3-2;
+ * CO J=1-0 .. 10-9).  The level columns are formed once per call for all
+ * levels; call with coolant = -1 to do all four in turn.  Asynchronous on
+ * stream; 0 on success, -1 on a bad argument. */
+int pdrchem_level_populations_device(const pdrchem_coolant_tables *tables, int coolant, const pdrchem_phys *phys,
+                                     int npts, int nspec, int nrays, int pseudo_ray, const double *d_T,
+                                     const double *d_Td, const double *d_rho, const double *d_ab,
+                                     const double *d_pop, const int *d_count, const long long *d_offs,
+                                     const int *d_proj, const double *d_step, int co_fix, int co_average_co,
+                                     int levpop_iteration, const unsigned char *d_skip, double *d_S,
+                                     double *d_solution, double *d_cool, double *d_lines, void *stream);
+/* LTE populations (3DPDR.F90:520-552) of every coolant at the gas
+ * temperature: d_pop[npts*nlevtot]; points with d_skip[p] != 0 untouched. */
+int pdrchem_lte_populations_device(const pdrchem_coolant_tables *tables, int npts, int nspec, const double *d_T,
+                                   const double *d_ab, const double *d_rho, const unsigned char *d_skip,
+                                   double *d_pop, void *stream);
+/* CALCULATE_DUST_TEMPERATURES (dust_t.F90:44-97): d_G [npts*nrays] surface
+ * field per ray, d_uv [npts] attenuated field -> d_Td [npts]. */
+int pdrchem_dust_temperature_device(int npts, int nrays, const double *d_G, const double *d_uv, double *d_Td,
+                                    void *stream);
+
+/* Thermal-balance parameters (params.dat, input_parameters.F90:45-51) and
+ * the reference's build options. */
+typedef struct {
+  double Tmin, Tmax;  /* absolute temperature bounds (K) */
+  double Fcrit;       /* convergence: 2|H - C|/|H + C| <= Fcrit */
+  double Tdiff;       /* bracket width below which the search gives up (K) */
+  int pseudo_ray;     /* -1: 3D; else PSEUDO_1D along this ray (6 in the reference) */
+  int dust;           /* 1: dust temperatures from the field (DUST) */
+  int co_fix;         /* CO_FIX averaging of slow level populations */
+  int temp_fix;       /* TEMP_FIX bracket handling */
+} pdrchem_thermal_params;
+typedef struct {
+  int iteration;           /* ITERATION after this call */
+  int levpop_iteration;    /* levpop_iteration after this call (0: levels converged) */
+  int level_conv;          /* every point's populations converged in this call */
+  int first_time;
+  int done;                /* every point thermally converged: the loop ends */
+  int lvg;                 /* 1: escape-probability populations; 0: chemistry + LTE */
+  int thermal_points;      /* points whose heating and temperature were updated */
+  int thermally_converged; /* points thermally converged */
+  int level_converged;     /* points whose populations are converged */
+  int coolant_converged[4];
+  int chemistry_solves, failed;
+  long long nst, nfe, nje, nsetups, nni;
+  double ms;
+} pdrchem_thermal_stats;
+/* Turn on the thermal balance of a cloud (THERMALBALANCE with GUESS_TEMP):
+ * coolant tables and slots as pdrchem_coolant_tables_create; heat_index[13]
+ * as pdrchem_heating_device.  Sets the temperature brackets of every point
+ * (3DPDR.F90:1702-1716) and, with prm->dust, the dust temperatures from the
+ * field.  Works with the reduced and full networks. */
+int pdrchem_cloud_thermal_init(pdrchem_cloud *cloud, const pdrchem_coolant *coolants, const int *slots,
+                               const int *heat_index, const pdrchem_thermal_params *prm);
+/* One ITERATION of the reference's outer loop (3DPDR.F90:590-1241): LTE
+ * populations before the first; on the first iteration after the level
+ * populations converged, three chemistry iterations of the thermally
+ * unconverged points (pdrchem_cloud_iterate) and LTE populations; otherwise
+ * the escape-probability populations of the points whose populations have
+ * not converged; then heating, line cooling and the temperature update of
+ * the thermally unconverged points, the convergence counts and the new
+ * populations.  stats may be NULL; stats->done = 1 once every point is
+ * thermally converged (further calls do nothing). */
+int pdrchem_cloud_thermal_iterate(pdrchem_cloud *cloud, double t_end_years, int precision,
+                                  pdrchem_thermal_stats *stats);
 const char *pdrchem_cloud_last_error(const pdrchem_cloud *cloud);
 void pdrchem_cloud_destroy(pdrchem_cloud *cloud);
 

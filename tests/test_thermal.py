@@ -134,3 +134,66 @@ def test_collision_dominated_limit_is_lte(tables, name):
         if lte[il] > 0:
             for jl in range(il):
                 assert np.all(res["tau"][il, jl, :] == 0.0) and np.all(res["beta"][il, jl, :] == 1.0)
+
+
+def _search(T0, balance, Tmin=2.7, Tmax=30000.0, Fcrit=0.005, temp_fix=True, max_steps=200):
+    """the oracle's temperature search (thermal_loop.ThermalLoop.update_temperature)
+    on one point whose heating minus cooling is balance(T) -> (H, C)"""
+    from thermal_loop import ThermalLoop
+    L = ThermalLoop.__new__(ThermalLoop)
+    L.Tmin, L.Tmax, L.Fcrit, L.Tdiff, L.temp_fix = Tmin, Tmax, Fcrit, 0.01, temp_fix
+    L.T = np.array([T0])
+    L.Tprev = L.T.copy()
+    L.Tlow = np.maximum(L.T / 2.0, Tmin)
+    L.Thigh = np.minimum(L.T * 1.5, Tmax)
+    L.prev = [""]
+    L.binarychop = np.zeros(1, bool)
+    L.converged = np.zeros(1, bool)
+    L.doleveltmin = np.zeros(1, bool)
+    L.level_converged = np.zeros(1, bool)
+    L.first_time = True
+    trail = []
+    for k in range(max_steps):
+        Tnow = L.T[0]
+        h, c = balance(Tnow)
+        fmean = h - c
+        L.update_temperature(0, fmean, 2.0 * abs(fmean) / abs(h + c))
+        L.first_time = False
+        trail.append((Tnow, L.T[0], L.Tlow[0], L.Thigh[0], bool(L.binarychop[0])))
+        if L.converged[0]:
+            return L, trail
+    raise AssertionError("no convergence")
+
+
+@pytest.mark.parametrize("T0,Tstar", [(10.0, 137.0), (500.0, 41.5), (80.0, 80.6), (30.0, 3000.0)])
+def test_temperature_search_brackets_the_balance(T0, Tstar):
+    """heating a, cooling b T^2: the 30% steps bracket T* = sqrt(a/b), the
+    binary chop keeps it bracketed, and the search stops where the relative
+    imbalance 2|H-C|/(H+C) is within Fcrit -- the temperature the point
+    reports (previousgastemperature) within Fcrit/2 of T*."""
+    a = 1.0e-20
+    b = a / Tstar ** 2
+    L, trail = _search(T0, lambda T: (a, b * T * T))
+    for Told, Tnew, lo, hi, chop in trail:
+        if chop:
+            assert lo <= Tnew <= hi
+    # once chopping, the bracket holds T*
+    chopped = [t for t in trail if t[4]]
+    assert chopped and all(lo <= Tstar * (1 + 1e-12) and hi >= Tstar * (1 - 1e-12) for _, _, lo, hi, _ in chopped)
+    Trep = L.Tprev[0]
+    assert abs(Trep - Tstar) / Tstar <= 0.005 / 2 * 1.01, (Trep, Tstar)
+
+
+def test_temperature_search_stops_at_the_bounds():
+    """balance below Tmin: the 30% cooling steps stop at Tmin, where the
+    temperature no longer moves and TEMP_FIX ends the search; above Tmax the
+    search ends at Tmax"""
+    a = 1.0e-20
+    L, trail = _search(20.0, lambda T: (a, a * (T / 1.0) ** 2), max_steps=400)
+    assert L.T[0] == 2.7 and L.Tprev[0] == 2.7 and not L.doleveltmin[0]
+    assert [t[1] for t in trail[:3]] == [14.0, 0.7 * 14.0, 0.7 * (0.7 * 14.0)]
+    L, trail = _search(20000.0, lambda T: (a, a * (T / 1.0e5) ** 2))
+    assert L.T[0] == 30000.0 and L.Tprev[0] == 30000.0
+    # without TEMP_FIX a point held at Tmin never converges
+    with pytest.raises(AssertionError, match="no convergence"):
+        _search(20.0, lambda T: (a, a * (T / 1.0) ** 2), temp_fix=False)
