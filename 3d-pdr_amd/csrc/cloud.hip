@@ -533,6 +533,7 @@ struct CloudThermal {
   double *d_pop = nullptr, *d_sol = nullptr, *d_S = nullptr;  // [npts][nlevtot], S [npts][nrays][nlevtot]
   double *d_cool = nullptr, *d_total = nullptr;               // [npts][4], [npts]
   double *d_lines = nullptr, *d_heat = nullptr;               // [npts][17], [npts][12]
+  double *d_tau = nullptr;                                    // [npts][17] line optical depths
   double *d_fmean = nullptr, *d_fratio = nullptr;             // [npts]
   double *c_heat = nullptr, *c_uv = nullptr;                  // compact heating terms, field
   pdrchem::ThermalPoint *d_state = nullptr;
@@ -1050,6 +1051,9 @@ static int thermal_get(pdrchem_cloud *c, int field, double *out) {
     case PDRCHEM_CLOUD_LINES:
       good = ok(hipMemcpyAsync(out, th->d_lines, n * pdrchem::LvgArgs::kLines * 8, hipMemcpyDeviceToHost, s), "copy");
       break;
+    case PDRCHEM_CLOUD_OPTICAL_DEPTHS:
+      good = ok(hipMemcpyAsync(out, th->d_tau, n * pdrchem::LvgArgs::kLines * 8, hipMemcpyDeviceToHost, s), "copy");
+      break;
     case PDRCHEM_CLOUD_POPULATIONS:
       good = ok(hipMemcpyAsync(out, th->d_pop, n * th->nlevtot * 8, hipMemcpyDeviceToHost, s), "copy");
       break;
@@ -1100,6 +1104,7 @@ int pdrchem_cloud_get(pdrchem_cloud *c, int field, double *out) {
     case PDRCHEM_CLOUD_COOLING:
     case PDRCHEM_CLOUD_PREV_TEMPERATURE:
     case PDRCHEM_CLOUD_THERMAL_CONVERGED:
+    case PDRCHEM_CLOUD_OPTICAL_DEPTHS:
       return thermal_get(c, field, out);
     default:
       c->err = "unknown field";
@@ -1205,7 +1210,8 @@ int pdrchem_cloud_thermal_init(pdrchem_cloud *c, const pdrchem_coolant *coolants
   if (!c->alloc(&th->d_pop, n * off, "populations") || !c->alloc(&th->d_sol, n * off, "populations") ||
       !c->alloc(&th->d_S, n * c->nrays * off, "level columns") || !c->alloc(&th->d_cool, n * 4, "cooling") ||
       !c->alloc(&th->d_total, n, "cooling") || !c->alloc(&th->d_lines, n * LvgArgs::kLines, "lines") ||
-      !c->alloc(&th->d_heat, n * 12, "heating") || !c->alloc(&th->d_fmean, n, "balance") ||
+      !c->alloc(&th->d_heat, n * 12, "heating") || !c->alloc(&th->d_tau, n * LvgArgs::kLines, "optical depths") ||
+      !c->alloc(&th->d_fmean, n, "balance") ||
       !c->alloc(&th->d_fratio, n, "balance") || !c->alloc(&th->c_heat, n * 12, "heating") ||
       !c->alloc(&th->c_uv, n, "field") || !c->alloc(&th->d_state, n, "thermal state") ||
       !c->alloc(&th->d_lc, n, "flags") || !c->alloc(&th->d_cconv, n * 4, "flags") ||
@@ -1214,6 +1220,7 @@ int pdrchem_cloud_thermal_init(pdrchem_cloud *c, const pdrchem_coolant *coolants
       !ok(hipMemsetAsync(th->d_total, 0, n * 8, s), "memset") ||
       !ok(hipMemsetAsync(th->d_lines, 0, n * LvgArgs::kLines * 8, s), "memset") ||
       !ok(hipMemsetAsync(th->d_heat, 0, n * 12 * 8, s), "memset") ||
+      !ok(hipMemsetAsync(th->d_tau, 0, n * LvgArgs::kLines * 8, s), "memset") ||
       !ok(hipMemsetAsync(th->d_lc, 0, n, s), "memset")) {
     c->fail("thermal state");
     return -1;
@@ -1314,6 +1321,8 @@ int pdrchem_cloud_thermal_iterate(pdrchem_cloud *c, double t_end_years, int prec
     a.solution = th->d_sol;
     a.cool = th->d_cool;
     a.lines = th->d_lines;
+    a.tau_lines = th->d_tau;
+    a.tau_ray = p1 >= 0 ? p1 : (c->nrays > 6 ? 6 : 0);
     for (int k = 0; k < 4; k++)
       if (launch_lvg(t, a, k, s) != 0) return fail("level populations");
     if (launch_total_cool(n, th->d_cool, th->d_total, s) != 0) return fail("cooling");

@@ -137,6 +137,10 @@ struct LvgArgs {
   double *solution;                    // [npts][nlevtot]
   double *cool;                        // [npts][4]
   double *lines;                       // [npts][kLines]
+  // optical depth of each line along ray tau_ray (NULL: not wanted); 0 for a
+  // line without one (empty upper level, |tpop| < 1e-50)
+  double *tau_lines;                   // [npts][kLines]
+  int tau_ray;
   // index of (upper, lower) of coolant ci among the output lines, or -1:
   // CII 2-1; CI 2-1, 3-1, 3-2; OI 2-1, 3-1, 3-2; CO J+1 -> J for J = 0..9
   __host__ __device__ static int line_slot(int ci, int il, int jl) {

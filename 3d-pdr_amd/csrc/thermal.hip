@@ -217,7 +217,7 @@ __global__ void __launch_bounds__(64) lvg_kernel(LvgTables t, LvgArgs a, int ci)
     const double bbd = tmp2 * (1.0 / (exp(kHP * nu / kKB / Td) - 1.0) * emis);
     BB = BB + bbd;
     const double gi = cd.g[il], gj = cd.g[jl];
-    double beta, Sij;
+    double beta, Sij, tau_sel = 0.0;
     bool line = true;
     if (own[il] == 0.0) {
       Sij = 0.0;
@@ -239,6 +239,7 @@ __global__ void __launch_bounds__(64) lvg_kernel(LvgTables t, LvgArgs a, int ci)
             const double Sj = Sp[(long long)j * t.nlevtot + jl], Si = Sp[(long long)j * t.nlevtot + il];
             tau = frac1 * frac2 * kPC * ((gi * Sj - gj * Si) / (2.0 * gj));
           }
+          if (j == a.tau_ray) tau_sel = tau;
           double b;
           if (tau < -5.0)
             b = (1.0 - exp(5.0)) / (-5.0);
@@ -251,15 +252,15 @@ __global__ void __launch_bounds__(64) lvg_kernel(LvgTables t, LvgArgs a, int ci)
         beta = (a.pseudo_ray >= 0) ? bsum : bsum / (double)a.nrays;
       }
     }
+    const int k = a.line_slot(ci, il, jl);
     if (line) {
       const double lv = A * kHP * nu * own[il] * beta * (Sij - BB) / Sij;
       cool = cool + lv;
-      const int k = a.line_slot(ci, il, jl);
       if (k >= 0) a.lines[(long long)p * LvgArgs::kLines + k] = lv;
-    } else {
-      const int k = a.line_slot(ci, il, jl);
-      if (k >= 0) a.lines[(long long)p * LvgArgs::kLines + k] = 0.0;
+    } else if (k >= 0) {
+      a.lines[(long long)p * LvgArgs::kLines + k] = 0.0;
     }
+    if (k >= 0 && a.tau_lines) a.tau_lines[(long long)p * LvgArgs::kLines + k] = tau_sel;
     const double f = (1.0 - beta) * Sij + beta * BB;
     Fd[il + jl * n] = f;
     Fd[jl + il * n] = f;
@@ -709,10 +710,11 @@ int pdrchem_level_populations_device(const pdrchem_coolant_tables *tables, int c
                                      const double *d_pop, const int *d_count, const long long *d_offs,
                                      const int *d_proj, const double *d_step, int co_fix, int co_average_co,
                                      int levpop_iteration, const unsigned char *d_skip, double *d_S,
-                                     double *d_solution, double *d_cool, double *d_lines, void *stream) {
+                                     double *d_solution, double *d_cool, double *d_lines, int tau_ray,
+                                     double *d_tau_lines, void *stream) {
   using namespace pdrchem;
   if (!tables || !phys || coolant < -1 || coolant > 3 || npts < 0 || nrays < 1 || pseudo_ray < -1 ||
-      pseudo_ray >= nrays || !slots_ok(tables->tab, nspec)) {
+      pseudo_ray >= nrays || !slots_ok(tables->tab, nspec) || (d_tau_lines && (tau_ray < 0 || tau_ray >= nrays))) {
     set_last_error("pdrchem_level_populations_device: bad argument");
     return -1;
   }
@@ -750,6 +752,8 @@ int pdrchem_level_populations_device(const pdrchem_coolant_tables *tables, int c
   a.solution = d_solution;
   a.cool = d_cool;
   a.lines = d_lines;
+  a.tau_lines = d_tau_lines;
+  a.tau_ray = tau_ray;
   for (int k = coolant < 0 ? 0 : coolant; k <= (coolant < 0 ? 3 : coolant); k++)
     if (launch_lvg(t, a, k, s) != 0) {
       set_last_error("pdrchem_level_populations_device: launch failed");

@@ -21,7 +21,7 @@ from .netgen import Network
 
 ABUNDANCE, AV, G, COLUMNS, TEMPERATURE, UVFIELD = range(6)
 (DUST_TEMPERATURE, HEATING, COOLING, LINES, POPULATIONS, PREV_TEMPERATURE,
- THERMAL_CONVERGED) = range(6, 13)
+ THERMAL_CONVERGED, OPTICAL_DEPTHS) = range(6, 14)
 SHIELD_SPECIES = ("H2", "HD", "CO", "C", "S")   # column order expected by the rate kernel
 ISO, UNI = 0, 1
 
@@ -134,16 +134,19 @@ class Cloud:
             raise lib.PdrChemError(self._err())
         return {k: getattr(st, k) for k, _ in st._fields_}
 
-    def thermal_init(self, coolants=None, params: Optional[lib.ThermalParams] = None) -> None:
+    def thermal_init(self, coolants=None, params: Optional[lib.ThermalParams] = None,
+                     reference_slots: bool = False) -> None:
         """Turn on the thermal balance (THERMALBALANCE, GUESS_TEMP): the four
         coolants (lamda.Coolant; default the stored CII, CI, OI, CO tables)
-        and the parameters (default params.dat's and the reference build's)."""
+        and the parameters (default params.dat's and the reference build's).
+        reference_slots: the heating reads its species where the reference's
+        arrays hold them (lib.heating_index)."""
         from . import lamda
         L = lib.load()
         cools = coolants if coolants is not None else lamda.load_coolants()
         arr, keep = lib.coolant_structs(cools)
         slots = lib.thermal_slots(self.net)
-        hidx = lib.heating_index(self.net)
+        hidx = lib.heating_index(self.net, reference_slots)
         self._tparams = params or lib.default_thermal_params()
         ip = C.POINTER(C.c_int)
         rc = L.pdrchem_cloud_thermal_init(self._h, arr, slots.ctypes.data_as(ip), hidx.ctypes.data_as(ip),
@@ -167,7 +170,8 @@ class Cloud:
                   COLUMNS: (self.npts, self.nrays, 5), TEMPERATURE: (self.npts,), UVFIELD: (self.npts,),
                   DUST_TEMPERATURE: (self.npts,), HEATING: (self.npts, 12), COOLING: (self.npts, 5),
                   LINES: (self.npts, 17), POPULATIONS: (self.npts, getattr(self, "nlevtot", 0)),
-                  PREV_TEMPERATURE: (self.npts,), THERMAL_CONVERGED: (self.npts,)}
+                  PREV_TEMPERATURE: (self.npts,), THERMAL_CONVERGED: (self.npts,),
+                  OPTICAL_DEPTHS: (self.npts, 17)}
         out = np.zeros(shapes[field])
         if lib.load().pdrchem_cloud_get(self._h, field, out.ctypes.data_as(C.POINTER(C.c_double))) != 0:
             raise lib.PdrChemError(self._err())

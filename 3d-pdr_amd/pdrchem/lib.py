@@ -141,7 +141,7 @@ def load(path: Optional[str] = None):
     L.pdrchem_coolant_tables_levels.argtypes = [vp]
     L.pdrchem_level_populations_device.argtypes = [vp, C.c_int, C.POINTER(Phys), C.c_int, C.c_int, C.c_int, C.c_int,
                                                    vp, vp, vp, vp, vp, vp, vp, vp, vp, C.c_int, C.c_int, C.c_int,
-                                                   vp, vp, vp, vp, vp, vp]
+                                                   vp, vp, vp, vp, vp, C.c_int, vp, vp]
     L.pdrchem_lte_populations_device.argtypes = [vp, C.c_int, C.c_int, vp, vp, vp, vp, vp, vp]
     L.pdrchem_dust_temperature_device.argtypes = [C.c_int, C.c_int, vp, vp, vp, vp]
     L.pdrchem_cloud_thermal_init.argtypes = [vp, C.POINTER(CoolantStruct), ip, ip, C.POINTER(ThermalParams)]
@@ -565,12 +565,17 @@ def thermal_slots(net) -> np.ndarray:
     return np.array([net.labels[k] - 1 for k in THERMAL_SPECIES], dtype=np.int32)
 
 
-def heating_index(net) -> np.ndarray:
+def heating_index(net, reference_slots: bool = False) -> np.ndarray:
     """the 13 1-based slots of pdrchem_heating_device: NH, NH2, NC, NCO,
-    NHEx, NH3x, NH3Ox, NHCOx, NH2x, NELECT, NRGR, NRH2, NRCI"""
-    return np.array([net.labels[k] for k in ("NH", "NH2", "NC", "NCO", "NHEx", "NH3x", "NH3Ox", "NHCOx", "NH2x",
-                                             "NELECT")] + [net.nr_indices[k] for k in ("NRGR", "NRH2", "NRCI")],
-                    dtype=np.int32)
+    NHEx, NH3x, NH3Ox, NHCOx, NH2x, NELECT, NRGR, NRH2, NRCI.  With
+    reference_slots, each species label points where the reference's own
+    arrays hold it (netgen.reference_slot: its reduced network reads CH5+ as
+    NH2x, sub_calculate_heating.F90:229-230)."""
+    from .netgen import reference_slot
+    sp = [net.labels[k] for k in ("NH", "NH2", "NC", "NCO", "NHEx", "NH3x", "NH3Ox", "NHCOx", "NH2x", "NELECT")]
+    if reference_slots:
+        sp = [reference_slot(net.name, s - 1) + 1 for s in sp]
+    return np.array(sp + [net.nr_indices[k] for k in ("NRGR", "NRH2", "NRCI")], dtype=np.int32)
 
 
 def coolant_structs(coolants):
@@ -615,13 +620,15 @@ class CoolantTables:
 def level_populations_device(tables: CoolantTables, coolant: int, T, Td, rho, ab, pop, count, offs, proj, step,
                              nrays: int, pseudo_ray: int = -1, phys: Optional[Phys] = None, co_fix: int = 0,
                              co_average_co: int = 0, levpop_iteration: int = 0, skip=None, solution=None,
-                             cool=None, lines=None, stream=None):
+                             cool=None, lines=None, tau_ray: Optional[int] = None, tau_lines=None, stream=None):
     """Escape-probability level populations of one coolant (-1: all four) on
     device tensors (float64; count/proj int32, offs int64, skip uint8):
     T, Td, rho [n]; ab [n, nspec]; pop [nrows, nlevtot]; the evaluation
     points of ray j of point p are rows proj[offs[p*nrays+j] + i] of pop,
     i < count[p*nrays+j], at steps `step` (pc).  Returns (solution [n,
-    nlevtot], cool [n, 4], lines [n, 17]); outputs may be passed in."""
+    nlevtot], cool [n, 4], lines [n, 17]), and the lines' optical depths
+    along ray tau_ray [n, 17] as a fourth item when tau_ray is given;
+    outputs may be passed in."""
     import torch
     n = T.shape[0]
     dev = T.device
@@ -630,16 +637,19 @@ def level_populations_device(tables: CoolantTables, coolant: int, T, Td, rho, ab
     cool = cool if cool is not None else torch.zeros(n, 4, **f64)
     lines = lines if lines is not None else torch.zeros(n, 17, **f64)
     S = torch.empty(max(n * nrays * tables.nlevtot, 1), **f64)
+    if tau_ray is not None and tau_lines is None:
+        tau_lines = torch.zeros(n, 17, **f64)
     ph = phys or default_phys()
     s = stream if stream is not None else torch.cuda.current_stream(dev).cuda_stream
     ptr = lambda t: None if t is None else t.data_ptr()
     rc = load().pdrchem_level_populations_device(
         tables.h, coolant, C.byref(ph), n, ab.shape[1], nrays, pseudo_ray, ptr(T), ptr(Td), ptr(rho), ptr(ab),
         ptr(pop), ptr(count), ptr(offs), ptr(proj), ptr(step), co_fix, co_average_co, levpop_iteration, ptr(skip),
-        S.data_ptr(), solution.data_ptr(), cool.data_ptr(), lines.data_ptr(), s)
+        S.data_ptr(), solution.data_ptr(), cool.data_ptr(), lines.data_ptr(), -1 if tau_ray is None else tau_ray,
+        ptr(tau_lines), s)
     if rc != 0:
         raise PdrChemError(load().pdrchem_last_error().decode())
-    return solution, cool, lines
+    return (solution, cool, lines) if tau_ray is None else (solution, cool, lines, tau_lines)
 
 
 def lte_populations_device(tables: CoolantTables, T, ab, rho, skip=None, pop=None, stream=None):

@@ -169,6 +169,26 @@ class Network:
 
 
 # --------------------------------------------------------------------------
+# the reference's own slot numbering
+# --------------------------------------------------------------------------
+
+# The reference's shipped reduced-network ODE code (odes_reduced.c,
+# jacobian_reduced.c) numbers species slots 3-5 as CH5+, O2, H2+ where
+# species_reduced.d lists H2+, CH5+, O2: the loss terms of its ode[4] are O2's
+# reactions and its electron sum counts x[5], not x[4].  Its arrays therefore
+# hold, at the position of each species-file slot, the abundance of the
+# species-file slot mapped here; every reader of those arrays by label (the
+# heating's ABUNDANCE(NH2x), the output files' columns) sees that species.
+REFERENCE_SLOTS = {"reduced": {3: 4, 4: 5, 5: 3}}
+
+
+def reference_slot(network: str, slot: int) -> int:
+    """the species-file slot (0-based) whose abundance the reference's
+    arrays hold at position `slot`"""
+    return REFERENCE_SLOTS.get(network, {}).get(slot, slot)
+
+
+# --------------------------------------------------------------------------
 # readers
 # --------------------------------------------------------------------------
 

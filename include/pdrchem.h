@@ -374,6 +374,8 @@ int pdrchem_cloud_set_server(pdrchem_cloud *cloud, pdrchem_server *server);
 #define PDRCHEM_CLOUD_POPULATIONS 10        /* [npts*nlevtot] CII, CI, OI, CO level populations (cm^-3) */
 #define PDRCHEM_CLOUD_PREV_TEMPERATURE 11   /* [npts] previousgastemperature: the reported temperature */
 #define PDRCHEM_CLOUD_THERMAL_CONVERGED 12  /* [npts] 1 where the point's thermal balance converged */
+#define PDRCHEM_CLOUD_OPTICAL_DEPTHS 13     /* [npts*17] the lines' optical depths along ray 6 (.opdp.fin), or
+                                               along the PSEUDO_1D ray */
 int pdrchem_cloud_get(pdrchem_cloud *cloud, int field, double *out);
 /* Evaluation points: total count, then per (point, ray) the count and CSR
  * offset (offsets[npts*nrays+1]) and per evaluation point the projected
@@ -433,16 +435,21 @@ int pdrchem_coolant_tables_levels(const pdrchem_coolant_tables *tables);
  * writes d_solution[npts*nlevtot] (this coolant's levels), d_cool[npts*4]
  * (this coolant's column: line cooling, erg cm^-3 s^-1) and d_lines[npts*17]
  * (this coolant's lines among CII 158um; CI 2-1, 3-1, 3-2; OI 2-1, 3-1, 3-2;
- * CO J=1-0 .. 10-9).  The level columns are formed once per call for all
- * levels; call with coolant = -1 to do all four in turn.  Asynchronous on
- * stream; 0 on success, -1 on a bad argument. */
+ * CO J=1-0 .. 10-9) and, when d_tau_lines is not NULL, the optical depth
+ * of the same lines along ray tau_ray into d_tau_lines[npts*17] (0 for a
+ * line without one: empty upper level or |tpop| < 1e-50), which the
+ * reference writes for ray 6 to .opdp.fin (3DPDR.F90:1484-1500).  The level
+ * columns are formed once per call for all levels; call with coolant = -1 to
+ * do all four in turn.  Asynchronous on stream; 0 on success, -1 on a bad
+ * argument. */
 int pdrchem_level_populations_device(const pdrchem_coolant_tables *tables, int coolant, const pdrchem_phys *phys,
                                      int npts, int nspec, int nrays, int pseudo_ray, const double *d_T,
                                      const double *d_Td, const double *d_rho, const double *d_ab,
                                      const double *d_pop, const int *d_count, const long long *d_offs,
                                      const int *d_proj, const double *d_step, int co_fix, int co_average_co,
                                      int levpop_iteration, const unsigned char *d_skip, double *d_S,
-                                     double *d_solution, double *d_cool, double *d_lines, void *stream);
+                                     double *d_solution, double *d_cool, double *d_lines, int tau_ray,
+                                     double *d_tau_lines, void *stream);
 /* LTE populations (3DPDR.F90:520-552) of every coolant at the gas
  * temperature: d_pop[npts*nlevtot]; points with d_skip[p] != 0 untouched. */
 int pdrchem_lte_populations_device(const pdrchem_coolant_tables *tables, int npts, int nspec, const double *d_T,

@@ -72,6 +72,8 @@ class ThermalLoop:
         self.cool = np.zeros((n, 4))
         self.total = np.zeros(n)
         self.lines = np.zeros((n, 17))
+        self.tau_lines = np.zeros((n, 17))     # optical depths along tau_ray (.opdp.fin)
+        self.tau_ray = pseudo_ray if pseudo_ray >= 0 else 6
         self.heat = np.zeros((n, 12))
         self.iteration = 0
         self.levpop_iteration = 0
@@ -123,6 +125,7 @@ class ThermalLoop:
                         k = line_slot(ci, il, jl)
                         if k >= 0:
                             self.lines[p, k] = r["line"][il, jl]
+                            self.tau_lines[p, k] = r["tau"][il, jl, self.tau_ray]
         self.total = ((self.cool[:, 0] + self.cool[:, 1]) + self.cool[:, 2]) + self.cool[:, 3]
 
     # -- temperature --------------------------------------------------------

@@ -34,6 +34,7 @@ GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 # solve of the level populations carries through its condition number
 SOL_TOL = 1e-11    # of the coolant's total density
 COOL_RTOL = 1e-11
+TAU_RTOL = 1e-11
 
 
 def dev(a, dtype=torch.float64):
@@ -98,16 +99,19 @@ def test_level_populations_match_reference_fixtures(fx_lvg, tables, coolants, ci
     phys = lib.default_phys(v_turb=float(sc[0, 4]), metallicity=float(sc[0, 3]), g2d=float(sc[0, 5]))
     assert np.all(sc[:, 3:6] == sc[0, 3:6])
     print(f"level populations {key}: {nc} points, {len(proj)} evaluation points", flush=True)
-    sol, cool, lines = lib.level_populations_device(
+    tau_ray = 6 if variant == "1d" else 5
+    sol, cool, lines, tau = lib.level_populations_device(
         tables, ci, dev(sc[:, 0]), dev(sc[:, 1]), dev(sc[:, 2]), dev(ab), dev(rows), dev(count, torch.int32),
         dev(offs, torch.int64), dev(proj, torch.int32), dev(step), nrays=12, pseudo_ray=6 if variant == "1d" else -1,
-        phys=phys)
+        phys=phys, tau_ray=tau_ray)
     torch.cuda.synchronize()
-    sol, cool, lines = sol.cpu().numpy(), cool.cpu().numpy(), lines.cpu().numpy()
+    sol, cool, lines, tau = sol.cpu().numpy(), cool.cpu().numpy(), lines.cpu().numpy(), tau.cpu().numpy()
     nlev = coolants[ci].nlev
     ref_sol = fx_lvg[key + "_solution"]
     ref_cool = fx_lvg[key + "_cooling"]
     ref_line = fx_lvg[key + "_line"]
+    ref_tau = fx_lvg[key + "_tau"]
+    ref_pop = fx_lvg[key + "_pop"]
     for k in range(nc):
         tot = ref_sol[k].sum()
         assert np.max(np.abs(sol[k, off:off + nlev] - ref_sol[k])) <= SOL_TOL * tot, (k, sol[k], ref_sol[k])
@@ -118,6 +122,14 @@ def test_level_populations_match_reference_fixtures(fx_lvg, tables, coolants, ci
             s = lib_line_slot(ci, il, jl)
             if s >= 0:
                 assert nan_close(lines[:, s], ref_line[:, il, jl], COOL_RTOL), (il, jl)
+                # optical depth along tau_ray where the reference forms one
+                # (a populated upper level off the weights' ratio)
+                for k in range(nc):
+                    tpop = (ref_pop[k, jl] * coolants[ci].weights[il]) / (ref_pop[k, il] * coolants[ci].weights[jl]) \
+                        - 1.0 if ref_pop[k, il] != 0 else 0.0
+                    r = ref_tau[k, il, jl, tau_ray] if ref_pop[k, il] != 0 and abs(tpop) >= 1e-50 else 0.0
+                    scale = np.max(np.abs(ref_tau[k, il, jl]))
+                    assert abs(tau[k, s] - r) <= TAU_RTOL * abs(r) + 1e-14 * scale, (k, il, jl, tau[k, s], r)
     # the other coolants' columns are left alone
     others = [c for c in range(4) if c != ci]
     assert np.all(cool[:, others] == 0.0)
@@ -232,6 +244,8 @@ def test_thermal_loop_matches_oracle(coolants, pseudo_ray):
             hs = np.maximum(np.abs(L.heat).max(axis=1, keepdims=True), 1e-300)
             worst["heat"] = max(worst["heat"], float(np.max(np.abs(heat - L.heat) / hs)))
             assert np.allclose(c.get(CL.LINES), L.lines, rtol=1e-8, atol=1e-8 * np.abs(L.lines).max()), it
+            assert np.allclose(c.get(CL.OPTICAL_DEPTHS), L.tau_lines, rtol=1e-8,
+                               atol=1e-8 * max(np.abs(L.tau_lines).max(), 1e-300)), it
             if ref["done"]:
                 break
         print(f"  {it} iterations, {chem_steps} chemistry steps, {updates} temperature updates; worst {worst}",
