@@ -103,6 +103,23 @@ def test_device_rates_match_reference(name):
     assert int(neg.item()) == 0
 
 
+@pytest.mark.parametrize("precision", [lib.MIXED, lib.FP64])
+def test_vs_radau_truth_full(precision):
+    """the device solve of the full network against independent Radau IIA
+    solutions (tests/golden/truth_full.npz: rtol 1e-12, atol 1e-30, four slab
+    and four random points with columns)"""
+    d = np.load(os.path.join(GOLD, "truth_full.npz"))
+    ab = d["x0"].copy()
+    lib.set_globals(rtol=1e-8, atol=1e-30, start_time=0.0, end_time=float(d["t_end_yr"]), nelect=215)
+    print(f"full network vs Radau truth, precision={precision}", flush=True)
+    st = lib.calculate_abundances(ab, np.ascontiguousarray(d["rates"]), d["n_H"].copy(), d["n_H"].copy(),
+                                  precision=precision)
+    assert st == 0
+    err = max_rel(ab[:, :-1], d["radau"])
+    print(f"  max rel err vs Radau {err:.3e}", flush=True)
+    assert err <= 1e-5
+
+
 def test_host_path_full():
     """calculate_abundances_ with the full network's (nspec, nreac): the drop-in
     call, globals as the Fortran host sets them, short end time."""

@@ -166,6 +166,22 @@ def test_integrator_vs_radau(mode):
     assert rel.max() < 1e-5, rel.max()
 
 
+@pytest.mark.parametrize("mode", ["MODE_GPU_WG", "MODE_GPU_WG_MIXED", "MODE_GPU"])
+def test_integrator_vs_radau_full(mode):
+    """the full network's integrator configurations against independent
+    Radau IIA solutions (rtol 1e-12, atol 1e-30; make_truth.py full) on four
+    slab and four random points with columns: all within 1e-6"""
+    d = np.load(os.path.join(GOLD, "truth_full.npz"))
+    o = oracle.Oracle("full")
+    st, out, _ = o.calculate_abundances(d["x0"].copy(), d["rates"], d["n_H"], None, mode=getattr(oracle, mode),
+                                        nthreads=4)
+    assert st == 0
+    truth = d["radau"]
+    m = truth > 1e-20
+    rel = np.abs(out[:, :-1] - truth)[m] / truth[m]
+    assert rel.max() < 5e-6, rel.max()
+
+
 def test_calculate_abundances_semantics():
     """clip below atol, electron slot from the charge sum, status 0."""
     net = load_network("reduced")
