@@ -196,6 +196,46 @@ __device__ __forceinline__ void for_items_gc(const W *tab, int off, int n, int t
   }
 }
 
+// Workgroup barrier that orders LDS only: __syncthreads' release fence also
+// waits for the wavefront's outstanding global loads, which would expose the
+// table loads issued ahead of the next phase at every barrier.  For phases
+// that exchange data through LDS alone (the solve).
+__device__ __forceinline__ void lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// The first U words of a lane's table run, loaded ahead of the phase that
+// consumes them: issued before the preceding barrier, so that the table's
+// L2 latency overlaps the barrier instead of opening the phase.
+template <int U, typename W>
+struct Ahead {
+  W w[U];
+  __device__ __forceinline__ void load(const W *tab, int off, int n, int tid) {
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      if (u < n) w[u] = tab[(size_t)(off + u) * BLOCK + tid];
+  }
+};
+
+// for_items_gc over n items whose first min(n, U) words are already in `a`
+template <int U, typename W, typename G, typename C>
+__device__ __forceinline__ void for_items_gc_ahead(const Ahead<U, W> &a, const W *tab, int off, int n, int tid,
+                                                   G &&gather, C &&consume) {
+  using V = decltype(gather(W()));
+  {
+    V g[U];
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      if (u < n) g[u] = gather(a.w[u]);
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      if (u < n) consume(a.w[u], g[u]);
+  }
+  if (n > U) for_items_gc<U>(tab, off + U, n - U, tid, gather, consume);
+}
+
 template <typename MT>
 struct Pair {
   MT a, b;
@@ -559,11 +599,12 @@ __device__ __forceinline__ int wg_setup(Team<MT> &c, BdfU LDSA &u, BdfWG<MT> &b,
 // one chunk of a sparse mat-vec per lane: P[tid] = sum of LU[pos] * B[col]
 // over the lane's entries (pos | col << 14)
 template <typename MT>
-__device__ __forceinline__ void spmv_partials(Team<MT> &c, const uint32_t *tab_, int rounds,
-                                              const MT LDSA *LU) {
+__device__ __forceinline__ void spmv_partials(Team<MT> &c, const Ahead<8, uint32_t> &pre, const uint32_t *tab_,
+                                              int rounds, const MT LDSA *LU) {
   double acc = 0.0;
-  for_items_gc<8>(opaque(tab_), 0, rounds, c.tid, [&](uint32_t w) { return Pair<double>{(double)LU[w & 0x3fff], c.S->B[w >> 14]}; },
-                  [&](uint32_t, Pair<double> g) { acc = fma(g.a, g.b, acc); });
+  for_items_gc_ahead<8>(pre, opaque(tab_), 0, rounds, c.tid,
+                        [&](uint32_t w) { return Pair<double>{(double)LU[w & 0x3fff], c.S->B[w >> 14]}; },
+                        [&](uint32_t, Pair<double> g) { acc = fma(g.a, g.b, acc); });
   c.S->P[c.tid] = acc;
 }
 
@@ -584,18 +625,23 @@ __device__ __forceinline__ double wg_solve(Team<MT> &c, const BdfU LDSA &u, cons
   WgLDS<MT> LDSA &S = *c.S;
   const MT LDSA *LU = regionLU(c.S);
   const double gamrat = u.gamrat;
+  // each mat-vec's first table words are loaded during the phase before it
+  Ahead<8, uint32_t> pre;
+  if constexpr (NET::NHEAD > 0) pre.load(opaque(NET::s1_tab), 0, NET::S1_ROUNDS, c.tid);
   if (c.tid < NET::NB) S.B[c.tid] = c.own_ ? r * b.wsc : 0.0;
   if (c.tid == 0) S.B[NET::NB] = 0.0;
-  __syncthreads();
+  lds_sync();
   if constexpr (NET::NHEAD > 0) {
     // S1: y_H = b_H + Linv b_H
-    spmv_partials(c, NET::s1_tab, NET::S1_ROUNDS, LU);
-    __syncthreads();
+    spmv_partials(c, pre, NET::s1_tab, NET::S1_ROUNDS, LU);
+    pre.load(opaque(NET::s2_tab), 0, NET::S2_ROUNDS, c.tid);
+    lds_sync();
     if (c.tid < NET::NHEAD) S.B[NET::head_row[c.tid]] += row_total(S, NET::s1_base[c.tid], NET::s1_cnt[c.tid]);
-    __syncthreads();
+    lds_sync();
     // S2: the tail rows' gather L_TH y_H (combined in the tail step)
-    spmv_partials(c, NET::s2_tab, NET::S2_ROUNDS, LU);
-    __syncthreads();
+    spmv_partials(c, pre, NET::s2_tab, NET::S2_ROUNDS, LU);
+    pre.load(opaque(NET::s4_tab), 0, NET::S4_ROUNDS, c.tid);
+    lds_sync();
   }
   // tail: x_T = S^-1 b_T; the permuted right-hand side from wavefront 0,
   // 16-column partial products from every wavefront, combined in wavefront 0
@@ -607,32 +653,33 @@ __device__ __forceinline__ double wg_solve(Team<MT> &c, const BdfU LDSA &u, cons
     }
     S.V[b.invp] = bt;
   }
-  __syncthreads();
+  lds_sync();
   {
     double acc = 0.0;
 #pragma unroll
     for (int j = 0; j < 16; j++) acc = fma((double)b.A4[j], S.V[16 * c.wave + j], acc);
     S.P[c.wave * 64 + c.lane] = acc;
   }
-  __syncthreads();
+  lds_sync();
   if (c.wave == 0) {
     const double acc = (S.P[c.lane] + S.P[64 + c.lane]) + (S.P[128 + c.lane] + S.P[192 + c.lane]);
     S.V2[b.invp] = acc;
     wsync();
     if (c.lane < NET::NT) S.B[NET::tail_row[c.lane]] = S.V2[c.lane];
   }
-  __syncthreads();
+  lds_sync();
   if constexpr (NET::NHEAD > 0) {
     // S4: z_H = y_H - U_HT x_T
-    spmv_partials(c, NET::s4_tab, NET::S4_ROUNDS, LU);
-    __syncthreads();
+    spmv_partials(c, pre, NET::s4_tab, NET::S4_ROUNDS, LU);
+    pre.load(opaque(NET::s5_tab), 0, NET::S5_ROUNDS, c.tid);
+    lds_sync();
     if (c.tid < NET::NHEAD) S.B[NET::head_row[c.tid]] -= row_total(S, NET::s4_base[c.tid], NET::s4_cnt[c.tid]);
-    __syncthreads();
+    lds_sync();
     // S5: x_H = Winv z_H
-    spmv_partials(c, NET::s5_tab, NET::S5_ROUNDS, LU);
-    __syncthreads();
+    spmv_partials(c, pre, NET::s5_tab, NET::S5_ROUNDS, LU);
+    lds_sync();
     if (c.tid < NET::NHEAD) S.B[NET::head_row[c.tid]] = row_total(S, NET::s5_base[c.tid], NET::s5_cnt[c.tid]);
-    __syncthreads();
+    lds_sync();
   }
   double x = c.own_ ? S.B[c.tid] / b.wsc : 0.0;
   if (gamrat != 1.0) x *= 2.0 / (1.0 + gamrat);
