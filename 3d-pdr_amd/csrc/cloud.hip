@@ -1176,9 +1176,8 @@ int pdrchem_cloud_thermal_init(pdrchem_cloud *c, const pdrchem_coolant *coolants
   if (!c) return -1;
   c->err.clear();
   t_err.clear();
-  if (!coolants || !slots || !heat_index || !prm || (c->net != 0 && c->net != 1)) {
-    c->err = "bad argument to pdrchem_cloud_thermal_init (the reduced or full network, four coolants, 9 slots, "
-             "13 heating indices)";
+  if (!coolants || !slots || !heat_index || !prm) {
+    c->err = "bad argument to pdrchem_cloud_thermal_init (four coolants, 9 slots, 13 heating indices)";
     return -1;
   }
   for (int k = 0; k < 9; k++)

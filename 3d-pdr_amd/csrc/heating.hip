@@ -1,6 +1,6 @@
 // Heating terms of grid points on the device: CALC_HEATING
-// (sub_calculate_heating.F90:6-386), one thread per point, for the reduced and
-// full networks.  Row-major inputs as the solver's: abundances [npts*nspec],
+// (sub_calculate_heating.F90:6-386), one thread per point, for the three
+// networks.  Row-major inputs as the solver's: abundances [npts*nspec],
 // rates [npts*nreac]; output heat [npts*12] in the order photoelectric, PAH
 // photoelectric, Weingartner & Draine, C ionization, H2 formation, H2
 // photodissociation, FUV pumping, cosmic rays, turbulence, chemical,
@@ -13,6 +13,7 @@
 #include "../../include/pdrchem.h"
 #include "gen/net_reduced.h"
 #include "gen/wg_full.h"
+#include "gen/wg_mynetwork.h"
 #include "pdr_internal.h"
 
 #pragma clang fp contract(off)
@@ -46,6 +47,13 @@ constexpr ChemTerm kChemFull[] = {
     {S_HCOX, S_E, 730, 7.51f, 1}, {S_H3X, S_E, 706, 4.76f, 1},   {S_H3X, S_E, 705, 9.23f, 1},
     {S_H3OX, S_E, 717, 1.16f, 1}, {S_H3OX, S_E, 716, 5.63f, 1},  {S_H3OX, S_E, 714, 6.27f, 1},
     {S_HEX, S_H2, 1227, 6.51f, 1}, {S_HEX, S_H2, 265, 6.51f, 1}, {S_HEX, S_CO, 1541, 2.22f, 1}};
+// sub_calculate_heating.F90:254-259: the pre-set MYNETWORK channels (the
+// reference guards them with a STOP that its user comments out; the same
+// reactions as the full network's, under this network's numbers)
+constexpr ChemTerm kChemMynetwork[] = {
+    {S_HCOX, S_E, 185, 7.51f, 1}, {S_H3X, S_E, 173, 4.76f, 1},  {S_H3X, S_E, 172, 9.23f, 1},
+    {S_H3OX, S_E, 179, 1.16f, 1}, {S_H3OX, S_E, 178, 5.63f, 1}, {S_H3OX, S_E, 176, 6.27f, 1},
+    {S_HEX, S_H2, 297, 6.51f, 1}, {S_HEX, S_H2, 67, 6.51f, 1},  {S_HEX, S_CO, 364, 2.22f, 1}};
 
 struct HeatArgs {
   PhysParams ph;
@@ -156,8 +164,12 @@ int launch_heating(int net, const PhysParams &ph, const int *index, int npts, co
   HeatArgs a{};
   a.zeta_pts = zeta;
   a.ph = ph;
-  int nspec = net == NET_REDUCED ? pdrnet_reduced::NSPEC : pdrwg_full::NSPEC;
-  int nreac = net == NET_REDUCED ? pdrnet_reduced::NREAC : pdrwg_full::NREAC;
+  const int nspec = net == NET_REDUCED ? pdrnet_reduced::NSPEC
+                    : net == NET_FULL  ? pdrwg_full::NSPEC
+                                       : pdrwg_mynetwork::NSPEC;
+  const int nreac = net == NET_REDUCED ? pdrnet_reduced::NREAC
+                    : net == NET_FULL  ? pdrwg_full::NREAC
+                                       : pdrwg_mynetwork::NREAC;
   for (int i = 0; i < NIDX; i++) {
     a.idx[i] = index[i] - 1;
     if (a.idx[i] < 0 || a.idx[i] >= (i < R_GR ? nspec : nreac)) return -3;
@@ -175,6 +187,8 @@ int launch_heating(int net, const PhysParams &ph, const int *index, int npts, co
       return launch(heating_kernel<pdrnet_reduced::NSPEC, pdrnet_reduced::NREAC, 13>, kChemReduced, a, s);
     case NET_FULL:
       return launch(heating_kernel<pdrwg_full::NSPEC, pdrwg_full::NREAC, 9>, kChemFull, a, s);
+    case NET_MYNETWORK:
+      return launch(heating_kernel<pdrwg_mynetwork::NSPEC, pdrwg_mynetwork::NREAC, 9>, kChemMynetwork, a, s);
     default:
       return -2;
   }
@@ -189,7 +203,7 @@ extern "C" int pdrchem_heating_device_zeta(int net, const pdrchem_phys *phys, co
                                            const double *d_uv, const double *d_abundance, const double *d_rate,
                                            const double *d_zeta, double *d_heat, void *stream) {
   if (npts == 0) return 0;
-  if ((net != NET_REDUCED && net != NET_FULL) || !phys || !index || npts < 0 || !d_density || !d_T || !d_Td ||
+  if (net < 0 || net >= NET_COUNT || !phys || !index || npts < 0 || !d_density || !d_T || !d_Td ||
       !d_uv || !d_abundance || !d_rate || !d_heat)
     return -1;
   const PhysParams ph{phys->zeta, phys->v_turb, phys->omega, phys->grain_radius, phys->metallicity, phys->g2d};

@@ -428,7 +428,7 @@ def heating_device(net: int, index, density, T, Td, uv, abundance, rate, phys: O
         if not (t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()):
             raise PdrChemError("heating_device takes contiguous float64 device tensors")
     npts = density.shape[0]
-    dims = {0: (33, 329), 1: (215, 2926)}.get(net)
+    dims = dict(enumerate(NETWORKS.values())).get(net)
     if dims is None or any(t.numel() != npts for t in args[1:4]) or abundance.numel() != npts * dims[0] \
             or rate.numel() != npts * dims[1]:
         raise PdrChemError("heating_device: unsupported network or shapes that do not match it")

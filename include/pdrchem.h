@@ -132,7 +132,9 @@ int pdrchem_rates_device_zeta(int net, const pdrchem_phys *phys, const double *d
                               const double *d_zeta, int npts, double *d_rate, int *d_neg, void *stream);
 
 /* Batched CALC_HEATING (sub_calculate_heating.F90:6-386) on device memory,
- * reduced and full networks (PDRCHEM_NET ids 0 and 1; others return -1).
+ * all three networks (PDRCHEM_NET ids 0-2).  MYNETWORK's chemical heating
+ * is the reference's pre-set channel list, which its source guards with a
+ * STOP for the user to comment out (sub_calculate_heating.F90:249-260).
  * index[13]: 1-based slots as the Fortran module variables hold them -- NH,
  * NH2, NC, NCO, NHEx, NH3x, NH3Ox, NHCOx, NH2x, NELECT, then reactions NRGR,
  * NRH2, NRCI.  Per point: density, T, Td, uv [npts] (uv in Draine units, as
@@ -490,7 +492,7 @@ typedef struct {
  * coolant tables and slots as pdrchem_coolant_tables_create; heat_index[13]
  * as pdrchem_heating_device.  Sets the temperature brackets of every point
  * (3DPDR.F90:1702-1716) and, with prm->dust, the dust temperatures from the
- * field.  Works with the reduced and full networks. */
+ * field.  Works with all three networks. */
 int pdrchem_cloud_thermal_init(pdrchem_cloud *cloud, const pdrchem_coolant *coolants, const int *slots,
                                const int *heat_index, const pdrchem_thermal_params *prm);
 /* One ITERATION of the reference's outer loop (3DPDR.F90:590-1241): LTE

@@ -13,7 +13,8 @@
  * idx[13]: 0-based abundance slots of H, H2, C, CO, He+, H3+, H3O+, HCO+, H2+
  * and e-, then 0-based reactions NRGR, NRH2, NRCI.  The chemical heating uses
  * the reference's fixed reaction numbers per network (net 0 = reduced, 1 =
- * full).  Literals written without a D exponent in the reference are single
+ * full, 2 = MYNETWORK: the pre-set channels the reference lists after the
+ * STOP that guards them, sub_calculate_heating.F90:249-260).  Literals written without a D exponent in the reference are single
  * precision there and are used as such here.
  */
 #include <math.h>
@@ -95,6 +96,12 @@ void pdr_heating(const pdr_phys *ph, int net, const int *idx, double n, double T
            xH3Ox * n * xe * n * (R(717) * (SP(1.16) * ev) + R(716) * (SP(5.63) * ev) + R(714) * (SP(6.27) * ev)) +
            xHEx * n * xH2 * n * (R(1227) * (SP(6.51) * ev) + R(265) * (SP(6.51) * ev)) +
            xHEx * n * xCO * n * (R(1541) * (SP(2.22) * ev));
+  } else if (net == 2) {
+    chem = xHCOx * n * xe * n * (R(185) * (SP(7.51) * ev)) +
+           xH3x * n * xe * n * (R(173) * (SP(4.76) * ev) + R(172) * (SP(9.23) * ev)) +
+           xH3Ox * n * xe * n * (R(179) * (SP(1.16) * ev) + R(178) * (SP(5.63) * ev) + R(176) * (SP(6.27) * ev)) +
+           xHEx * n * xH2 * n * (R(297) * (SP(6.51) * ev) + R(67) * (SP(6.51) * ev)) +
+           xHEx * n * xCO * n * (R(364) * (SP(2.22) * ev));
   }
 #undef R
   /* gas-grain energy exchange */

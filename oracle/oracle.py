@@ -423,7 +423,7 @@ def ref_eval_points(xyz, level=0, theta_crit=1.3, maxpoints=600, etype=None, wor
 
 HEAT_SPECIES = ("NH", "NH2", "NC", "NCO", "NHEx", "NH3x", "NH3Ox", "NHCOx", "NH2x", "NELECT")
 HEAT_REACTIONS = ("NRGR", "NRH2", "NRCI")
-HEAT_NET = {"reduced": 0, "full": 1}
+HEAT_NET = {"reduced": 0, "full": 1, "mynetwork": 2}
 
 
 def heating_indices(net) -> np.ndarray:

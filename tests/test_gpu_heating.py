@@ -15,7 +15,7 @@ torch = pytest.importorskip("torch")
 pytestmark = pytest.mark.gpu
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-NET_ID = {"reduced": 0, "full": 1}
+NET_ID = {"reduced": 0, "full": 1, "mynetwork": 2}
 # device pow/exp/sqrt against the host libm, and the chemical channels summed
 # in another order (all positive): a few ulps per term
 RTOL = 1e-12
@@ -40,7 +40,7 @@ def test_heating_device_matches_reference_fixtures(name):
     assert np.allclose(h, z["heat"], rtol=RTOL, atol=0)
 
 
-@pytest.mark.parametrize("name,npts", [("reduced", 20000), ("full", 3000)])
+@pytest.mark.parametrize("name,npts", [("reduced", 20000), ("full", 3000), ("mynetwork", 5000)])
 def test_heating_device_matches_oracle(name, npts):
     rng = np.random.default_rng(5)
     net = load_network(name)
@@ -69,4 +69,4 @@ def test_heating_device_rejects_bad_input():
     with pytest.raises(lib.PdrChemError):
         lib.heating_device(0, bad, *args)
     with pytest.raises(lib.PdrChemError):
-        lib.heating_device(2, idx, *args)        # MYNETWORK: no device heating
+        lib.heating_device(3, idx, *args)        # no such network

@@ -190,9 +190,9 @@ STAT_KEYS = ("lvg", "thermal_points", "thermally_converged", "done", "iteration"
              "first_time")
 
 
-@pytest.mark.parametrize("pseudo_ray", [-1, 6])
-def test_thermal_loop_matches_oracle(coolants, pseudo_ray):
-    name, shape, H = "reduced", (4, 3, 3), 0.0625
+@pytest.mark.parametrize("name,pseudo_ray", [("reduced", -1), ("reduced", 6), ("mynetwork", -1)])
+def test_thermal_loop_matches_oracle(coolants, name, pseudo_ray):
+    shape, H = (4, 3, 3), 0.0625
     net = load_network(name)
     n = int(np.prod(shape))
     rho = 1e3 * np.exp(0.3 * np.random.default_rng(3).normal(size=n))
@@ -200,7 +200,7 @@ def test_thermal_loop_matches_oracle(coolants, pseudo_ray):
     try:
         c.thermal_init(coolants, lib.default_thermal_params(pseudo_ray=pseudo_ray))
         c.set_abundances(workloads.initial_abundances(net, n))
-        print(f"thermal loop {shape} pseudo_ray={pseudo_ray}: 3 chemistry iterations", flush=True)
+        print(f"thermal loop {name} {shape} pseudo_ray={pseudo_ray}: 3 chemistry iterations", flush=True)
         for _ in range(3):
             st = c.iterate(t_end=1e7, conv_tol=0.0, precision=lib.FP64)
             assert st["failed"] == 0

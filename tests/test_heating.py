@@ -67,3 +67,40 @@ def test_heating_live_reference(name):
     uv = 10 ** rng.uniform(-3, 4, npts)
     ref = O.ref_heating(name, n, T, Td, uv, ab, rate)
     assert np.allclose(O.heating(name, load_network(name), n, T, Td, uv, ab, rate), ref, rtol=RTOL, atol=0)
+
+
+def _chemical_channels(name):
+    """{reaction number: heating per unit rate} of the oracle's chemical term,
+    found by switching on one reaction at a time (every other rate zero)"""
+    net = load_network(name)
+    idx = O.heating_indices(net)
+    ab = np.full((1, net.nspec), 1.0e-3)
+    n, T, Td, uv = np.array([100.0]), np.array([50.0]), np.array([20.0]), np.array([1.0])
+    out = {}
+    for k in range(net.nreac):
+        rate = np.zeros((1, net.nreac))
+        rate[0, k] = 1.0
+        chem = O.heating(name, net, n, T, Td, uv, ab, rate)[0, 9]
+        if chem != 0.0:
+            r = net.reactions[k]
+            out[k + 1] = (tuple(sorted(x for x in r.reactants if x)), tuple(sorted(x for x in r.products if x)), chem)
+    assert idx[9] >= 0
+    return out
+
+
+def test_mynetwork_chemical_channels_match_the_full_networks():
+    """MYNETWORK's pre-set chemical-heating channels (sub_calculate_heating.F90:
+    254-259, behind the reference's STOP) are the full network's reactions
+    under MYNETWORK's own numbers: each channel's reaction has the same
+    reactants and products as a full-network channel and the same energy, and
+    the two networks have the same number of channels.  The full network's
+    channels are pinned against the reference build (the fixtures above), so
+    a wrong reaction number or energy in MYNETWORK's list fails here."""
+    full = _chemical_channels("full")
+    mine = _chemical_channels("mynetwork")
+    assert sorted(mine) == [67, 172, 173, 176, 178, 179, 185, 297, 364]
+    by_reaction = {(a, b): e for a, b, e in full.values()}
+    assert len(by_reaction) == len(full) == len(mine)
+    for k, (a, b, e) in mine.items():
+        assert (a, b) in by_reaction, (k, a, b)
+        assert e == by_reaction[(a, b)], k
