@@ -670,7 +670,54 @@ struct PointArgs {
   int *status;             // [nsys]
   int *stats;              // [nsys][8] or null
   unsigned int *counter;   // work queue
+  // standing queue (refill): null for a closed batch of nsys points.  Else
+  // the point arrays are a ring of ring_mask + 1 points in host memory that
+  // the host keeps extending while the launch runs: *published counts the
+  // points handed over so far (point k at ring position k & ring_mask), and
+  // *counter the points taken; a worker stops taking points once *stop is
+  // set, after life_ticks of its own run or after idle_ticks without work
+  // (real-time clock ticks), so every launch ends by itself.
+  const unsigned int *published;
+  const unsigned int *stop;
+  unsigned int ring_mask;
+  unsigned long long idle_ticks, life_ticks;
 };
+
+constexpr unsigned int kNoPoint = 0xffffffffu;
+
+__device__ __forceinline__ unsigned long long rt_ticks() { return __builtin_amdgcn_s_memrealtime(); }
+
+// next point number of a standing queue, or kNoPoint when the worker is to
+// stop; `born` is the worker's start (rt_ticks).  Claims only published
+// points (compare-and-swap on the counter), so a worker that stops never
+// holds a point; waits with growing sleeps between polls of the host words.
+__device__ __forceinline__ unsigned int refill_take(const PointArgs &a, unsigned long long born) {
+  const unsigned long long t0 = rt_ticks();
+  int naps = 1;
+  for (;;) {
+    const unsigned long long now = rt_ticks();
+    if (__hip_atomic_load(a.stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) != 0u || now - born > a.life_ticks)
+      return kNoPoint;
+    const unsigned int p = __hip_atomic_load(a.published, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+    unsigned int c = __hip_atomic_load(a.counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    while ((int)(p - c) > 0) {
+      if (__hip_atomic_compare_exchange_strong(a.counter, &c, c + 1u, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT))
+        return c;
+    }
+    if (now - t0 > a.idle_ticks) return kNoPoint;
+    for (int k = 0; k < naps; k++) __builtin_amdgcn_s_sleep(127);
+    if (naps < 16) naps *= 2;
+  }
+}
+
+// a point's status goes out last: in a standing queue the host takes a
+// status other than the pending value it wrote as the sign that the point's
+// abundances and counters are there, so every lane's stores are made visible system-wide
+// first (each wavefront of the team runs this before the status store)
+__device__ __forceinline__ void refill_release(const PointArgs &a) {
+  if (a.published) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+}
 
 // CVodeInit: uniform state of a fresh integration (leader writes)
 __device__ __forceinline__ void bdf_init_uniform(BdfU LDSA &u, double t0, double xe0) {

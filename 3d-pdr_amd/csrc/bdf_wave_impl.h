@@ -495,13 +495,14 @@ __device__ __forceinline__ void solve_point(Ctx<MT> &c, const PointArgs &a, int 
   wsync();
   if (ln.L == 0) {
     ab[a.nelect0] = a.xe_policy ? xe_fin : u.xe_last;
-    a.status[sys] = status;
     if (a.stats) {
       int *st = a.stats + (size_t)sys * 8;
       st[0] = u.nst; st[1] = u.nfe; st[2] = u.nje; st[3] = u.nsetups;
       st[4] = u.nni; st[5] = u.ncfn; st[6] = u.netf; st[7] = flag;
     }
   }
+  refill_release(a);
+  if (ln.L == 0) __hip_atomic_store(a.status + sys, status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
   wsync();
 }
 
@@ -534,12 +535,13 @@ bdf_wave_kernel(PointArgs a, WaveWS<MT> *ws) {
   __syncthreads();
   Ctx<MT> c;
   init_ctx<MT, NW>(c, blk, ws, a.rtol, a.atol);
+  const unsigned long long born = rt_ticks();
   for (;;) {
     unsigned int sys = 0;
-    if (c.ln.L == 0) sys = atomicAdd(a.counter, 1u);
+    if (c.ln.L == 0) sys = a.published ? refill_take(a, born) : atomicAdd(a.counter, 1u);
     sys = (unsigned int)uni((int)sys);
-    if (sys >= (unsigned int)a.nsys) break;
-    solve_point(c, a, (int)sys);
+    if (a.published ? sys == kNoPoint : sys >= (unsigned int)a.nsys) break;
+    solve_point(c, a, (int)(a.published ? sys & a.ring_mask : sys));
   }
 }
 

@@ -78,6 +78,11 @@ void fill_args(Args &a, const SolveRequest &r) {
   a.status = r.status;
   a.stats = r.stats;
   a.counter = r.counter;
+  a.published = r.published;
+  a.stop = r.stop;
+  a.ring_mask = r.ring_mask;
+  a.idle_ticks = r.idle_ticks;
+  a.life_ticks = r.life_ticks;
 }
 
 template <typename Args>

@@ -752,12 +752,18 @@ __device__ __forceinline__ void wg_solve_point(Team<MT> &c, const PointArgs &a, 
   __syncthreads();
   if (c.tid == 0) {
     ab[a.nelect0] = a.xe_policy ? xe_fin : u.xe_last;
-    a.status[sys] = status;
     if (a.stats) {
       int *st = a.stats + (size_t)sys * 8;
       st[0] = u.nst; st[1] = u.nfe; st[2] = u.nje; st[3] = u.nsetups;
       st[4] = u.nni; st[5] = u.ncfn; st[6] = u.netf; st[7] = flag;
     }
+  }
+  if (a.published) {
+    refill_release(a);
+    __syncthreads();
+  }
+  if (c.tid == 0) {
+    __hip_atomic_store(a.status + sys, status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
   }
   __syncthreads();
 }
@@ -793,13 +799,14 @@ bdf_wg_kernel(PointArgs a, unsigned char *ws) {
   __shared__ unsigned int next;
   Team<MT> c;
   init_team(c, lds, ws, a.rtol, a.atol);
+  const unsigned long long born = rt_ticks();
   for (;;) {
-    if (c.tid == 0) next = atomicAdd(a.counter, 1u);
+    if (c.tid == 0) next = a.published ? refill_take(a, born) : atomicAdd(a.counter, 1u);
     __syncthreads();
     const unsigned int sys = next;
     __syncthreads();
-    if (sys >= (unsigned int)a.nsys) break;
-    wg_solve_point(c, a, (int)sys);
+    if (a.published ? sys == kNoPoint : sys >= (unsigned int)a.nsys) break;
+    wg_solve_point(c, a, (int)(a.published ? sys & a.ring_mask : sys));
   }
 }
 

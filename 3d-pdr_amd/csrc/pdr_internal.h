@@ -47,7 +47,18 @@ struct SolveRequest {
   // workspace fits in it and allocates nothing.
   void *workspace;
   size_t workspace_bytes;
+  // standing queue (all null/zero for a closed batch): the point arrays are
+  // a ring of ring_mask + 1 points in host memory; published and stop are
+  // host words, idle/life bounds in real-time clock ticks (PointArgs)
+  const unsigned int *published;
+  const unsigned int *stop;
+  unsigned int ring_mask;
+  unsigned long long idle_ticks, life_ticks;
 };
+
+// status of a standing-queue point not yet solved (the host writes it before
+// publishing the point; the solver's statuses are small integers)
+constexpr int kPendingStatus = -2147483647 - 1;
 
 // library-wide solver options (pdrchem_set_mxstep, pdrchem_set_electron_policy)
 struct SolverOptions {

@@ -24,6 +24,21 @@
 // the capture allocates.  PDRCHEM_SERVER_GRAPHS=0 in the environment at
 // creation selects plain stream launches instead.
 //
+// Standing launches (pdrchem_server_set_refill) replace the batches for
+// host requests: each network keeps a ring of points in pinned host memory
+// that the solver reads and writes in place, and a persistent launch that
+// keeps taking points from it while the host appends new requests -- a
+// request's points start as soon as any worker of the running launch is
+// free, instead of waiting for a slot whose batch may hold a straggler.  The
+// host publishes a request by copying it into the ring and advancing the
+// published count; workers claim published points one at a time and write
+// each point's status last, which is how the host sees it finish.  A
+// launch's workers stop after an idle spell or a lifetime, so launches of
+// the networks take turns on the device and every launch ends by itself;
+// the host starts a successor on the lane's second stream before a launch's
+// lifetime runs out, and a new launch whenever work is waiting and none
+// runs.
+//
 // Requests may also hold device arrays (pdrchem_server_submit_device): their
 // batch gathers inputs into the arena block by device-to-device copies and
 // scatters abundances (and, if asked, per-point statuses and integrator
@@ -39,6 +54,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/pdrchem.h"
@@ -137,6 +153,38 @@ struct Slot {
   hipGraphExec_t exec = nullptr;  // executable graph of the last batch, updated per batch
 };
 
+// a host request held in a standing-launch ring
+struct LaneReq {
+  int ticket, npts;
+  unsigned int first;  // point number of its first point (ring position first & mask)
+  double *ab;          // caller's array, written when the request completes
+  double t_submit;
+  int checked = 0;     // leading points seen finished
+  int status = 0;
+  bool done = false;
+};
+
+// one network's standing launches: the ring (pinned, coherent host memory,
+// read and written by the solver in place), the published count and stop
+// word (host memory), the device counter of points taken, two streams that
+// alternate launches
+struct Lane {
+  bool ready = false;
+  unsigned int cap = 0, mask = 0;
+  char *ring = nullptr;
+  size_t off_rate = 0, off_dens = 0, off_tend = 0, off_status = 0;
+  unsigned int *ctl = nullptr;  // [0] published, [1] stop
+  unsigned int *d_counter = nullptr;
+  unsigned int head = 0, tail = 0;  // next point number to publish; oldest point still held
+  std::deque<LaneReq> held;         // published requests, in ring order
+  std::deque<Req> waiting;          // requests waiting for ring space
+  hipStream_t st[2] = {nullptr, nullptr};
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  bool launched[2] = {false, false};
+  double t_launch[2] = {0.0, 0.0};
+  long long launches = 0;
+};
+
 }  // namespace
 
 struct pdrchem_server {
@@ -152,6 +200,11 @@ struct pdrchem_server {
   bool graphs = true;
   long long graph_launches = 0, graph_instantiations = 0;
   std::string err;
+  // standing launches for host requests (0 points: batches)
+  unsigned int refill_points = 0;
+  double idle_s = 2e-3, life_s = 0.25;
+  double tick_hz = 1e8;  // real-time clock of the device
+  Lane lanes[NET_COUNT];
 };
 
 namespace {
@@ -331,6 +384,220 @@ void retire(pdrchem_server *S) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// standing launches
+// ---------------------------------------------------------------------------
+void lane_free(Lane &ln);
+
+bool lane_setup(pdrchem_server *S, Lane &ln, int net) {
+  if (ln.ready) return true;
+  const Shape &s = kShape[net];
+  const size_t cap = S->refill_points;
+  ln.off_rate = align256(cap * s.nspec * 8);
+  ln.off_dens = ln.off_rate + align256(cap * s.nreac * 8);
+  ln.off_tend = ln.off_dens + align256(cap * 8);
+  ln.off_status = ln.off_tend + align256(cap * 8);
+  const size_t bytes = ln.off_status + align256(cap * 4);
+  const int mixed = S->precision == PDRCHEM_MIXED ? 1 : 0;
+  const size_t ws = net == NET_REDUCED ? wave_solver_workspace_bytes(net, mixed) : wg_solver_workspace_bytes(net, mixed);
+  bool ok = hipHostMalloc((void **)&ln.ring, bytes, hipHostMallocCoherent) == hipSuccess &&
+            hipHostMalloc((void **)&ln.ctl, 256, hipHostMallocCoherent) == hipSuccess &&
+            hipMalloc(&ln.d_counter, sizeof(unsigned int)) == hipSuccess &&
+            hipMemset(ln.d_counter, 0, sizeof(unsigned int)) == hipSuccess;
+  for (int k = 0; ok && k < 2; k++)
+    ok = hipStreamCreateWithFlags(&ln.st[k], hipStreamNonBlocking) == hipSuccess &&
+         hipEventCreateWithFlags(&ln.ev[k], hipEventDisableTiming) == hipSuccess && stream_workspace(ln.st[k], ws);
+  if (!ok) {
+    lane_free(ln);
+    S->err = "standing launch set-up failed (pinned ring or device memory)";
+    return false;
+  }
+  ln.ctl[0] = 0;
+  ln.ctl[1] = 0;
+  ln.cap = (unsigned int)cap;
+  ln.mask = ln.cap - 1;
+  ln.head = ln.tail = 0;
+  ln.ready = true;
+  return true;
+}
+
+void lane_free(Lane &ln) {
+  if (ln.ctl) __atomic_store_n(&ln.ctl[1], 1u, __ATOMIC_RELEASE);
+  for (int k = 0; k < 2; k++) {
+    if (ln.st[k]) (void)hipStreamSynchronize(ln.st[k]);
+    if (ln.ev[k]) (void)hipEventDestroy(ln.ev[k]);
+    if (ln.st[k]) (void)hipStreamDestroy(ln.st[k]);
+  }
+  if (ln.d_counter) (void)hipFree(ln.d_counter);
+  if (ln.ring) (void)hipHostFree(ln.ring);
+  if (ln.ctl) (void)hipHostFree(ln.ctl);
+  ln = Lane();
+}
+
+// copy n rows of `row` bytes between the ring (from point number k) and a
+// flat array, in at most two pieces around the ring's end
+void ring_rows(const Lane &ln, char *base, unsigned int k, size_t n, size_t row, char *flat, bool to_ring) {
+  const size_t pos = k & ln.mask;
+  const size_t n1 = std::min(n, (size_t)ln.cap - pos);
+  char *r1 = base + pos * row;
+  if (to_ring) {
+    std::memcpy(r1, flat, n1 * row);
+    if (n > n1) std::memcpy(base, flat + n1 * row, (n - n1) * row);
+  } else {
+    std::memcpy(flat, r1, n1 * row);
+    if (n > n1) std::memcpy(flat + n1 * row, base, (n - n1) * row);
+  }
+}
+
+// waiting requests into the ring while they fit, then one publication
+void lane_admit(pdrchem_server *S, Lane &ln, int net) {
+  const Shape &s = kShape[net];
+  bool any = false;
+  while (!ln.waiting.empty()) {
+    const Req &r = ln.waiting.front();
+    if ((size_t)(ln.head - ln.tail) + (size_t)r.npts > ln.cap) break;
+    const unsigned int k = ln.head;
+    ring_rows(ln, ln.ring, k, r.npts, (size_t)s.nspec * 8, (char *)r.ab, true);
+    ring_rows(ln, ln.ring + ln.off_rate, k, r.npts, (size_t)s.nreac * 8, (char *)r.rate, true);
+    ring_rows(ln, ln.ring + ln.off_dens, k, r.npts, 8, (char *)r.dens, true);
+    double *tend = (double *)(ln.ring + ln.off_tend);
+    int *status = (int *)(ln.ring + ln.off_status);
+    for (int i = 0; i < r.npts; i++) {
+      tend[(k + i) & ln.mask] = r.tend_s;
+      status[(k + i) & ln.mask] = kPendingStatus;
+    }
+    LaneReq h;
+    h.ticket = r.ticket;
+    h.npts = r.npts;
+    h.first = k;
+    h.ab = r.ab;
+    h.t_submit = r.t_submit;
+    ln.held.push_back(h);
+    ln.head += (unsigned int)r.npts;
+    ln.waiting.pop_front();
+    any = true;
+  }
+  if (any) __atomic_store_n(&ln.ctl[0], ln.head, __ATOMIC_RELEASE);
+  (void)S;
+}
+
+// finished requests: abundances back to the callers, ring space freed in order
+void lane_retire(pdrchem_server *S, Lane &ln, int net) {
+  const Shape &s = kShape[net];
+  const int *status = (const int *)(ln.ring + ln.off_status);
+  for (LaneReq &h : ln.held) {
+    if (h.done) continue;
+    while (h.checked < h.npts) {
+      const int st = __atomic_load_n(&status[(h.first + (unsigned int)h.checked) & ln.mask], __ATOMIC_ACQUIRE);
+      if (st == kPendingStatus) break;
+      if (h.status == 0) h.status = st;
+      h.checked++;
+    }
+    if (h.checked < h.npts) continue;
+    ring_rows(ln, ln.ring, h.first, h.npts, (size_t)s.nspec * 8, (char *)h.ab, false);
+    h.done = true;
+    S->done.push_back(Done{h.ticket, h.status, (now_s() - h.t_submit) * 1e3});
+  }
+  while (!ln.held.empty() && ln.held.front().done) {
+    ln.tail += (unsigned int)ln.held.front().npts;
+    ln.held.pop_front();
+  }
+}
+
+bool lane_launch(pdrchem_server *S, Lane &ln, int net, int k) {
+  const Shape &s = kShape[net];
+  SolveRequest q{};
+  q.abundance = (double *)ln.ring;
+  q.rate = (const double *)(ln.ring + ln.off_rate);
+  q.density = (const double *)(ln.ring + ln.off_dens);
+  q.npts = (int)ln.cap;  // the launch fills the device
+  q.nspec = s.nspec;
+  q.nreac = s.nreac;
+  q.rtol = S->rtol;
+  q.atol = S->atol;
+  q.t0 = 0.0;
+  q.tend = 0.0;
+  q.tend_pts = (const double *)(ln.ring + ln.off_tend);
+  q.tout0 = 1.0e-4 * kYear;  // calculate_abundances.c:214
+  q.nelect0 = s.nspec - 1;
+  q.mxstep = solver_options().mxstep;
+  q.xe_policy = solver_options().xe_policy;
+  q.status = (int *)(ln.ring + ln.off_status);
+  q.stats = nullptr;
+  q.counter = ln.d_counter;
+  q.mixed = S->precision == PDRCHEM_MIXED ? 1 : 0;
+  q.published = &ln.ctl[0];
+  q.stop = &ln.ctl[1];
+  q.ring_mask = ln.mask;
+  q.idle_ticks = (unsigned long long)(S->idle_s * S->tick_hz);
+  q.life_ticks = (unsigned long long)(S->life_s * S->tick_hz);
+  const int rc = net == NET_REDUCED ? launch_wave_solver(net, q, ln.st[k]) : launch_wg_solver(net, q, ln.st[k]);
+  if (rc != 0 || hipEventRecord(ln.ev[k], ln.st[k]) != hipSuccess) {
+    // nothing more runs on this lane: once its streams are idle no launch
+    // can touch the ring, and its requests complete with the set-up failure
+    // status of calculate_abundances_ (1), their arrays untouched
+    __atomic_store_n(&ln.ctl[1], 1u, __ATOMIC_RELEASE);
+    for (int j = 0; j < 2; j++) (void)hipStreamSynchronize(ln.st[j]);
+    const double t = now_s();
+    for (const LaneReq &h : ln.held)
+      if (!h.done) S->done.push_back(Done{h.ticket, 1, (t - h.t_submit) * 1e3});
+    for (const Req &r : ln.waiting) S->done.push_back(Done{r.ticket, 1, (t - r.t_submit) * 1e3});
+    lane_free(ln);
+    S->err = "standing launch failed";
+    return false;
+  }
+  ln.launched[k] = true;
+  ln.t_launch[k] = now_s();
+  ln.launches++;
+  return true;
+}
+
+// keep a launch running while points wait: one when none runs, and a
+// successor on the other stream once the running one is half way through
+// its lifetime (its workers then stop at the lifetime and hand their
+// compute units over)
+bool lane_pump(pdrchem_server *S, Lane &ln, int net) {
+  bool work = false;
+  for (const LaneReq &h : ln.held)
+    if (!h.done) {
+      work = true;
+      break;
+    }
+  if (!work) return true;
+  bool live[2];
+  for (int k = 0; k < 2; k++) {
+    if (ln.launched[k] && hipEventQuery(ln.ev[k]) == hipSuccess) ln.launched[k] = false;
+    live[k] = ln.launched[k];
+  }
+  const double t = now_s();
+  if (!live[0] && !live[1]) return lane_launch(S, ln, net, 0);
+  if (live[0] != live[1]) {
+    const int k = live[0] ? 0 : 1;
+    if (t - ln.t_launch[k] > 0.5 * S->life_s) return lane_launch(S, ln, net, 1 - k);
+  }
+  return true;
+}
+
+bool lanes_step(pdrchem_server *S) {
+  for (int net = 0; net < NET_COUNT; net++) {
+    Lane &ln = S->lanes[net];
+    if (!ln.ready) continue;
+    lane_retire(S, ln, net);
+    lane_admit(S, ln, net);
+    if (!lane_pump(S, ln, net)) return false;
+  }
+  return true;
+}
+
+int lanes_outstanding(const pdrchem_server *S) {
+  int n = 0;
+  for (const Lane &ln : S->lanes) {
+    n += (int)ln.waiting.size();
+    for (const LaneReq &h : ln.held) n += h.done ? 0 : 1;
+  }
+  return n;
+}
+
 // start batches on free slots while pending requests fit
 bool admit(pdrchem_server *S) {
   for (Slot &sl : S->slots) {
@@ -431,9 +698,22 @@ int pdrchem_server_submit(pdrchem_server *S, int net, int npts, double *abundanc
                           const double *density, double t_end_years) {
   if (!S || net < 0 || net >= NET_COUNT || npts <= 0 || !abundance || !rate || !density) return -1;
   std::lock_guard<std::mutex> lock(S->mu);
-  if ((size_t)npts > S->max_batch_pts || layout(net, npts).total > S->arena.cap) {
+  if (!S->refill_points && ((size_t)npts > S->max_batch_pts || layout(net, npts).total > S->arena.cap)) {
     S->err = "request larger than the serving budget";
     return -1;
+  }
+  if (S->refill_points) {
+    Lane &ln = S->lanes[net];
+    if (!lane_setup(S, ln, net)) return -1;
+    if ((unsigned int)npts > ln.cap) {
+      S->err = "request larger than the standing launch's ring";
+      return -1;
+    }
+    Req r{S->next_ticket++, net, npts, abundance, rate, density, t_end_years * kYear, now_s()};
+    ln.waiting.push_back(r);
+    lane_admit(S, ln, net);
+    if (!lane_pump(S, ln, net)) return -1;
+    return r.ticket;
   }
   Req r{S->next_ticket++, net, npts, abundance, rate, density, t_end_years * kYear, now_s()};
   S->pending.push_back(r);
@@ -469,24 +749,37 @@ int pdrchem_server_set_max_batch(pdrchem_server *S, int npts) {
 
 int pdrchem_server_progress(pdrchem_server *S, int wait) {
   if (!S) return -1;
-  std::lock_guard<std::mutex> lock(S->mu);
+  std::unique_lock<std::mutex> lock(S->mu);
   for (;;) {
     retire(S);
     if (!admit(S)) return -1;
+    if (!lanes_step(S)) return -1;
     int busy = 0;
     for (const Slot &sl : S->slots) busy += sl.busy;
-    if (!wait || (busy == 0 && S->pending.empty())) return busy + (int)S->pending.size();
-    for (Slot &sl : S->slots)
-      if (sl.busy) {
-        (void)hipEventSynchronize(sl.done);
-        break;
-      }
+    const int lane_work = lanes_outstanding(S);
+    if (!wait || (busy == 0 && S->pending.empty() && lane_work == 0))
+      return busy + (int)S->pending.size() + lane_work;
+    if (busy) {
+      for (Slot &sl : S->slots)
+        if (sl.busy) {
+          (void)hipEventSynchronize(sl.done);
+          break;
+        }
+    } else {
+      // standing launches finish points one by one: poll the ring with the
+      // lock released, so that other threads can submit meanwhile
+      lock.unlock();
+      std::this_thread::sleep_for(std::chrono::microseconds(100));
+      lock.lock();
+    }
   }
 }
 
 int pdrchem_server_poll(pdrchem_server *S, int max, int *tickets, int *status, double *latency_ms) {
   if (!S || max < 0) return -1;
   std::lock_guard<std::mutex> lock(S->mu);
+  for (int net = 0; net < NET_COUNT; net++)
+    if (S->lanes[net].ready) lane_retire(S, S->lanes[net], net);
   const int n = (int)std::min<size_t>((size_t)max, S->done.size());
   for (int i = 0; i < n; i++) {
     if (tickets) tickets[i] = S->done[i].ticket;
@@ -499,6 +792,7 @@ int pdrchem_server_poll(pdrchem_server *S, int max, int *tickets, int *status, d
 
 void pdrchem_server_destroy(pdrchem_server *S) {
   if (!S) return;
+  for (Lane &ln : S->lanes) lane_free(ln);
   for (Slot &sl : S->slots) {
     if (sl.stream) (void)hipStreamSynchronize(sl.stream);
     if (sl.exec) (void)hipGraphExecDestroy(sl.exec);
@@ -509,6 +803,53 @@ void pdrchem_server_destroy(pdrchem_server *S) {
   }
   if (S->arena.base) (void)hipFree(S->arena.base);
   delete S;
+}
+
+int pdrchem_server_set_refill(pdrchem_server *S, int ring_points, double idle_ms, double life_ms) {
+  if (!S || ring_points < 0 || (ring_points & (ring_points - 1)) != 0 || ring_points > (1 << 24) ||
+      (ring_points && (ring_points < 64 || !(idle_ms > 0.0) || !(life_ms > 0.0) || idle_ms > 6e4 || life_ms > 6e5)))
+    return -1;
+  std::lock_guard<std::mutex> lock(S->mu);
+  if (S->refill_points == (unsigned int)ring_points) {
+    if (ring_points) {
+      S->idle_s = idle_ms * 1e-3;
+      S->life_s = life_ms * 1e-3;
+    }
+    return 0;
+  }
+  for (const Lane &ln : S->lanes)
+    if (!ln.waiting.empty() || !ln.held.empty()) {
+      S->err = "standing launches hold requests";
+      return -1;
+    }
+  for (Lane &ln : S->lanes) lane_free(ln);
+  if (ring_points) {
+    int dev = 0, khz = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz <= 0) {
+      S->err = "no real-time clock rate for the device";
+      return -1;
+    }
+    S->tick_hz = 1e3 * khz;
+    S->idle_s = idle_ms * 1e-3;
+    S->life_s = life_ms * 1e-3;
+  }
+  S->refill_points = (unsigned int)ring_points;
+  return 0;
+}
+
+int pdrchem_server_refill_stats(pdrchem_server *S, long long *launches, int *held_points) {
+  if (!S) return -1;
+  std::lock_guard<std::mutex> lock(S->mu);
+  long long n = 0;
+  int held = 0;
+  for (const Lane &ln : S->lanes) {
+    n += ln.launches;
+    held += (int)(ln.head - ln.tail);
+  }
+  if (launches) *launches = n;
+  if (held_points) *held_points = held;
+  return (int)S->refill_points;
 }
 
 int pdrchem_server_graph_stats(pdrchem_server *S, long long *launches, long long *instantiations) {

@@ -27,6 +27,7 @@ EXPORTED_FUNCTIONS = [
     "pdrchem_server_create", "pdrchem_server_submit", "pdrchem_server_progress", "pdrchem_server_poll",
     "pdrchem_server_last_error", "pdrchem_server_destroy", "pdrchem_server_graph_stats",
     "pdrchem_server_submit_device", "pdrchem_server_max_batch", "pdrchem_server_set_max_batch",
+    "pdrchem_server_set_refill", "pdrchem_server_refill_stats",
     "pdrchem_cloud_set_server",
     "pdrchem_cloud_create_points", "pdrchem_trace_points", "pdrchem_cloud_set_zeta",
     "pdrchem_cloud_create", "pdrchem_cloud_set_abundances", "pdrchem_cloud_iterate", "pdrchem_cloud_get",
@@ -164,6 +165,8 @@ def load(path: Optional[str] = None):
     L.pdrchem_server_submit_device.argtypes = [vp, C.c_int, C.c_int, vp, vp, vp, C.c_double, vp, vp]
     L.pdrchem_server_max_batch.argtypes = [vp]
     L.pdrchem_server_set_max_batch.argtypes = [vp, C.c_int]
+    L.pdrchem_server_set_refill.argtypes = [vp, C.c_int, C.c_double, C.c_double]
+    L.pdrchem_server_refill_stats.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_int)]
     L.pdrchem_cloud_set_server.argtypes = [vp, vp]
     L.pdrchem_cloud_set_zeta.argtypes = [vp, dp]
     L.pdrchem_cloud_create_points.argtypes = [C.c_int, C.c_int, dp, ip, C.c_int, dp, C.c_int, C.c_double, C.c_int,
@@ -533,6 +536,18 @@ class Server:
     def set_max_batch(self, npts: int) -> None:
         if self.L.pdrchem_server_set_max_batch(self.h, npts) != 0:
             raise PdrChemError("bad batch size")
+
+    def set_refill(self, ring_points: int, idle_ms: float = 2.0, life_ms: float = 250.0) -> None:
+        """Standing launches for host requests (ring_points a power of two,
+        >= 64; 0: batches).  Refused while standing launches hold requests."""
+        if self.L.pdrchem_server_set_refill(self.h, ring_points, idle_ms, life_ms) != 0:
+            raise PdrChemError("set_refill: " + self.L.pdrchem_server_last_error(self.h).decode())
+
+    def refill_stats(self):
+        """(ring points or 0, standing launches so far, points held in the rings)."""
+        la, held = C.c_longlong(), C.c_int()
+        r = self.L.pdrchem_server_refill_stats(self.h, C.byref(la), C.byref(held))
+        return r, la.value, held.value
 
     def graph_stats(self):
         """(uses_graphs, graph launches, graph instantiations)."""

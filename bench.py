@@ -257,6 +257,10 @@ def main():
                     help="networks of the requests, chosen with equal odds")
     ap.add_argument("--serve-batch", type=int, default=65536,
                     help="largest batch of the request loop (points)")
+    ap.add_argument("--serve-refill", type=int, default=0,
+                    help="ring points per network of standing launches (power of two; 0: batches)")
+    ap.add_argument("--serve-idle-ms", type=float, default=2.0, help="standing launch idle limit")
+    ap.add_argument("--serve-life-ms", type=float, default=250.0, help="standing launch lifetime")
     ap.add_argument("--no-autotune", action="store_true",
                     help="launch with the default shape instead of the autotuner's choice")
     ap.add_argument("--cloud", action="store_true",
@@ -470,27 +474,47 @@ def serve(a):
                 tuned[f"{name}:{n}"] = (r["systems_per_block"], r["blocks_per_cu"])
     srv = lib.Server(device_bytes=4 << 30, nslots=4, precision=prec)
     srv.set_max_batch(a.serve_batch)
+    if a.serve_refill:
+        srv.set_refill(a.serve_refill, idle_ms=a.serve_idle_ms, life_ms=a.serve_life_ms)
     lat, status = {}, {}
+    size = {}
+    backlog = []            # (seconds, points submitted and not yet done)
     t0 = time.perf_counter()
     i = 0
+    open_pts = 0
+    t_report = 0.0
     while i < len(R) or len(lat) < len(R):
         now = time.perf_counter() - t0
         while i < len(R) and R[i].arrival <= now:
             r, d = R[i], data[R[i].network]
             sl = slice(r.first, r.first + r.npts)
-            srv.submit(d["nid"], d["ab"][sl], d["rate"][sl], d["nH"][sl], r.t_end)
+            tk = srv.submit(d["nid"], d["ab"][sl], d["rate"][sl], d["nH"][sl], r.t_end)
+            size[tk] = (r.npts, r.network)
+            open_pts += r.npts
             i += 1
         srv.progress()
         for tk, st, ms in srv.poll():
             lat[tk], status[tk] = ms, st
+            open_pts -= size[tk][0]
+        backlog.append((now, open_pts))
+        if now - t_report > 30.0:
+            t_report = now
+            print(f"serving: {now:.0f} s, {i} of {len(R)} submitted, {len(lat)} done, backlog {open_pts} points",
+                  file=sys.stderr, flush=True)
         if i < len(R):
             time.sleep(max(0.0, min(0.002, R[i].arrival - (time.perf_counter() - t0))))
         else:
             time.sleep(0.001)
     wall = time.perf_counter() - t0
+    refill = srv.refill_stats()
     srv.close()
     L = np.array([lat[k] for k in sorted(lat)])
     npts = sum(r.npts for r in R)
+    B = np.array(backlog)
+    t_last = R[-1].arrival
+    during = B[B[:, 0] <= t_last]
+    # queue growth over the arrival window: least-squares slope of the backlog
+    growth = float(np.polyfit(during[:, 0], during[:, 1], 1)[0]) if len(during) > 2 else 0.0
     res = {
         "metric": "grid-point chemistry solves/s at <=1e-4 rel err vs FP64 CVODE; p50/p99 batch ms",
         "value": npts / wall, "unit": "solves/s", "n_gpus": 1, "steps": 1, "warmup": 0,
@@ -505,6 +529,16 @@ def serve(a):
                    "max_batch": a.serve_batch},
         "failed_requests": int(sum(1 for v in status.values() if v != 0)),
         "launch_shapes": tuned or None,
+        "mode": (f"standing launches (ring {a.serve_refill} points per network, idle {a.serve_idle_ms} ms, "
+                 f"life {a.serve_life_ms} ms)" if a.serve_refill else "batches"),
+        "standing_launches": refill[1] if a.serve_refill else None,
+        "backlog_points_at_last_arrival": int(during[-1, 1]) if len(during) else 0,
+        "backlog_points_max": int(B[:, 1].max()),
+        "backlog_growth_points_per_s": growth,
+        "arrival_points_per_s": npts / max(t_last, 1e-9),
+        "per_network": {nm: {"requests": len(v), "p50_ms": float(np.percentile(v, 50)),
+                             "p99_ms": float(np.percentile(v, 99))}
+                        for nm in nets for v in [[lat[k] for k in lat if size[k][1] == nm]] if v},
     }
     print(json.dumps(res))
 

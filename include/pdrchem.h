@@ -276,6 +276,23 @@ int pdrchem_server_progress(pdrchem_server *server, int wait);
  * not be launched) and submit-to-completion latency in ms.  Returns the count. */
 int pdrchem_server_poll(pdrchem_server *server, int max, int *tickets, int *status, double *latency_ms);
 
+/* Standing launches for host requests (pdrchem_server_submit) instead of
+ * batches; no counterpart in the reference.  Each network gets a ring of
+ * ring_points points (a power of two, 64 .. 2^24; 0 turns standing launches
+ * off) in pinned host memory, allocated on the network's first request.  A
+ * request is copied into the ring and published; a persistent solver launch
+ * keeps taking published points while it runs, so a request starts as soon
+ * as a worker is free rather than when a whole batch slot is.  Workers stop
+ * after idle_ms without work or life_ms of running; the server keeps a
+ * launch running while points wait (a successor is started half way through
+ * a launch's life).  Results, statuses and latencies come back through
+ * pdrchem_server_progress / pdrchem_server_poll as for batches; a request
+ * larger than the ring is refused.  Device requests keep using batches.
+ * Refused (-1) while standing launches hold requests; 0 on success. */
+int pdrchem_server_set_refill(pdrchem_server *server, int ring_points, double idle_ms, double life_ms);
+/* Standing launches started so far and points now held in the rings;
+ * returns the ring size (0: standing launches off), -1 on a null server. */
+int pdrchem_server_refill_stats(pdrchem_server *server, long long *launches, int *held_points);
 /* Batches go to the device as HIP graphs (capture of copy-in, work-queue
  * reset, solver launch and copy-out on the slot's stream; one executable
  * graph per slot, updated in place per batch).  Returns 1 if the server

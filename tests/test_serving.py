@@ -3,8 +3,8 @@
 CPU: the seeded request stream of the serving workload (Poisson arrivals,
 log-uniform sizes and end times) and the library's exports.
 GPU: requests of two networks and many end times submitted while others run
-are batched together on several streams; every request's result must be
-bit-identical to the same points solved alone with the same end time (each
+are batched together on several streams, or taken point by point by
+standing launches; every request's result must be bit-identical to the same points solved alone with the same end time (each
 point is integrated by its own wavefront / workgroup with a fixed order of
 operations, whatever else shares the launch), and the bookkeeping (tickets,
 statuses, latencies) must account for every request.
@@ -163,4 +163,116 @@ def test_server_rejects_oversized():
     ab = np.zeros((n, 215))
     with pytest.raises(lib.PdrChemError):
         srv.submit(lib.network_id("full"), ab, np.zeros((n, 2926)), np.ones(n), 1e3)
+    srv.close()
+
+
+def _drain(srv, n, seconds=120.0):
+    """progress/poll until n requests are done (bounded: fails rather than
+    waits for ever if results stop coming)"""
+    import time
+    done, t0 = [], time.perf_counter()
+    while len(done) < n:
+        assert time.perf_counter() - t0 < seconds, f"{len(done)} of {n} requests done after {seconds} s"
+        srv.progress()
+        done += srv.poll()
+        time.sleep(0.001)
+    return done
+
+
+def _alone(name, x0, r, nH, te, prec):
+    import torch
+    from pdrchem import lib
+    dev = torch.device("cuda:0")
+    ab = torch.tensor(x0, device=dev)
+    status, _ = lib.solve_device(lib.network_id(name), ab, torch.tensor(r, device=dev), torch.tensor(nH, device=dev),
+                                 t_end=te, precision=prec)
+    torch.cuda.synchronize()
+    return ab.cpu().numpy(), status.cpu().numpy()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec_name", ["mixed", "fp64"])
+def test_refill_requests_equal_alone(prec_name):
+    """Standing launches (pdrchem_server_set_refill): requests of two networks
+    and many end times, more points than a ring holds (so requests wrap round
+    the ring's end and wait for space), submitted in two waves, the second
+    while the first runs.  Every request's result is bitwise that of the same
+    points solved alone, every ticket comes back once with status 0, and no
+    point stays held."""
+    import time
+    import oracle
+    from pdrchem import lib
+    prec = lib.MIXED if prec_name == "mixed" else lib.FP64
+    orc = {nm: oracle.Oracle(nm) for nm in ("reduced", "full")}
+    sizes = [("reduced", 37), ("full", 3), ("reduced", 200), ("reduced", 1), ("full", 5), ("reduced", 64),
+             ("full", 2), ("reduced", 9), ("reduced", 150), ("full", 60), ("reduced", 100), ("full", 70)]
+    reqs = _requests(orc, sizes, seed=5)
+    lib.set_globals(rtol=1e-8, atol=1e-30)
+    srv = lib.Server(device_bytes=64 << 20, nslots=1, precision=prec)
+    srv.set_refill(256, idle_ms=5.0, life_ms=200.0)
+    print(f"refill {prec_name}: ring 256, {len(reqs)} requests", flush=True)
+    outs, tickets = [], []
+    for k, (name, x0, r, nH, te) in enumerate(reqs):
+        if k == len(reqs) // 2:
+            srv.progress()
+            time.sleep(0.05)                       # the first wave is running
+        ab = x0.copy()
+        outs.append(ab)
+        tickets.append(srv.submit(lib.network_id(name), ab, r, nH, te))
+        srv.progress()
+    done = _drain(srv, len(reqs))
+    ring, launches, held = srv.refill_stats()
+    srv.close()
+    print(f"  {launches} standing launches", flush=True)
+    assert ring == 256 and held == 0 and launches >= 2
+    assert sorted(t for t, _, _ in done) == sorted(tickets)
+    assert all(st == 0 for _, st, _ in done) and all(lat > 0 for _, _, lat in done)
+    for (name, x0, r, nH, te), got in zip(reqs, outs):
+        alone, st = _alone(name, x0, r, nH, te, prec)
+        assert not st.any()
+        assert np.array_equal(alone, got), (name, te, float(np.max(np.abs(alone - got))))
+
+
+@pytest.mark.gpu
+def test_refill_failure_statuses_equal_alone():
+    """With a step limit that stops some points (pdrchem_set_mxstep), a
+    standing launch reports each request's first failing status and the
+    partial abundances exactly as the same points solved alone."""
+    import oracle
+    from pdrchem import lib
+    reqs = _requests({"reduced": oracle.Oracle("reduced")}, [("reduced", 40), ("reduced", 25)], seed=9)
+    lib.set_globals(rtol=1e-8, atol=1e-30)
+    old = lib.mxstep()
+    lib.set_mxstep(60)
+    try:
+        srv = lib.Server(device_bytes=64 << 20, nslots=1, precision=lib.MIXED)
+        srv.set_refill(64)
+        outs, tickets = [], []
+        for name, x0, r, nH, te in reqs:
+            ab = x0.copy()
+            outs.append(ab)
+            tickets.append(srv.submit(lib.network_id(name), ab, r, nH, 1e7))
+        done = {t: st for t, st, _ in _drain(srv, len(reqs))}
+        srv.close()
+        for (name, x0, r, nH, te), got, tk in zip(reqs, outs, tickets):
+            alone, st = _alone(name, x0, r, nH, 1e7, lib.MIXED)
+            first = int(st[np.nonzero(st)[0][0]]) if st.any() else 0
+            assert first != 0 and done[tk] == first
+            assert np.array_equal(alone, got)
+    finally:
+        lib.set_mxstep(old)
+
+
+@pytest.mark.gpu
+def test_refill_rejects_bad_settings():
+    from pdrchem import lib
+    srv = lib.Server(device_bytes=1 << 20, nslots=1)
+    for ring in (100, 32, -1, 1 << 25):
+        with pytest.raises(lib.PdrChemError):
+            srv.set_refill(ring)
+    srv.set_refill(64)
+    with pytest.raises(lib.PdrChemError):
+        srv.submit(lib.network_id("reduced"), np.zeros((65, 33)), np.zeros((65, 329)), np.ones(65), 1e3)
+    assert srv.refill_stats() == (64, 0, 0)
+    srv.set_refill(0)
     srv.close()
