@@ -57,6 +57,10 @@ struct BdfU {
   double rl1, gamma, gammap, gamrat, crate, acnrm, saved_tq5, xe_last;
   int q, qprime, L, qwait, nscon, jcur;
   int nst, nfe, nsetups, nje, nni, ncfn, netf, nstlp, nstlj;
+  // inputs of the last coefficient evaluation (set_coeffs): order, step,
+  // step history, and whether tq[1], tq[3] were evaluated for them
+  double ch, ctau[QMAX + 1];
+  int cq, cqw;
 };
 
 // a[k] for a runtime k without a runtime index: the empty asm keeps the
@@ -185,11 +189,36 @@ __device__ __forceinline__ void restore(T &c, BdfU LDSA &u, S &b, double saved_t
   c.sync();
 }
 
-// BDF coefficients l[] and error-test constants tq[] (uniform scalar code)
+// BDF coefficients l[] and error-test constants tq[] (uniform scalar code).
+// They are a function of q, h and tau[1..q] alone (tq[1] and tq[3] only
+// evaluated when qwait is 1, else kept), so when those equal the inputs of
+// the last evaluation -- a run of steps at one step size and order, the
+// common case once h settles -- the values already in u are the ones the
+// evaluation would give, bit for bit, and only gammap and gamrat are due.
 template <class T>
 __device__ __forceinline__ void set_coeffs(T &c, BdfU LDSA &u) {
   const int q = uni(u.q);
   const double h = u.h;
+  const int qw1 = uni(u.qwait) == 1;
+  {
+    bool same = uni(u.cq) == q && u.ch == h && (!qw1 || uni(u.cqw) != 0);
+#pragma unroll
+    for (int j = 1; j <= QMAX; j++)
+      if (j <= q) same = same && u.tau[j] == u.ctau[j];
+    if (uni((int)same)) {
+      const double gamma = u.gamma;
+      const int nst = uni(u.nst);
+      const double gammap = (nst == 0) ? gamma : u.gammap;
+      const double gamrat = (nst > 0) ? gamma / gammap : 1.0;
+      c.sync();
+      if (c.lead()) {
+        u.gammap = gammap;
+        u.gamrat = gamrat;
+      }
+      c.sync();
+      return;
+    }
+  }
   double l[QMAX + 1], tau[QMAX + 2], tq[6];
 #pragma unroll
   for (int i = 0; i <= QMAX + 1; i++) tau[i] = u.tau[i];
@@ -227,7 +256,7 @@ __device__ __forceinline__ void set_coeffs(T &c, BdfU LDSA &u) {
   double A2 = 1.0 + q * A1;
   tq[2] = fabs(A1 / (alpha0 * A2));
   tq[5] = fabs(A2 * xistar_inv / (lq * xi_inv));
-  if (uni(u.qwait) == 1) {
+  if (qw1) {
     if (q > 1) {
       double C = xistar_inv / lq;
       double A3 = alpha0 + kRecip[q];
@@ -260,6 +289,11 @@ __device__ __forceinline__ void set_coeffs(T &c, BdfU LDSA &u) {
     u.gamma = gamma;
     u.gammap = gammap;
     u.gamrat = gamrat;
+    u.cq = q;
+    u.ch = h;
+#pragma unroll
+    for (int j = 1; j <= QMAX; j++) u.ctau[j] = tau[j];
+    u.cqw = qw1;
   }
   c.sync();
 }
@@ -734,6 +768,10 @@ __device__ __forceinline__ void bdf_init_uniform(BdfU LDSA &u, double t0, double
   for (int i = 0; i < 6; i++) u.tq[i] = 0.0;
   for (int i = 0; i <= QMAX; i++) u.l[i] = 0.0;
   u.gammap = u.gamma = u.gamrat = 0.0;
+  u.cq = 0;  // no coefficients evaluated yet
+  u.cqw = 0;
+  u.ch = 0.0;
+  for (int i = 0; i <= QMAX; i++) u.ctau[i] = 0.0;
   u.rl1 = 1.0;
   u.crate = 1.0;
   u.acnrm = 0.0;
