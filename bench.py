@@ -270,6 +270,12 @@ def main():
     ap.add_argument("--cloud-tol", type=float, default=1e-3, help="convergence tolerance of a point")
     ap.add_argument("--cloud-direct", action="store_true",
                     help="solve the cloud's batches by direct launches instead of through the request loop")
+    ap.add_argument("--cloud-thermal", action="store_true",
+                    help="thermal balance on (THERMALBALANCE): after the chemistry iterations, the outer "
+                         "thermal-balance loop until every point converges or the limits below")
+    ap.add_argument("--cloud-thermal-iters", type=int, default=200, help="most thermal-balance iterations")
+    ap.add_argument("--cloud-thermal-seconds", type=float, default=300.0,
+                    help="no new thermal-balance iteration after this many seconds of the loop")
     ap.add_argument("--cloud-fp64-side", type=int, default=24,
                     help="side of the cloud on which the plain FP64 form is compared (0 = skip)")
     a = ap.parse_args()
@@ -574,6 +580,11 @@ def cloud_bench(a):
         return c, rho, x0, t_setup
 
     c, rho, x0, t_setup = make(side)
+    if a.cloud_thermal:
+        # coolants, temperature brackets and dust temperatures from the field
+        # (THERMALBALANCE, GUESS_TEMP, DUST) before the first chemistry
+        # iteration, as the reference's main program sets them up
+        c.thermal_init(params=lib.default_thermal_params(pseudo_ray=-1))
     srv = None
     if not a.cloud_direct:
         # the shrinking, uneven batches of the iterations go through the request
@@ -584,6 +595,7 @@ def cloud_bench(a):
     G = c.get(CL.G)
     Av = c.get(CL.AV)
     cols0 = c.get(CL.COLUMNS)
+    Td = c.get(CL.DUST_TEMPERATURE) if a.cloud_thermal else np.full(side ** 3, 20.0)
     n_ep = int(lib.load().pdrchem_cloud_num_evaluation_points(c._h))
     iters, wall, solved = [], 0.0, 0
     tot = dict(nst=0, nfe=0, nje=0, nsetups=0, nni=0)
@@ -603,6 +615,41 @@ def cloud_bench(a):
               f"failed {st['failed']} in {dt:.2f} s", file=sys.stderr, flush=True)
         if it == 0:
             ab1 = c.get(CL.ABUNDANCE)
+    thermal = None
+    if a.cloud_thermal:
+        # the outer thermal-balance loop (3DPDR.F90:590-1241): level
+        # populations, cooling, heating and the temperature update every
+        # iteration; chemistry of the thermally unconverged points when the
+        # populations have converged; converged points drop out
+        th_iters, th_wall, th_solves = [], 0.0, 0
+        t_loop = time.perf_counter()
+        for it in range(a.cloud_thermal_iters):
+            if time.perf_counter() - t_loop > a.cloud_thermal_seconds:
+                break
+            t0 = time.perf_counter()
+            st = c.thermal_iterate(t_end=1e7, precision=prec)
+            dt = time.perf_counter() - t0
+            th_wall += dt
+            th_solves += st["chemistry_solves"]
+            th_iters.append({"iteration": st["iteration"], "lvg": st["lvg"], "levels_converged": st["level_converged"],
+                             "thermally_converged": st["thermally_converged"], "updated": st["thermal_points"],
+                             "chemistry_solves": st["chemistry_solves"], "failed": st["failed"],
+                             "wall_ms": dt * 1e3})
+            if st["chemistry_solves"] or it % 10 == 0 or st["done"]:
+                print(f"thermal iteration {st['iteration']}: levels {st['level_converged']}, thermally converged "
+                      f"{st['thermally_converged']} of {side ** 3}, chemistry solves {st['chemistry_solves']} "
+                      f"in {dt:.2f} s", file=sys.stderr, flush=True)
+            if st["done"]:
+                break
+        Tf = c.get(CL.TEMPERATURE)
+        thermal = {"iterations": len(th_iters), "done": bool(th_iters and th_iters[-1]["thermally_converged"] ==
+                                                          side ** 3) or bool(st["done"]),
+                   "thermally_converged": th_iters[-1]["thermally_converged"] if th_iters else 0,
+                   "chemistry_solves": th_solves, "wall_s": th_wall,
+                   "chemistry_solves_per_s": th_solves / th_wall if th_wall else 0.0,
+                   "iterations_per_s": len(th_iters) / th_wall if th_wall else 0.0,
+                   "T_gas_percentiles_K": [float(np.percentile(Tf, q)) for q in (1, 50, 99)],
+                   "chemistry_rounds": [i for i in th_iters if i["chemistry_solves"]]}
     f_rhs, f_jac = network_flops(net)
     stt = np.array([[tot["nst"], tot["nfe"], tot["nje"], tot["nsetups"], tot["nni"], 0, 0, 0]], dtype=np.float64)
     w64, wlu = algorithmic_flops(stt, net.neq, f_rhs, f_jac)
@@ -628,6 +675,7 @@ def cloud_bench(a):
                    "rtol": 1e-8, "atol": 1e-30,
                    "solves": "direct launches" if srv is None else "request loop (device requests, 4 slots)"},
         "iterations": iters,
+        "thermal_balance": thermal,
         "raytrace_setup_s": t_setup, "evaluation_points": n_ep,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": measured_traffic(kname, solved // max(len(iters), 1)),
@@ -641,7 +689,7 @@ def cloud_bench(a):
     if k:
         sel = np.linspace(0, side ** 3 - 1, k).astype(int)
         o = oracle.Oracle(name)
-        rate = o.rates(T[sel], 20.0, G[sel], Av[sel], cols0[sel])
+        rate = o.rates(T[sel], Td[sel], G[sel], Av[sel], cols0[sel])
         _, ref, _ = o.calculate_abundances(x0[sel].copy(), rate, rho[sel], T[sel], mode=oracle_mode(name),
                                            nthreads=thr)
         got = ab1[sel]
@@ -655,7 +703,7 @@ def cloud_bench(a):
         done, t0, chunk = 0, time.perf_counter(), thr * 2
         while time.perf_counter() - t0 < a.cpu_seconds and done < side ** 3:
             sel = order[done:done + chunk]
-            rate = o.rates(T[sel], 20.0, G[sel], Av[sel], cols0[sel])
+            rate = o.rates(T[sel], Td[sel], G[sel], Av[sel], cols0[sel])
             o.calculate_abundances(x0[sel].copy(), rate, rho[sel], T[sel], mode=oracle_mode(name), nthreads=thr)
             done += len(sel)
         el = time.perf_counter() - t0
