@@ -181,9 +181,8 @@ void pdrchem_clear_tuning(void) {
 }
 
 int pdrchem_set_tuned_config(int net, int precision, int npts, int systems_per_block, int blocks_per_cu) {
-  if (net < 0 || net >= NET_COUNT || npts <= 0 || systems_per_block < 0 ||
-      (systems_per_block > 4 && !(systems_per_block == 8 && net == NET_REDUCED)) || systems_per_block == 3 ||
-      blocks_per_cu < 0)
+  if (net < 0 || net >= NET_COUNT || npts <= 0 || systems_per_block < 0 || systems_per_block > 4 ||
+      systems_per_block == 3 || blocks_per_cu < 0)
     return -1;
   std::lock_guard<std::mutex> lock(g_mu);
   Entry &e = g_tab[net][precision == PDRCHEM_MIXED ? 1 : 0][size_bucket(npts)];

@@ -85,11 +85,6 @@ int launch_wave_solver(int net, const SolveRequest &r, hipStream_t s) {
       fill_args(a, r);
       LaunchConfig cfg = r.cfg;
       if (cfg.systems_per_block == 0 && cfg.blocks_per_cu == 0) cfg = tuned_config(net, r.mixed, r.npts);
-      if (cfg.systems_per_block == 8 && !r.published) {
-        SolveRequest q = r;
-        q.cfg = cfg;
-        return launch_pair_solver(net, q, 4, s);
-      }
       if (r.mixed) return launch_shape<float>(a, r.npts, s, r.max_blocks, cfg, r.workspace, r.workspace_bytes);
       return launch_shape<double>(a, r.npts, s, r.max_blocks, cfg, r.workspace, r.workspace_bytes);
     }
@@ -135,7 +130,7 @@ static size_t full_workspace() {
 
 size_t wave_solver_workspace_bytes(int net, int mixed) {
   if (net != NET_REDUCED) return 0;
-  return std::max(mixed ? full_workspace<float>() : full_workspace<double>(), pair_solver_workspace_bytes(net, mixed));
+  return mixed ? full_workspace<float>() : full_workspace<double>();
 }
 
 template <typename Kern>

@@ -13,8 +13,7 @@ enum NetId { NET_REDUCED = 0, NET_FULL = 1, NET_MYNETWORK = 2, NET_COUNT = 3 };
 
 // launch shape of the persistent solver (0 = the default / the device's limit)
 struct LaunchConfig {
-  int systems_per_block;  // wave kernel: systems per block, 1, 2 or 4 (one per wavefront), or 8
-                          // (two per wavefront, four wavefronts)
+  int systems_per_block;  // wave kernel: wavefronts (systems) per block, 1, 2 or 4
   int blocks_per_cu;      // cap on resident blocks per CU
 };
 
@@ -81,10 +80,6 @@ void *stream_workspace(hipStream_t s, size_t bytes);
 // reduced network: one system per wavefront (bdf_wave.hip)
 int launch_wave_solver(int net, const SolveRequest &r, hipStream_t s);
 int wave_solver_info(int net, int mixed, KernelInfo *info);
-// reduced network, two systems per wavefront (bdf_pair.hip): launch shape
-// systems_per_block = 8 (four wavefronts); closed batches only
-int launch_pair_solver(int net, const SolveRequest &r, int nw, hipStream_t s);
-size_t pair_solver_workspace_bytes(int net, int mixed);
 // full network and MYNETWORK: one system per workgroup (bdf_wg.hip)
 int launch_wg_solver(int net, const SolveRequest &r, hipStream_t s);
 int wg_solver_info(int net, int mixed, KernelInfo *info);

@@ -269,7 +269,6 @@ class WaveTables:
         L.append("__device__ const int8_t charge[N] = {")
         L.append(_fmt_list([str(x) for x in self.charges]))
         L.append("};")
-        L.extend(PairTables(net, self).emit_lines())
         emit_rate_params(L, net)
         L.append(f"}}  // namespace {ns}")
         with open(path, "w") as fh:
@@ -278,156 +277,6 @@ class WaveTables:
     def summary(self) -> str:
         return (f"{self.net.name}: slots={len(self.slot_reac)}/{self.nslot} TM={self.tm} PM={self.pm} "
                 f"lanes={self.lanes_used} JT={self.jt} JT_TOTAL={sum(self.jt)}")
-
-
-class PairTables:
-    """Lane tables of the half-wavefront layout: two systems per wave64, one
-    per 32-lane half, lane = row i of its system (N = 32 rows).
-
-    * Fluxes: slot s = round*32 + lane over the same active-reaction order
-      as the wavefront layout (P_ROUNDS rounds, zero slot P_NSLOT).
-    * RHS: every lane runs P_NSEG segments of fixed lengths P_SEG[k] (the
-      same for all lanes, so segment ends are compile-time); each (lane,
-      segment) cell sums one chunk of one row's terms into its partial
-      P[k*32 + lane], and row i adds its partials in chunk order (row_cells,
-      padded with the zero partial P[P_NSEG*32]).  The chunks are the
-      wavefront layout's (the same terms, order and row sums), so both
-      layouts give the same RHS bit for bit; the segment lengths are the
-      smallest total into which the chunks pack, largest first.
-    * Jacobian: lane i accumulates every column of row i from per-column
-      term lists in two passes (columns 0-15 and the x_e pseudo-column, then
-      columns 16-31), each with at most 17 accumulators; p_jac_col holds the
-      accumulator of each round."""
-    LANES = 32
-
-    def __init__(self, net: Network, wt: "WaveTables"):
-        self.net = net
-        neq = net.neq
-        if neq > 32:
-            raise ValueError(f"{net.name}: half-wavefront layout supports up to 32 equations, got {neq}")
-        nl = self.LANES
-        self.slot_reac = list(wt.slot_reac)
-        nact = len(self.slot_reac)
-        self.rounds = max(1, math.ceil(nact / nl))
-        self.nslot = self.rounds * nl
-        self.slot_fab = list(wt.slot_fab[:nact]) + [(neq + 1) | ((neq + 1) << 8)] * (self.nslot - nact)
-        rows = wt.row_terms
-        # the wavefront layout's chunks: row i in row_nlanes[i] contiguous
-        # pieces of ceil(n_i / row_nlanes[i]) terms
-        chunks = []                               # (row, u, terms)
-        for i in range(neq):
-            k = wt.row_nlanes[i]
-            csz = math.ceil(len(rows[i]) / k) if rows[i] else 0
-            for u in range(k):
-                chunks.append((i, u, rows[i][u * csz:(u + 1) * csz]))
-        self.segs, place = self._pack([len(c[2]) for c in chunks])
-        self.nseg = len(self.segs)
-        self.zero_cell = self.nseg * nl
-        off = [sum(self.segs[:k]) for k in range(self.nseg)]
-        self.tm = sum(self.segs)
-        self.rhs_term = [[(self.nslot, 0)] * nl for _ in range(self.tm)]
-        cells = [[] for _ in range(neq)]
-        for (i, u, terms), (lane, k) in zip(chunks, place):
-            for t, term in enumerate(terms):
-                self.rhs_term[off[k] + t][lane] = term
-            cells[i].append(k * nl + lane)        # chunks of a row come in order u
-        self.pm = max(1, max(len(c) for c in cells))
-        self.row_cells = [c + [self.zero_cell] * (self.pm - len(c)) for c in cells] + \
-            [[self.zero_cell] * self.pm for _ in range(nl - neq)]
-        # Jacobian lists per (row, column); column neq is d ydot / d x_e
-        self.jzero = 2 * self.nslot
-        reac_slot = {r: s for s, r in enumerate(self.slot_reac)}
-        jl = {(i, c): [] for i in range(nl) for c in range(neq + 1)}
-        for r in net.reactions:
-            if not r.stoich:
-                continue
-            s_ = reac_slot[r.index]
-            for i, coef in r.stoich.items():
-                for which, f in ((0, r.fa), (1, r.fb)):
-                    if f <= neq:
-                        jl[(i, f)].append((2 * s_ + which, coef))
-        cols_a = [c for c in range(min(16, neq))] + [neq]
-        cols_b = [c for c in range(16, neq)]
-        self.jac_term, self.jac_acc = [], []
-        for cols in (cols_a, cols_b):
-            for c in cols:
-                depth = max(len(jl[(i, c)]) for i in range(nl))
-                acc = 16 if c == neq else c % 16
-                for t in range(depth):
-                    self.jac_term.append([jl[(i, c)][t] if t < len(jl[(i, c)]) else (self.jzero, 0)
-                                          for i in range(nl)])
-                    self.jac_acc.append(acc)
-            if cols is cols_a:
-                self.ja = len(self.jac_term)
-        self.charges = net.charges() + [0] * (nl - neq)
-
-    @staticmethod
-    def _pack_with(sizes, segs, nl=32):
-        """(lane, segment) of each chunk, largest chunks first, each into the
-        smallest free cell that holds it; None if they do not fit"""
-        free = [(L, k, lane) for k, L in enumerate(segs) for lane in range(nl)]
-        place = [None] * len(sizes)
-        for c in sorted(range(len(sizes)), key=lambda c: (-sizes[c], c)):
-            fit = [x for x in free if x[0] >= sizes[c]]
-            if not fit:
-                return None
-            x = min(fit)
-            free.remove(x)
-            place[c] = (x[2], x[1])
-        return place
-
-    def _pack(self, sizes):
-        best = None
-        top = max(sizes) if sizes else 1
-        for a in range(top, top + 1):
-            for b in range(0, a + 1):
-                for c in range(0, b + 1):
-                    segs = [x for x in (a, b, c) if x > 0]
-                    if best is not None and sum(segs) >= sum(best[0]):
-                        continue
-                    r = self._pack_with(sizes, segs)
-                    if r is not None:
-                        best = (segs, r)
-        return best
-
-    def emit_lines(self) -> List[str]:
-        nl = self.LANES
-        L = ["// ---- half-wavefront layout (two systems per wave64, lane = row)",
-             f"constexpr int P_ROUNDS = {self.rounds};      // flux slot rounds of 32 lanes",
-             f"constexpr int P_NSLOT = {self.nslot};",
-             f"constexpr int P_NSEG = {self.nseg};",
-             "constexpr int P_SEG[P_NSEG] = {" + ", ".join(str(x) for x in self.segs) + "};",
-             f"constexpr int P_TM = {self.tm};         // RHS term rounds (sum of P_SEG)",
-             f"constexpr int P_PM = {self.pm};          // partials per row (max)",
-             f"constexpr int P_JA = {self.ja};         // Jacobian rounds of the first pass",
-             f"constexpr int P_JT = {len(self.jac_term)};",
-             "__device__ const int16_t p_slot_reac[P_NSLOT] = {",
-             _fmt_list([str(x) for x in self.slot_reac] + ["-1"] * (self.nslot - len(self.slot_reac))),
-             "};",
-             "__device__ const uint32_t p_slot_fab[P_NSLOT] = {",
-             _fmt_list([f"0x{x:x}u" for x in self.slot_fab], 8),
-             "};",
-             "// [t][lane]: 8 * slot | coefficient's FP64 top 16 bits << 16 (as rhs_term)",
-             "__device__ const uint32_t p_rhs_term[P_TM * 32] = {",
-             _fmt_list([f"0x{rhs_word(*self.rhs_term[t][ln]):x}u" for t in range(self.tm) for ln in range(nl)], 8),
-             "};",
-             "// [row][u]: partial indices of row i (P_NSEG*32: the zero partial)",
-             "__device__ const uint8_t p_row_cells[32 * P_PM] = {",
-             _fmt_list([str(x) for r in self.row_cells for x in r]),
-             "};",
-             "// accumulator of each Jacobian round (16: d/dx_e in the first pass)",
-             "constexpr int8_t p_jac_acc[P_JT] = {",
-             _fmt_list([str(x) for x in self.jac_acc]),
-             "};",
-             "// [round][lane]: D index | (int16 coefficient) << 16",
-             "__device__ const uint32_t p_jac_term[P_JT * 32] = {",
-             _fmt_list([f"0x{(d & 0xffff) | ((c & 0xffff) << 16):x}u" for row in self.jac_term for (d, c) in row], 8),
-             "};"]
-        return L
-
-    def summary(self) -> str:
-        return (f"{self.net.name} (pairs): rounds={self.rounds} segs={self.segs} TM={self.tm} PM={self.pm} "
-                f"JT={len(self.jac_term)} (first pass {self.ja})")
 
 
 # Shielding / extinction tables (constant data, identical by necessity to the
@@ -507,4 +356,3 @@ def generate(names=("reduced",), net_dir=None, out_dir=None) -> List[WaveTables]
 if __name__ == "__main__":
     for wt in generate():
         print(wt.summary())
-        print(PairTables(wt.net, wt).summary())

@@ -117,91 +117,6 @@ def test_jacobian_tables(case):
     assert np.all(J[n:, :] == 0.0) and np.all(J[:, n:] == 0.0)
 
 
-@pytest.fixture(scope="module")
-def pairs(case):
-    net, tab = case[0], case[1]
-    return gpugen.PairTables(net, tab)
-
-
-def test_pair_layout(case, pairs):
-    """Half-wavefront tables: the same active reactions in 32-lane rounds,
-    segment lengths covering every row's terms exactly once, partial and
-    Jacobian indices in range."""
-    net, tab = case[0], case[1]
-    pt = pairs
-    assert pt.nslot % 32 == 0 and pt.slot_reac == tab.slot_reac and pt.jzero == 2 * pt.nslot
-    assert sum(pt.segs) == pt.tm and pt.zero_cell == 32 * pt.nseg
-    # every (slot, coefficient) term of every row appears once in the lane lists
-    seen = {}
-    for t in range(pt.tm):
-        for lane in range(32):
-            s, c = pt.rhs_term[t][lane]
-            if s != pt.nslot:
-                seen[(s, c)] = seen.get((s, c), 0) + 1
-    want = {}
-    for row in tab.row_terms:
-        for term in row:
-            want[term] = want.get(term, 0) + 1
-    assert seen == want
-    assert max(d for row in pt.jac_term for (d, _) in row) <= 2 * pt.nslot
-    assert len(pt.jac_acc) == len(pt.jac_term) and max(pt.jac_acc) <= 16
-    assert pt.tm < 2 * tab.tm    # two systems' terms in fewer than twice the rounds of one
-
-
-def test_pair_rhs_and_jacobian_tables(case, pairs):
-    """The half-wavefront lists run lane by lane in numpy (fluxes, segment
-    partials, row sums of partials; the two Jacobian passes and the x_e
-    chain column) give the oracle's RHS and exact Jacobian."""
-    net, _, orc, rate, nH, x = case
-    pt = pairs
-    X = _factor_vector(net, x)
-    kp = _kp(pt, net, rate, nH)
-    fab = np.asarray(pt.slot_fab, dtype=np.int64)
-    F = np.zeros(pt.nslot + 1)
-    F[: pt.nslot] = kp * X[fab & 0xFF] * X[(fab >> 8) & 0xFF]
-    P = np.zeros(pt.zero_cell + 1)
-    t0 = 0
-    for k, L in enumerate(pt.segs):
-        for lane in range(32):
-            P[k * 32 + lane] = sum(c * F[s] for (s, c) in (pt.rhs_term[t][lane] for t in range(t0, t0 + L)))
-        t0 += L
-    yd = np.array([sum(P[pt.row_cells[i][u]] for u in range(pt.pm)) for i in range(net.neq)])
-    # the same chunks summed in the same order as the wavefront layout's
-    tab = case[1]
-    F64 = np.zeros(tab.nslot + 1)
-    F64[: len(pt.slot_reac)] = F[: len(pt.slot_reac)]
-    P64 = np.zeros(65)
-    for lane in range(64):
-        P64[lane] = sum(c * F64[s] for (s, c) in (tab.rhs_term[t][lane] for t in range(tab.tm)))
-    yd64 = np.array([sum(P64[tab.row_base[i] + u] if u < tab.row_nlanes[i] else 0.0 for u in range(pt.pm))
-                     for i in range(net.neq)])
-    assert np.array_equal(yd, yd64)
-    ref, _ = orc.rhs(rate, nH, x)
-    mag = np.zeros(net.neq)
-    for r in net.reactions:
-        fl = abs(rate[r.index] * (nH if r.nh else 1.0) * X[r.fa] * X[r.fb])
-        for i, c in r.stoich.items():
-            mag[i] += abs(c) * fl
-    assert np.all(np.abs(yd - ref) <= 1e-13 * np.maximum(mag, np.abs(ref)))
-    D = np.zeros(2 * pt.nslot + 1)
-    D[0:2 * pt.nslot:2] = kp * X[(fab >> 8) & 0xFF]
-    D[1:2 * pt.nslot:2] = kp * X[fab & 0xFF]
-    acc_a = np.zeros((32, 17))
-    acc_b = np.zeros((32, 16))
-    for g, (row, a) in enumerate(zip(pt.jac_term, pt.jac_acc)):
-        for lane in range(32):
-            d, c = row[lane]
-            (acc_a if g < pt.ja else acc_b)[lane, a] += c * D[d]
-    q = pt.charges
-    J = np.zeros((32, 32))
-    for i in range(32):
-        for j in range(16):
-            J[i, j] = acc_a[i, j] + acc_a[i, 16] * q[j]
-            J[i, 16 + j] = acc_b[i, j] + acc_a[i, 16] * q[16 + j]
-    n = net.neq
-    np.testing.assert_allclose(J[:n, :n], orc.jac(rate, nH, x), rtol=1e-12, atol=1e-300)
-
-
 def _pivot_key(a, i):
     """the kernel's pivot_key<5>: the leading word of |a| (FP32: its bits,
     FP64: the high word) with 31 - row in the low five bits"""
