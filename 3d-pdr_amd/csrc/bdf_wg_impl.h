@@ -93,27 +93,37 @@ struct BdfWG {
   __device__ __forceinline__ void hist_out() {}
 };
 
-// Per-workgroup slot of the device workspace: the saved Jacobian and the
-// lanes' slot rate constants (lane-private: written and read by the same
-// lane, [r * BLOCK + lane]).
+// Per-workgroup slot of the device workspace: the saved Jacobian, by LU
+// position (the M build reads it in position order), and the lanes' slot rate
+// constants (lane-private: written and read by the same lane, [r * BLOCK +
+// lane]).
 template <typename MT>
 struct Slot {
-  static constexpr size_t jac_bytes = ((size_t)NET::NJ * sizeof(MT) + 255) / 256 * 256;
+  static constexpr size_t jac_bytes = ((size_t)NET::MTMP * BLOCK * sizeof(MT) + 255) / 256 * 256;
   static constexpr size_t kp_off = jac_bytes;
   static constexpr size_t bytes = kp_off + (size_t)NET::FR * BLOCK * sizeof(double);
 };
+
+// Pointers into device memory in the global address space: loads through
+// them are global loads, which the hardware counts apart from LDS accesses,
+// so a wait for LDS operands does not also wait for table words in flight
+// (a generic pointer gives flat loads, counted with both).
+template <typename T>
+using gptr = const __attribute__((address_space(1))) T *;
+template <typename T>
+using gmut = __attribute__((address_space(1))) T *;
 
 template <typename MT>
 struct Team {
   int tid, lane, wave;
   bool own_;                 // tid < NEQ
   double q;                  // charge of row tid
-  double *Kp;                // rate * (n_H) of this lane's flux slots: Kp[r * BLOCK]
+  gmut<double> Kp;           // rate * (n_H) of this lane's flux slots: Kp[r * BLOCK]
   int rbase, rnl;            // RHS partial run of row tid
   int red_buf;
   WgLDS<MT> LDSA *S;
-  MT *Jg;                    // device-memory Jacobian slot [NJ]
-  const uint32_t *fab;       // slot factor table, this lane's column: fab[r * BLOCK]
+  gmut<MT> Jg;               // device-memory Jacobian slot, by LU position [MTMP * BLOCK]
+  gptr<uint32_t> fab;        // slot factor table, this lane's column: fab[r * BLOCK]
   double rtol, atol;
 
   __device__ __forceinline__ bool lead() const { return tid == 0; }
@@ -144,40 +154,22 @@ struct Team {
 // Table base address made opaque at the point of use.  Without it the
 // compiler hoists the per-round addresses of the lane tables (one 64-bit
 // register pair per unrolled round) out of the integrator's loops, where they
-// stay live across the whole solve and push the kernel into spills.
+// stay live across the whole solve and push the kernel into spills.  The
+// result keeps the global address space.
 template <typename T>
-__device__ __forceinline__ const T *opaque(const T *p) {
+__device__ __forceinline__ gptr<T> opaque(const T *p) {
   asm volatile("" : "+s"(p));
-  return p;
+  return (gptr<T>)p;
 }
 
-// Runs f over a lane's n table items (item t at tab[(off + t)*BLOCK + tid])
-// in order, loading U items at a time ahead of their use: the loads of a
-// group are independent and overlap, so a lane waits for the table once per
-// U items instead of once per item.  (A plain loop whose body ends in a
-// conditional store is not unrolled by the compiler and pays the full
-// load latency every item.)
-template <int U, typename W, typename F>
-__device__ __forceinline__ void for_items(const W *tab, int off, int n, int tid, F &&f) {
-  int t = 0;
-  for (; t + U <= n; t += U) {
-    W w[U];
-#pragma unroll
-    for (int u = 0; u < U; u++) w[u] = tab[(size_t)(off + t + u) * BLOCK + tid];
-#pragma unroll
-    for (int u = 0; u < U; u++) f(w[u]);
-  }
-  for (; t < n; t++) f(tab[(size_t)(off + t) * BLOCK + tid]);
-}
-
-// As for_items, and the LDS operands of a group's items are fetched by
-// gather(w) for the whole group before consume(w, g) runs on them in order,
-// so that a lane waits for LDS once per group as well.  Valid only where no
-// item's consume writes what another item of the same call gathers: the
-// elimination, inverse and solve tables are level-scheduled so that within
-// one call no target is also a source (which concurrent lanes need anyway).
+// Runs a lane's n table items (item t at tab[(off + t)*BLOCK + tid]) in
+// groups of U: the group's words are loaded together, the LDS operands of
+// the whole group are fetched by gather(w), then consume(w, g) runs on the
+// group in order, so that a lane waits for the table and for LDS once per
+// group.  Valid only where no item's consume writes what another item of the
+// same call gathers (the Jacobian's terms write device memory only).
 template <int U, typename W, typename G, typename C>
-__device__ __forceinline__ void for_items_gc(const W *tab, int off, int n, int tid, G &&gather, C &&consume) {
+__device__ __forceinline__ void for_items_gc(gptr<W> tab, int off, int n, int tid, G &&gather, C &&consume) {
   using V = decltype(gather(W()));
   int t = 0;
   for (; t + U <= n; t += U) {
@@ -206,39 +198,101 @@ __device__ __forceinline__ void lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-// The first U words of a lane's table run, loaded ahead of the phase that
-// consumes them: issued before the preceding barrier, so that the table's
-// L2 latency overlaps the barrier instead of opening the phase.
+// Team sum across a barrier that orders LDS only (for phases whose table
+// loads stay in flight across it)
+template <typename TeamT>
+__device__ __forceinline__ double sum_lds(TeamT &c, double v) {
+  v = wave_sum(v);
+  if (c.lane == 0) c.S->red[c.red_buf][c.wave] = v;
+  lds_sync();
+  const double r = (c.S->red[c.red_buf][0] + c.S->red[c.red_buf][1]) + (c.S->red[c.red_buf][2] + c.S->red[c.red_buf][3]);
+  c.red_buf ^= 1;
+  return r;
+}
+
+// The next U words of a lane's table runs, in flight ahead of their use
+// (words of narrower tables are widened to W).
 template <int U, typename W>
 struct Ahead {
   W w[U];
-  __device__ __forceinline__ void load(const W *tab, int off, int n, int tid) {
+  template <typename T>
+  __device__ __forceinline__ void load(gptr<T> tab, int off, int n, int tid) {
 #pragma unroll
     for (int u = 0; u < U; u++)
-      if (u < n) w[u] = tab[(size_t)(off + u) * BLOCK + tid];
+      if (u < n) w[u] = (W)tab[(size_t)(off + u) * BLOCK + tid];
   }
 };
 
-// for_items_gc over n items whose first min(n, U) words are already in `a`
-template <int U, typename W, typename G, typename C>
-__device__ __forceinline__ void for_items_gc_ahead(const Ahead<U, W> &a, const W *tab, int off, int n, int tid,
-                                                   G &&gather, C &&consume) {
+// A lane's run of N table items (item t at tab[(off + t)*BLOCK + tid]) in
+// groups of U, software-pipelined: `a` holds the group's words on entry, the
+// LDS operands of the whole group are fetched by gather(w), then the next
+// group's words are loaded, then consume(w, g) runs on the group in order.
+// The group after the run's last is the first of the run that follows
+// (ntab, noff, NN items), so on return `a` holds its words and their latency
+// overlaps this run's last group and the barrier between the two runs.  The
+// lengths are compile-time constants: the group loop unrolls and every
+// length test folds away, so all loads are unconditional and the compiler
+// waits for exactly the group it needs while the next one stays in flight
+// (a load under a run-time condition makes it wait for every load).  Valid
+// only where no item's consume writes what another item of the same run
+// gathers: the RHS terms write registers, the solve's chunks partial sums.
+template <int U, int N, int NN, typename W, typename T, typename TN, typename G, typename C>
+__device__ __forceinline__ void run_const(Ahead<U, W> &a, gptr<T> tab, int off, gptr<TN> ntab, int noff, int tid,
+                                          G &&gather, C &&consume) {
   using V = decltype(gather(W()));
-  {
-    V g[U];
+  if constexpr (N <= 0) {
+    a.load(ntab, noff, NN, tid);
+  } else {
 #pragma unroll
-    for (int u = 0; u < U; u++)
-      if (u < n) g[u] = gather(a.w[u]);
+    for (int t = 0; t < N; t += U) {
+      V g[U];
+      W cur[U];
 #pragma unroll
-    for (int u = 0; u < U; u++)
-      if (u < n) consume(a.w[u], g[u]);
+      for (int u = 0; u < U; u++) {
+        cur[u] = a.w[u];
+        if (t + u < N) g[u] = gather(cur[u]);
+      }
+      if (t + U < N) a.load(tab, off + t + U, N - t - U, tid);
+      else a.load(ntab, noff, NN, tid);
+#pragma unroll
+      for (int u = 0; u < U; u++)
+        if (t + u < N) consume(cur[u], g[u]);
+    }
   }
-  if (n > U) for_items_gc<U>(tab, off + U, n - U, tid, gather, consume);
+}
+
+// One group of U rounds of a [round][lane] stream at round `pos`: the
+// group's words are in `a`; the LDS operands of the whole group are fetched
+// by gather(w), then the next group's words (round pos + U) are loaded
+// unconditionally, then consume(w, g) runs on the group in order.
+template <int U, typename G, typename C>
+__device__ __forceinline__ void stream_group(Ahead<U, uint64_t> &a, gptr<uint64_t> st, int pos, int tid, G &&gather,
+                                             C &&consume) {
+  using V = decltype(gather(uint64_t()));
+  uint64_t cur[U];
+  V g[U];
+#pragma unroll
+  for (int u = 0; u < U; u++) {
+    cur[u] = a.w[u];
+    g[u] = gather(cur[u]);
+  }
+#pragma unroll
+  for (int u = 0; u < U; u++) a.w[u] = st[(size_t)(pos + U + u) * BLOCK + tid];
+#pragma unroll
+  for (int u = 0; u < U; u++) consume(cur[u], g[u]);
 }
 
 template <typename MT>
 struct Pair {
   MT a, b;
+};
+template <typename MT>
+struct Triple {
+  MT a, b, t;
+};
+template <typename MT>
+struct Quad {
+  MT a, b, p, t;
 };
 
 // the region as LU storage (positions: [0, NHV) head entries, NHV the zero
@@ -263,33 +317,45 @@ template <typename MT>
 __device__ __forceinline__ double wg_rhs(Team<MT> &c, double y, double &xe_out) {
   WgLDS<MT> LDSA &S = *c.S;
   double LDSA *F = regionF(c.S);
-  if (c.own_) S.X[c.tid] = y;
-  const double xe = c.sum(c.own_ ? c.q * y : 0.0);
-  xe_out = xe;
-  if (c.tid == 0) {
-    S.X[NET::NEQ] = xe;
-    F[NET::NSLOT] = 0.0;
-  }
-  __syncthreads();
+  // the lane's flux factors and rate constants and its first term words do
+  // not depend on the state: their loads are issued first and arrive under
+  // the reduction; every barrier here orders LDS only, so none waits for them
+  uint32_t fw[NET::FR];
+  double kp[NET::FR];
 #pragma unroll
   for (int r = 0; r < NET::FR; r++) {
-    const uint32_t f = c.fab[r * BLOCK];
-    F[r * BLOCK + c.tid] = c.Kp[r * BLOCK] * S.X[f & 0xff] * S.X[(f >> 8) & 0xff];
+    fw[r] = c.fab[r * BLOCK];
+    kp[r] = c.Kp[r * BLOCK];
   }
-  __syncthreads();
+  const gptr<uint32_t> rt = opaque(NET::rhs_term);
+  Ahead<6, uint32_t> pre;
+  pre.load(rt, 0, NET::TM, c.tid);
+  if (c.own_) S.X[c.tid] = y;
+  const double xe = sum_lds(c, c.own_ ? c.q * y : 0.0);
+  xe_out = xe;
+  // x_e (factor index NEQ) is taken from the sum rather than from X
+  if (c.tid == 0) F[NET::NSLOT] = 0.0;
+#pragma unroll
+  for (int r = 0; r < NET::FR; r++) {
+    const int ia = fw[r] & 0xff, ib = (fw[r] >> 8) & 0xff;
+    const double xa = (ia == NET::NEQ) ? xe : S.X[ia];
+    const double xb = (ib == NET::NEQ) ? xe : S.X[ib];
+    F[r * BLOCK + c.tid] = kp[r] * xa * xb;
+  }
+  lds_sync();
   // term word: byte offset of the flux | the coefficient's FP64 high 16
   // bits (its other bits are zero), so decoding is two masks
-  const uint32_t *rt = opaque(NET::rhs_term);
   const unsigned char LDSA *Fb = (const unsigned char LDSA *)F;
 #pragma unroll
   for (int p = 0; p < NET::VL; p++) {
     double acc = 0.0;
-    for_items_gc<6>(rt, p * NET::TM, NET::TM, c.tid,
-                    [&](uint32_t w) { return *(const double LDSA *)(Fb + (w & 0xffff)); },
-                    [&](uint32_t w, double f) { acc = fma(__hiloint2double((int)(w & 0xffff0000u), 0), f, acc); });
+    // (the last chunk's "next run" is its own first words again: loaded, unused)
+    run_const<6, NET::TM, NET::TM>(pre, rt, p * NET::TM, rt, (p + 1 < NET::VL) ? (p + 1) * NET::TM : 0, c.tid,
+              [&](uint32_t w) { return *(const double LDSA *)(Fb + (w & 0xffff)); },
+              [&](uint32_t w, double f) { acc = fma(__hiloint2double((int)(w & 0xffff0000u), 0), f, acc); });
     S.P[p * BLOCK + c.tid] = acc;
   }
-  __syncthreads();
+  lds_sync();
   double yd = 0.0;
   for (int u = 0; u < c.rnl; u++) yd += S.P[c.rbase + u];
   return yd;
@@ -345,8 +411,8 @@ __device__ __forceinline__ void schur_mfma(const Team<float> &c, float LDSA *LU)
     const int i = 32 * ti + 8 * (r >> 2) + 4 * lk + (r & 3);
     acc[r] = Tl[i * NET::TS + 32 * tj + li];
   }
-  const uint16_t *lrow = opaque(NET::lth_pos) + (32 * ti + li) * NET::KH;
-  const uint16_t *ucol = opaque(NET::uht_pos) + 32 * tj + li;
+  const gptr<uint16_t> lrow = opaque(NET::lth_pos) + (32 * ti + li) * NET::KH;
+  const gptr<uint16_t> ucol = opaque(NET::uht_pos) + 32 * tj + li;
 #pragma unroll 4
   for (int k0 = 0; k0 < NET::KH; k0 += 2) {
     const int k = k0 + lk;
@@ -373,8 +439,8 @@ __device__ __forceinline__ void schur_mfma(const Team<double> &c, double LDSA *L
     f64x4 acc;
 #pragma unroll
     for (int r = 0; r < 4; r++) acc[r] = Tl[(i0 + 4 * r + lk) * NET::TS + j0 + li];
-    const uint16_t *lrow = opaque(NET::lth_pos) + (i0 + li) * NET::KH;
-    const uint16_t *ucol = opaque(NET::uht_pos) + j0 + li;
+    const gptr<uint16_t> lrow = opaque(NET::lth_pos) + (i0 + li) * NET::KH;
+    const gptr<uint16_t> ucol = opaque(NET::uht_pos) + j0 + li;
 #pragma unroll 2
     for (int k0 = 0; k0 < NET::KH; k0 += 4) {
       const int k = k0 + lk;
@@ -484,14 +550,16 @@ __device__ __forceinline__ int wg_setup(Team<MT> &c, BdfU LDSA &u, BdfWG<MT> &b,
   __syncthreads();
   if (c.tid == 0) S.badpiv = 0;
   // M = I - gamma J, scaled, in LU storage order.  Position entry d: row,
-  // column, source of J in the saved Jacobian; the bordered row holds the
-  // charges.  The table entries and the J values or charges of a group of
-  // positions are fetched before they are used.
-  const uint32_t *mp = opaque(NET::mpos);
-  auto m_value = [&](uint32_t d) -> double {
+  // column, whether J has an entry there (the saved Jacobian holds it at the
+  // same position); the bordered row holds the charges.  The table entries
+  // and J values of the next group of positions are loaded while the current
+  // group is scaled and stored.
+  const gptr<uint32_t> mp = opaque(NET::mpos);
+  const gptr<MT> jg = (gptr<MT>)c.Jg;
+  auto m_value = [&](uint32_t d, MT j) -> double {
     const int row = d & 0xff, col = (d >> 8) & 0xff, src = d >> 16;
     if (row == NET::NEQ) return (col == NET::NEQ) ? 0.0 : (double)NET::charge[col];
-    return (row < 0xfd && src != 0xffff) ? (double)c.Jg[src] : 0.0;
+    return (row < 0xfd && src != 0xffff) ? (double)j : 0.0;
   };
   auto m_entry = [&](uint32_t d, double jv) -> double {
     const int row = d & 0xff, col = (d >> 8) & 0xff;
@@ -506,80 +574,123 @@ __device__ __forceinline__ int wg_setup(Team<MT> &c, BdfU LDSA &u, BdfWG<MT> &b,
     }
     return m * S.W[row] * S.Winv[col];
   };
-  constexpr int U = 8;
-  int t0 = 0;
-  for (; t0 + U <= NET::MTM; t0 += U) {
+  {
+    // the position table and the Jacobian slot are padded to whole groups, so
+    // every load is unconditional (the last group's prefetch reads one group
+    // past them: the table's and the slot's first group, unused)
+    constexpr int U = NET::MGROUP;
+    static_assert(NET::MTMP % U == 0, "M build in whole groups");
     uint32_t d[U];
-    double jv[U];
+    MT j[U];
 #pragma unroll
-    for (int u = 0; u < U; u++) d[u] = mp[(t0 + u) * BLOCK + c.tid];
+    for (int u = 0; u < U; u++) {
+      d[u] = mp[u * BLOCK + c.tid];
+      j[u] = jg[u * BLOCK + c.tid];
+    }
+#pragma unroll 1
+    for (int t0 = 0; t0 < NET::MTMP; t0 += U) {
+      uint32_t dc[U];
+      MT jc[U];
 #pragma unroll
-    for (int u = 0; u < U; u++) jv[u] = m_value(d[u]);
+      for (int u = 0; u < U; u++) {
+        dc[u] = d[u];
+        jc[u] = j[u];
+      }
+      const int tn = (t0 + U < NET::MTMP) ? t0 + U : 0;
 #pragma unroll
-    for (int u = 0; u < U; u++)
-      if ((d[u] & 0xff) != 0xff) LU[(t0 + u) * BLOCK + c.tid] = (MT)m_entry(d[u], jv[u]);
+      for (int u = 0; u < U; u++) {
+        d[u] = mp[(tn + u) * BLOCK + c.tid];
+        j[u] = jg[(tn + u) * BLOCK + c.tid];
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++)
+        if ((dc[u] & 0xff) != 0xff) LU[(t0 + u) * BLOCK + c.tid] = (MT)m_entry(dc[u], m_value(dc[u], jc[u]));
+    }
   }
-  for (; t0 < NET::MTM; t0++) {
-    const uint32_t d = mp[t0 * BLOCK + c.tid];
-    if ((d & 0xff) != 0xff) LU[t0 * BLOCK + c.tid] = (MT)m_entry(d, m_value(d));
-  }
-  __syncthreads();
-  // head elimination, level by level.  The head has no pivoting: a zero or
-  // non-finite pivot makes the setup fail (CVODE's lsetup failure, as the
-  // tail's singular Gauss-Jordan step does) instead of passing inf/NaN on.
-  for (int lv = 0; lv < NET::NLEV; lv++) {
-    for_items_gc<4>(opaque(NET::elimA), NET::elimA_off[lv], NET::elimA_n[lv], c.tid,
-                    [&](uint32_t w) { return Pair<MT>{LU[w & 0x3fff], LU[(w >> 14) & 0x3fff]}; },
-                    [&](uint32_t w, Pair<MT> g) {
-                      const int kind = w >> 28, tgt = w & 0x3fff;
-                      // FP32 form: the hardware reciprocal (1 ulp) in place of
-                      // an exactly rounded division; the factors only
-                      // precondition the FP64 Newton iteration.  A pivot
-                      // whose reciprocal is not finite (zero, or denormal
-                      // and flushed) is rejected.
-                      if constexpr (sizeof(MT) == 4) {
-                        const MT r = __builtin_amdgcn_rcpf(g.b);
-                        if (kind == 1) LU[tgt] = g.a * r;       // multiplier
-                        else if (kind == 2) LU[tgt] = r;        // diagonal of the head's U^-1
-                        if ((kind == 1 || kind == 2) && !(isfinite(r) && isfinite(g.b))) S.badpiv = 1;
-                      } else {
-                        if (kind == 1) LU[tgt] = g.a / g.b;
-                        else if (kind == 2) LU[tgt] = (MT)1.0 / g.b;
-                        if ((kind == 1 || kind == 2) && !(g.b != (MT)0 && isfinite(g.b))) S.badpiv = 1;
-                      }
-                    });
-    __syncthreads();
-    MT acc = (MT)0.0;
-    for_items_gc<8>(opaque(NET::elimB), NET::elimB_off[lv], NET::elimB_n[lv], c.tid,
-                    [&](uint64_t w) { return Pair<MT>{LU[(w >> 14) & 0x3fff], LU[(w >> 28) & 0x3fff]}; },
-                    [&](uint64_t w, Pair<MT> g) {
-                      acc = fma(g.a, g.b, acc);
-                      if ((w >> 42) & 1) {
-                        LU[w & 0x3fff] -= acc;
-                        acc = (MT)0.0;
-                      }
-                    });
-    __syncthreads();
-  }
-  if (S.badpiv) return 1;  // uniform: read after the level loop's last barrier
-  if constexpr (NET::NHEAD > 0) {
-    // explicit inverses of the head triangles, level by level:
-    // target <- -(sum of src1*src2) * LU[scale]
-    for (int lv = 0; lv < NET::NINV; lv++) {
+  // Head elimination and head inverses: the setup stream, phase by phase
+  // (one barrier each), in whole groups of SU table rounds.  The first group
+  // is loaded before the barrier; each group's loads are issued before the
+  // previous group is consumed, and the group after a phase's last is the
+  // next phase's first, so table latency overlaps the work and the barriers
+  // (which order LDS only from here on).
+  const gptr<uint64_t> st = opaque(NET::setup_stream);
+  Ahead<NET::SU, uint64_t> pre;
+  if constexpr (NET::NSP > 0) pre.load(st, 0, NET::SU, c.tid);
+  lds_sync();
+  if constexpr (NET::NSP > 0) {
+    int pos = 0;
+    // Elimination phase lv: the updates of the level-lv pivots and the stores
+    // of the level-(lv-1) multipliers and diagonal reciprocals.  An update
+    // recomputes its multiplier m = a_ik / a_kk exactly as the store does
+    // (same operands, same operation), so the values equal the stored ones.
+    // The head has no pivoting: a zero or non-finite pivot makes the setup
+    // fail (CVODE's lsetup failure, as the tail's singular Gauss-Jordan step
+    // does) instead of passing inf/NaN on.  Every item carries its run's
+    // target, whose old value is gathered with the operands (no target is an
+    // operand within a phase), so the run's last item stores without waiting
+    // for an LDS read.
+#pragma unroll 1
+    for (int ph = 0; ph < NET::NEP; ph++) {
       MT acc = (MT)0.0;
-      for_items_gc<8>(opaque(NET::inv_tab), NET::inv_off[lv], NET::inv_n[lv], c.tid,
-                      [&](uint64_t w) { return Pair<MT>{LU[w & 0x3fff], LU[(w >> 14) & 0x3fff]}; },
-                      [&](uint64_t w, Pair<MT> g) {
-                        acc = fma(g.a, g.b, acc);
-                        if ((w >> 56) & 1) {
-                          LU[(w >> 28) & 0x3fff] = -acc * LU[(w >> 42) & 0x3fff];
-                          acc = (MT)0.0;
-                        }
-                      });
-      __syncthreads();
+      const int ng = NET::sp_groups[ph];
+#pragma unroll 1
+      for (int g = 0; g < ng; g++, pos += NET::SU)
+        stream_group(pre, st, pos, c.tid,
+                     [&](uint64_t w) {
+                       return Quad<MT>{LU[w & 0x3fff], LU[(w >> 14) & 0x3fff], LU[(w >> 28) & 0x3fff],
+                                       LU[(w >> 42) & 0x3fff]};
+                     },
+                     [&](uint64_t w, Quad<MT> q) {
+                       // FP32 form: the hardware reciprocal (1 ulp) in place of
+                       // an exactly rounded division; the factors only
+                       // precondition the FP64 Newton iteration.  A pivot whose
+                       // reciprocal is not finite (zero, or denormal and
+                       // flushed) is rejected.
+                       MT m;
+                       bool bad;
+                       if constexpr (sizeof(MT) == 4) {
+                         const MT r = __builtin_amdgcn_rcpf(q.p);
+                         m = q.a * r;
+                         bad = !(isfinite(r) && isfinite(q.p));
+                       } else {
+                         m = q.a / q.p;
+                         bad = !(q.p != (MT)0 && isfinite(q.p));
+                       }
+                       acc = fma(m, q.b, acc);
+                       if ((w >> 56) & 1) {
+                         const bool store = (w >> 57) & 1;
+                         LU[(w >> 42) & 0x3fff] = store ? m : q.t - acc;
+                         acc = (MT)0.0;
+                         if (store && bad) S.badpiv = 1;
+                       }
+                     });
+      lds_sync();
+    }
+    if (S.badpiv) return 1;  // uniform: read after the last elimination barrier
+    // explicit inverses of the head triangles, level by level:
+    // target <- -(sum of src1*src2) * LU[scale]; the scale is gathered with
+    // the operands
+#pragma unroll 1
+    for (int ph = NET::NEP; ph < NET::NSP; ph++) {
+      MT acc = (MT)0.0;
+      const int ng = NET::sp_groups[ph];
+#pragma unroll 1
+      for (int g = 0; g < ng; g++, pos += NET::SU)
+        stream_group(pre, st, pos, c.tid,
+                     [&](uint64_t w) {
+                       return Triple<MT>{LU[w & 0x3fff], LU[(w >> 14) & 0x3fff], LU[(w >> 42) & 0x3fff]};
+                     },
+                     [&](uint64_t w, Triple<MT> g3) {
+                       acc = fma(g3.a, g3.b, acc);
+                       if ((w >> 56) & 1) {
+                         LU[(w >> 28) & 0x3fff] = -acc * g3.t;
+                         acc = (MT)0.0;
+                       }
+                     });
+      lds_sync();
     }
     schur_mfma(c, LU);
-    __syncthreads();
+    lds_sync();
   }
   // tail inverse: every wavefront loads its 16 columns of the block
   const MT LDSA *Tl = LU + NET::TBASE;
@@ -597,14 +708,15 @@ __device__ __forceinline__ int wg_setup(Team<MT> &c, BdfU LDSA &u, BdfWG<MT> &b,
 }
 
 // one chunk of a sparse mat-vec per lane: P[tid] = sum of LU[pos] * B[col]
-// over the lane's entries (pos | col << 14)
-template <typename MT>
-__device__ __forceinline__ void spmv_partials(Team<MT> &c, const Ahead<8, uint32_t> &pre, const uint32_t *tab_,
-                                              int rounds, const MT LDSA *LU) {
+// over the lane's entries (pos | col << 14); `pre` holds the chunk's first
+// words on entry and the next stage's (ntab_, nrounds) on return
+template <int ROUNDS, int NROUNDS, typename MT>
+__device__ __forceinline__ void spmv_partials(Team<MT> &c, Ahead<8, uint32_t> &pre, const uint32_t *tab_,
+                                              const uint32_t *ntab_, const MT LDSA *LU) {
   double acc = 0.0;
-  for_items_gc_ahead<8>(pre, opaque(tab_), 0, rounds, c.tid,
-                        [&](uint32_t w) { return Pair<double>{(double)LU[w & 0x3fff], c.S->B[w >> 14]}; },
-                        [&](uint32_t, Pair<double> g) { acc = fma(g.a, g.b, acc); });
+  run_const<8, ROUNDS, NROUNDS>(pre, opaque(tab_), 0, opaque(ntab_), 0, c.tid,
+            [&](uint32_t w) { return Pair<double>{(double)LU[w & 0x3fff], c.S->B[w >> 14]}; },
+            [&](uint32_t, Pair<double> g) { acc = fma(g.a, g.b, acc); });
   c.S->P[c.tid] = acc;
 }
 
@@ -633,14 +745,12 @@ __device__ __forceinline__ double wg_solve(Team<MT> &c, const BdfU LDSA &u, cons
   lds_sync();
   if constexpr (NET::NHEAD > 0) {
     // S1: y_H = b_H + Linv b_H
-    spmv_partials(c, pre, NET::s1_tab, NET::S1_ROUNDS, LU);
-    pre.load(opaque(NET::s2_tab), 0, NET::S2_ROUNDS, c.tid);
+    spmv_partials<NET::S1_ROUNDS, NET::S2_ROUNDS>(c, pre, NET::s1_tab, NET::s2_tab, LU);
     lds_sync();
     if (c.tid < NET::NHEAD) S.B[NET::head_row[c.tid]] += row_total(S, NET::s1_base[c.tid], NET::s1_cnt[c.tid]);
     lds_sync();
     // S2: the tail rows' gather L_TH y_H (combined in the tail step)
-    spmv_partials(c, pre, NET::s2_tab, NET::S2_ROUNDS, LU);
-    pre.load(opaque(NET::s4_tab), 0, NET::S4_ROUNDS, c.tid);
+    spmv_partials<NET::S2_ROUNDS, NET::S4_ROUNDS>(c, pre, NET::s2_tab, NET::s4_tab, LU);
     lds_sync();
   }
   // tail: x_T = S^-1 b_T; the permuted right-hand side from wavefront 0,
@@ -670,13 +780,12 @@ __device__ __forceinline__ double wg_solve(Team<MT> &c, const BdfU LDSA &u, cons
   lds_sync();
   if constexpr (NET::NHEAD > 0) {
     // S4: z_H = y_H - U_HT x_T
-    spmv_partials(c, pre, NET::s4_tab, NET::S4_ROUNDS, LU);
-    pre.load(opaque(NET::s5_tab), 0, NET::S5_ROUNDS, c.tid);
+    spmv_partials<NET::S4_ROUNDS, NET::S5_ROUNDS>(c, pre, NET::s4_tab, NET::s5_tab, LU);
     lds_sync();
     if (c.tid < NET::NHEAD) S.B[NET::head_row[c.tid]] -= row_total(S, NET::s4_base[c.tid], NET::s4_cnt[c.tid]);
     lds_sync();
     // S5: x_H = Winv z_H
-    spmv_partials(c, pre, NET::s5_tab, NET::S5_ROUNDS, LU);
+    spmv_partials<NET::S5_ROUNDS, 0>(c, pre, NET::s5_tab, NET::s5_tab, LU);
     lds_sync();
     if (c.tid < NET::NHEAD) S.B[NET::head_row[c.tid]] = row_total(S, NET::s5_base[c.tid], NET::s5_cnt[c.tid]);
     lds_sync();
@@ -779,11 +888,11 @@ __device__ __forceinline__ void init_team(Team<MT> &c, WgLDS<MT> &lds, unsigned 
   c.red_buf = 0;
   c.S = (WgLDS<MT> LDSA *)&lds;
   unsigned char *slot = ws + (size_t)blockIdx.x * Slot<MT>::bytes;
-  c.Jg = (MT *)slot;
-  c.Kp = (double *)(slot + Slot<MT>::kp_off) + c.tid;
+  c.Jg = (gmut<MT>)slot;
+  c.Kp = (gmut<double>)(slot + Slot<MT>::kp_off) + c.tid;
   c.rtol = rtol;
   c.atol = atol;
-  c.fab = NET::slot_fab + c.tid;
+  c.fab = (gptr<uint32_t>)NET::slot_fab + c.tid;
   c.rbase = (c.tid < NET::NEQ) ? NET::row_base[c.tid] : 0;
   c.rnl = (c.tid < NET::NEQ) ? NET::row_nl[c.tid] : 0;
 }
@@ -839,6 +948,7 @@ __global__ void __launch_bounds__(BLOCK) wg_jac_kernel(const double *x, const do
   for (int t = 0; t < NET::MTM; t++) {
     const uint32_t d = NET::mpos[t * BLOCK + c.tid];
     const int row = d & 0xff, col = (d >> 8) & 0xff, src = d >> 16;
-    if (row < NET::NEQ && col <= NET::NEQ && src != 0xffff) o[(size_t)row * (NET::NEQ + 1) + col] = (double)c.Jg[src];
+    if (row < NET::NEQ && col <= NET::NEQ && src != 0xffff)
+      o[(size_t)row * (NET::NEQ + 1) + col] = (double)c.Jg[t * BLOCK + c.tid];
   }
 }

@@ -32,8 +32,8 @@ namespace {
 // 48 MiB: the counter (first 256 bytes), then the solver workspace.  A
 // whole-device launch on MI355X (256 CUs) needs at most 34.6 MB (reduced
 // network, FP64 form: 768 blocks of four 11,264-byte wavefront workspaces);
-// the full network needs 27.5 MB (FP32 matrix, 512 workgroups of 53,760
-// bytes) or 21.2 MB (FP64, 256 of 82,944).  A device on which a whole-device
+// the full network needs 46.1 MB (FP32 matrix, 512 workgroups of 90,112
+// bytes) or 39.8 MB (FP64, 256 of 155,648).  A device on which a whole-device
 // launch would need more runs the persistent kernel on as many blocks as fit.
 constexpr size_t kScratchBytes = size_t(48) << 20;
 constexpr size_t kCounterBytes = 256;

@@ -45,6 +45,8 @@ else:
 
 BLOCK = 256
 POS_BITS = 14          # LU positions (NLU + 1 < 16384)
+MGROUP = 8             # rounds per load group of the M build
+SU = 4                 # rounds per load group of the setup stream
 COEF_BITS = 4          # signed stoichiometric coefficients (-8..7)
 
 
@@ -59,6 +61,30 @@ def _lpt(loads: List[Tuple[int, object]], nbins: int) -> List[List[object]]:
         bins[b].append(item)
         heapq.heappush(heap, (load + w, b))
     return bins
+
+
+def _check_phase(lanes: List[List[int]], elim: bool) -> None:
+    """Within one barrier-separated phase no target may be written twice or
+    read as an operand by another item (lanes run concurrently)."""
+    tgts, srcs = set(), set()
+    for lst in lanes:
+        for w in lst:
+            if elim:
+                a, b, p_, t = w & 0x3FFF, (w >> 14) & 0x3FFF, (w >> 28) & 0x3FFF, (w >> 42) & 0x3FFF
+                srcs.update(x for x in (a, b, p_) if x != t)   # a store rescales its own entry
+                if (w >> 56) & 1:
+                    if t in tgts:
+                        raise ValueError("a phase writes a position twice")
+                    tgts.add(t)
+            else:
+                a, b, t, sc = w & 0x3FFF, (w >> 14) & 0x3FFF, (w >> 28) & 0x3FFF, (w >> 42) & 0x3FFF
+                srcs.update((a, b, sc))
+                if (w >> 56) & 1:
+                    if t in tgts:
+                        raise ValueError("a phase writes a position twice")
+                    tgts.add(t)
+    if tgts & srcs:
+        raise ValueError("a phase reads a position it writes")
 
 
 def rhs_word32(w16: int) -> int:
@@ -244,6 +270,13 @@ class WgTables:
         self.nlu = self.tbase + self.nt * self.tstride
         if self.nlu >= (1 << POS_BITS):
             raise ValueError("LU storage exceeds the position field")
+        # the saved Jacobian is kept by LU position, so that building M reads
+        # it in the same order as the position table: the Jacobian terms'
+        # entry field becomes the entry's LU position
+        self.jpos = [int(pos_of[i, j]) for (i, j) in self.jent]
+        if min(self.jpos) < 0:
+            raise ValueError("a Jacobian entry has no LU position")
+        self.jac_term = [[(w & 0x3FFFF) | (self.jpos[w >> 18] << 18) for w in row] for row in self.jac_term]
         self.tail = tail
         self.head = head
         allpos = hv + [None, "one"] + ["pad"] * (self.tbase - self.nhv - 2)
@@ -268,53 +301,65 @@ class WgTables:
             src = jid.get((i, j), 0xFFFF) if i < n else 0xFFFF
             self.mpos.append(i | (j << 8) | (src << 16))
         self.mtm = math.ceil(self.nlu / block)
-        self.mpos += [0xFF] * (self.mtm * block - self.nlu)
+        # padded to whole groups of the M build's loads (MGROUP rounds)
+        self.mtmp = MGROUP * math.ceil(self.mtm / MGROUP)
+        self.mpos += [0xFF] * (self.mtmp * block - self.nlu)
         # ------------------------------------------ head elimination levels
         hl = an.head_level
         self.nlev = int(hl.max()) + 1 if an.nhead else 0
         later_mask = {}
         rank = {v: a for a, v in enumerate(order)}
-        self.elimA: List[List[List[int]]] = []   # [level][round][lane]
-        self.elimB: List[List[List[int]]] = []
+        # Elimination phases.  Phase lv (0..NLEV) runs, in one barrier-
+        # separated step, the updates of the level-lv pivots and the stores of
+        # the level-(lv-1) pivots' multipliers and diagonal reciprocals.  An
+        # update term recomputes its multiplier a_ik / a_kk from the unscaled
+        # entry (final after phase lv-1; the store of L_ik happens only in
+        # phase lv+1), so the two never meet in one phase.  Item word:
+        #   a | b << 14 | p << 28 | target << 42 | last << 56 | store << 57
+        # m = a / p (hardware reciprocal in the FP32 form); acc += m * b; at
+        # `last`: target <- store ? m : target - acc.
+        zero, one = self.luzero, self.luone
+        word = lambda a, b, p_, t, last, store: a | (b << 14) | (p_ << 28) | (t << 42) | (last << 56) | (store << 57)
+        stores: List[List[Tuple[int, object, List[int]]]] = []   # per level: (weight, key, words)
+        runs: List[List[Tuple[int, object, List[int]]]] = []
         for lev in range(self.nlev):
             piv = [head[a] for a in range(an.nhead) if hl[a] == lev]
-            itemsA = []
-            upd: Dict[int, List[Tuple[int, int]]] = {}
+            st, upd = [], {}
             for k in piv:
                 dk = int(pos_of[k, k])
-                itemsA.append((1, ('d', k), (2 << 28) | wi_idx[(hpos[k], hpos[k])] | (dk << 14)))
+                st.append((1, ('d', k), [word(one, zero, dk, wi_idx[(hpos[k], hpos[k])], 1, 1)]))
                 rows_l = [i for i in range(NB) if F[i, k] and rank[i] > rank[k]]
                 cols_u = [j for j in range(NB) if F[k, j] and rank[j] > rank[k]]
                 for i in rows_l:
                     lp = int(pos_of[i, k])
-                    itemsA.append((1, ('l', i, k), (1 << 28) | lp | (dk << 14)))
+                    st.append((1, ('l', i, k), [word(lp, zero, dk, lp, 1, 1)]))
                     for j in cols_u:
                         if not (is_head[i] or is_head[j]):
                             continue  # tail block: Schur complement product
                         tp = int(pos_of[i, j])
                         assert 0 <= tp < self.nhv, (i, j)
-                        upd.setdefault(tp, []).append((lp, int(pos_of[k, j])))
-            binsA = _lpt([(1, it) for it in itemsA], block)
-            ra = max(len(b) for b in binsA)
-            tabA = [[0] * block for _ in range(ra)]
-            for lane in range(block):
-                for t, it in enumerate(sorted(binsA[lane], key=str)):
-                    tabA[t][lane] = it[2]
-            self.elimA.append(tabA)
-            binsB = _lpt([(len(v), tp) for tp, v in upd.items()], block)
+                        upd.setdefault(tp, []).append((lp, int(pos_of[k, j]), dk))
+            stores.append(st)
+            rl = []
+            for tp in sorted(upd):
+                us = upd[tp]
+                rl.append((len(us), ('u', tp), [word(lp, up, dk, tp, 1 if t == len(us) - 1 else 0, 0)
+                                                for t, (lp, up, dk) in enumerate(us)]))
+            runs.append(rl)
+        pad_e = word(zero, zero, one, zero, 0, 0)
+        self.elim_phases: List[List[List[int]]] = []   # [phase][lane] -> words
+        for ph in range(self.nlev + 1 if self.nlev else 0):
+            jobs = (runs[ph] if ph < self.nlev else []) + (stores[ph - 1] if ph > 0 else [])
+            bins = _lpt([(w, (key, words)) for (w, key, words) in jobs], block)
             lanes = []
             for lane in range(block):
                 lst = []
-                for tp in sorted(binsB[lane]):
-                    us = upd[tp]
-                    for t, (lp, up) in enumerate(us):
-                        last = 1 if t == len(us) - 1 else 0
-                        lst.append(tp | (lp << 14) | (up << 28) | (last << 42))
+                for key, words in sorted(bins[lane], key=lambda kw: str(kw[0])):
+                    lst += words
                 lanes.append(lst)
-            rb = max((len(l) for l in lanes), default=0)
-            padB = self.luzero | (self.luzero << 14) | (self.luzero << 28)
-            self.elimB.append([[(lanes[lane][t] if t < len(lanes[lane]) else padB)
-                                for lane in range(block)] for t in range(rb)])
+            _check_phase(lanes, elim=True)
+            self.elim_phases.append(lanes)
+        self.pad_e = pad_e
         # ------------------------- Schur complement / tail coupling tables
         # KH head pivots padded to a multiple of 4 (MFMA K steps); the
         # multipliers L[t][k] of tail rows and U[k][j] of tail columns
@@ -374,8 +419,35 @@ class WgTables:
         s5 = [[(wi_idx[(a, cc)], head[cc]) for cc in sorted(wi_pat[a])] for a in range(h)]
         self.spmv = [self._spmv_tab(rows) for rows in (s1, s2, s4, s5)]
         self.n_inv_terms = sum(sum(1 for row in lev for v in row if (v >> 56) & 1) for lev in self.inv_levels)
+        self._build_stream()
         self.diag_pos = [int(pos_of[i, i]) for i in range(NB)]
         self.charges = net.charges()
+
+    def _build_stream(self) -> None:
+        """The setup's elimination and inverse phases as one [round][lane]
+        stream of 64-bit words, every phase padded to whole groups of SU
+        rounds (so the kernel's loads are unconditional and the group after a
+        phase's last is the next phase's first), plus one group of padding at
+        the end for the last prefetch.  Empty phases are dropped."""
+        block = self.block
+        pad_i = self.luzero | (self.luzero << 14) | (self.luzero << 28) | (self.luzero << 42)
+        phases = []
+        for lanes in self.elim_phases:
+            r = max((len(l) for l in lanes), default=0)
+            if r:
+                r = SU * math.ceil(r / SU)
+                phases.append([[lanes[l][t] if t < len(lanes[l]) else self.pad_e for l in range(block)]
+                               for t in range(r)])
+        self.nep = len(phases)
+        for lev in self.inv_levels:
+            if not lev:
+                continue
+            _check_phase([[lev[t][l] for t in range(len(lev))] for l in range(block)], elim=False)
+            r = SU * math.ceil(len(lev) / SU)
+            phases.append([list(lev[t]) if t < len(lev) else [pad_i] * block for t in range(r)])
+        self.SU = SU
+        self.sp_groups = [len(ph) // SU for ph in phases]
+        self.stream = [w for ph in phases for row in ph for w in row] + [self.pad_e] * (SU * block)
 
     def _term_tab(self, per_t):
         """Targets with their (src1, src2) terms -> [round][lane] 64-bit items,
@@ -452,6 +524,8 @@ class WgTables:
         c("NT", self.nt, "tail size")
         c("TS", self.tstride, "row stride of the tail block")
         c("NLU", self.nlu); c("MTM", self.mtm, "M-build rounds")
+        c("MTMP", self.mtmp, "M-build rounds padded to whole load groups (the saved Jacobian's size in rounds)")
+        c("MGROUP", MGROUP, "rounds per M-build load group")
         c("KH", self.kh, "head pivots padded to a multiple of 4")
         c("NHEAD", self.nh_piv); c("NLEV", self.nlev, "head elimination levels")
         c("LUONE", self.luone, "LU[LUONE] = 1")
@@ -477,37 +551,24 @@ class WgTables:
         rn = [self.row_nl[i] if i < self.neq else 0 for i in range(self.block)]
         arr("uint16_t", "row_base", rb)
         arr("uint8_t", "row_nl", rn)
-        L.append("// [t][lane]: D index | coef << 13 | last << 17 | entry << 18")
+        L.append("// [t][lane]: D index | coef << 13 | last << 17 | LU position of the entry << 18")
         arr("uint32_t", "jac_term", [v for row in self.jac_term for v in row], hx, 8)
         L.append("// entry -> row | col << 8 (col NEQ: d/dx_e)")
         arr("uint16_t", "jent", [i | (j << 8) for (i, j) in self.jent], hx, 12)
-        L.append("// [t][lane]: row | col << 8 | J entry << 16 (0xffff: none); row 255 = padding")
+        L.append("// [t][lane]: row | col << 8 | J entry << 16 (0xffff: none; the saved Jacobian holds")
+        L.append("// the entry at this position); row 255 = padding")
         arr("uint32_t", "mpos", self.mpos, hx, 8)
-        # elimination
-        offA, offB, ra, rbl = [], [], [], []
-        flatA, flatB = [], []
-        for lev in range(self.nlev):
-            offA.append(len(flatA) // self.block); ra.append(len(self.elimA[lev]))
-            for row in self.elimA[lev]:
-                flatA += row
-            offB.append(len(flatB) // self.block); rbl.append(len(self.elimB[lev]))
-            for row in self.elimB[lev]:
-                flatB += row
-        L.append("// head elimination, phase A [round][lane]: kind << 28 | diag pos << 14 | target")
-        L.append("//   kind 1: multiplier at target pos; kind 2: reciprocal of pivot row `target`")
-        arr("int", "elimA_off", offA); arr("int", "elimA_n", ra)
-        arr("uint32_t", "elimA", flatA, hx, 8)
-        L.append("// phase B [round][lane]: target | l pos << 14 | u pos << 28 | last << 42")
-        arr("int", "elimB_off", offB); arr("int", "elimB_n", rbl)
-        arr("uint64_t", "elimB", flatB, hx64, 6)
-        off, cnt, flat = [], [], []
-        for lev in self.inv_levels:
-            off.append(len(flat) // self.block); cnt.append(len(lev))
-            for row in lev:
-                flat += row
-        L.append("// head inverses [round][lane]: src1 | src2 << 14 | target << 28 | scale << 42 | last << 56")
-        arr("int", "inv_off", off); arr("int", "inv_n", cnt)
-        arr("uint64_t", "inv_tab", flat, hx64, 6)
+        L.append("// setup stream [round][lane]: the elimination phases (NEP), then the head-inverse")
+        L.append("// phases; every phase a whole number of SU-round groups (sp_groups), one spare group")
+        L.append("// at the end.  Elimination word: a | b << 14 | p << 28 | target << 42 | last << 56 |")
+        L.append("// store << 57 (m = a / p; acc += m * b; at last: target <- store ? m : target - acc).")
+        L.append("// Inverse word: src1 | src2 << 14 | target << 28 | scale << 42 | last << 56")
+        L.append("// (acc += src1 * src2; at last: target <- -acc * scale).")
+        c("SU", SU, "rounds per setup-stream load group")
+        c("NEP", self.nep, "elimination phases")
+        c("NSP", len(self.sp_groups), "setup-stream phases")
+        arr("int", "sp_groups", self.sp_groups)
+        arr("uint64_t", "setup_stream", self.stream, hx64, 6)
         for nm, sp in zip(("s1", "s2", "s4", "s5"), self.spmv):
             L.append(f"// solve stage {nm} [round][lane]: LU position | B index << 14")
             c(f"{nm.upper()}_ROUNDS", sp["rounds"])
@@ -527,13 +588,12 @@ class WgTables:
             fh.write("\n".join(L) + "\n")
 
     def summary(self) -> str:
-        ra = sum(len(x) for x in self.elimA)
-        rb = sum(len(x) for x in self.elimB)
+        ne = sum(self.sp_groups[:self.nep]) * SU
+        ni = sum(self.sp_groups[self.nep:]) * SU
         return (f"{self.net.name}: NB={self.nb} FR={self.fr} VL={self.vl} TM={self.tm} PM={self.pm} "
                 f"NJ={self.nj} JTM={self.jtm} NLU={self.nlu} (NHV={self.nhv}, NT={self.nt}, KH={self.kh}) "
-                f"MTM={self.mtm} "
-                f"levels={self.nlev} elimA rounds={ra} elimB rounds={rb} "
-                f"NHI={self.nhi} inv rounds={sum(len(x) for x in self.inv_levels)}/{len(self.inv_levels)} "
+                f"MTM={self.mtm} levels={self.nlev} elimination phases={self.nep} rounds={ne} "
+                f"NHI={self.nhi} inverse phases={len(self.sp_groups) - self.nep} rounds={ni} "
                 f"spmv rounds={[sp['rounds'] for sp in self.spmv]}")
 
 

@@ -95,7 +95,7 @@ def test_jacobian_tables(case):
     D = np.zeros(2 * tab.nslot + 1)
     D[0:2 * tab.nslot:2] = kp * X[(fab >> 8) & 0xFF]
     D[1:2 * tab.nslot:2] = kp * X[fab & 0xFF]
-    Jg = np.full(tab.nj, np.nan)
+    Jg = np.full(tab.mtmp * tab.block, np.nan)   # the saved Jacobian, by LU position
     for lane in range(tab.block):
         acc = 0.0
         for t in range(tab.jtm):
@@ -104,11 +104,12 @@ def test_jacobian_tables(case):
             if (w >> 17) & 1:
                 Jg[w >> 18] = acc
                 acc = 0.0
-    assert not np.any(np.isnan(Jg)), "every entry is written"
+    assert not np.any(np.isnan(Jg[tab.jpos])), "every entry is written"
+    assert np.isnan(Jg).sum() == Jg.size - len(tab.jpos), "only the entries' positions are written"
     Jb = orc.jac_bordered(rate, nH, x)
     got = np.zeros_like(Jb)
     for e, (i, j) in enumerate(tab.jent):
-        got[i, j] = Jg[e]
+        got[i, j] = Jg[tab.jpos[e]]
     mask = np.zeros_like(Jb, dtype=bool)
     for (i, j) in tab.jent:
         mask[i, j] = True
@@ -123,6 +124,32 @@ def _spmv(tab, sp, LU, B):
         for lane, v in enumerate(rnd):
             P[lane] += LU[v & 0x3FFF] * B[v >> 14]
     return [P[sp["base"][r]:sp["base"][r] + sp["cnt"][r]].sum() for r in range(len(sp["base"]))]
+
+
+def _run_setup_stream(tab, LU):
+    """The kernel's setup stream on numpy arrays: phase by phase, every lane's
+    items in round order, all lanes of a round together (no item of a phase
+    reads what another writes, which the generator checks)."""
+    st = np.asarray(tab.stream, dtype=np.uint64).reshape(-1, tab.block)
+    f = lambda v, sh: ((v >> np.uint64(sh)) & np.uint64(0x3FFF)).astype(np.int64)
+    r0 = 0
+    for ph, ng in enumerate(tab.sp_groups):
+        acc = np.zeros(tab.block)
+        for rnd in st[r0:r0 + ng * tab.SU]:
+            last = ((rnd >> np.uint64(56)) & np.uint64(1)).astype(bool)
+            if ph < tab.nep:
+                a, b, p_, t = f(rnd, 0), f(rnd, 14), f(rnd, 28), f(rnd, 42)
+                store = ((rnd >> np.uint64(57)) & np.uint64(1)).astype(bool)
+                m = LU[a] / LU[p_]
+                acc += m * LU[b]
+                new = np.where(store, m, LU[t] - acc)
+                LU[t[last]] = new[last]
+            else:
+                a, b, t, sc = f(rnd, 0), f(rnd, 14), f(rnd, 28), f(rnd, 42)
+                acc += LU[a] * LU[b]
+                LU[t[last]] = (-acc * LU[sc])[last]
+            acc[last] = 0.0
+        r0 += ng * tab.SU
 
 
 def _factor_and_solve(tab, net, Jb, gamma, w, rhs):
@@ -144,32 +171,7 @@ def _factor_and_solve(tab, net, Jb, gamma, w, rhs):
         else:
             m = (-gamma * Jg[src] if src != 0xFFFF else 0.0) + (1.0 if row == col else 0.0)
         LU[p] = m * w[row] / w[col]
-    for lv in range(tab.nlev):
-        for rnd in tab.elimA[lv]:
-            for v in rnd:
-                kind, tgt, dp = v >> 28, v & 0x3FFF, (v >> 14) & 0x3FFF
-                if kind == 1:
-                    LU[tgt] /= LU[dp]
-                elif kind == 2:
-                    LU[tgt] = 1.0 / LU[dp]
-        acc = np.zeros(tab.block)
-        for rnd in tab.elimB[lv]:
-            for lane, v in enumerate(rnd):
-                tgt, lp, up = v & 0x3FFF, (v >> 14) & 0x3FFF, (v >> 28) & 0x3FFF
-                acc[lane] += LU[lp] * LU[up]
-                if (v >> 42) & 1:
-                    LU[tgt] -= acc[lane]
-                    acc[lane] = 0.0
-    for lev in tab.inv_levels:
-        acc = np.zeros(tab.block)
-        for rnd in lev:
-            for lane, v in enumerate(rnd):
-                s1, s2 = v & 0x3FFF, (v >> 14) & 0x3FFF
-                tgt, sc = (v >> 28) & 0x3FFF, (v >> 42) & 0x3FFF
-                acc[lane] += LU[s1] * LU[s2]
-                if (v >> 56) & 1:
-                    LU[tgt] = -acc[lane] * LU[sc]
-                    acc[lane] = 0.0
+    _run_setup_stream(tab, LU)
     NT, TS, KH = tab.nt, tab.tstride, tab.kh
     T = np.array([[LU[tab.tbase + r * TS + c] for c in range(NT)] for r in range(NT)])
     if KH:
