@@ -261,27 +261,6 @@ __device__ __forceinline__ void run_const(Ahead<U, W> &a, gptr<T> tab, int off, 
   }
 }
 
-// One group of U rounds of a [round][lane] stream at round `pos`: the
-// group's words are in `a`; the LDS operands of the whole group are fetched
-// by gather(w), then the next group's words (round pos + U) are loaded
-// unconditionally, then consume(w, g) runs on the group in order.
-template <int U, typename G, typename C>
-__device__ __forceinline__ void stream_group(Ahead<U, uint64_t> &a, gptr<uint64_t> st, int pos, int tid, G &&gather,
-                                             C &&consume) {
-  using V = decltype(gather(uint64_t()));
-  uint64_t cur[U];
-  V g[U];
-#pragma unroll
-  for (int u = 0; u < U; u++) {
-    cur[u] = a.w[u];
-    g[u] = gather(cur[u]);
-  }
-#pragma unroll
-  for (int u = 0; u < U; u++) a.w[u] = st[(size_t)(pos + U + u) * BLOCK + tid];
-#pragma unroll
-  for (int u = 0; u < U; u++) consume(cur[u], g[u]);
-}
-
 template <typename MT>
 struct Pair {
   MT a, b;
@@ -460,17 +439,34 @@ __device__ __forceinline__ void schur_mfma(const Team<double> &c, double LDSA *L
 // picks the pivot -- the unused row with the largest |a[r][k]|, ties to the
 // lower row, by a DPP max over packed value/row keys -- and publishes the
 // row index, its reciprocal and the whole column in LDS (double-buffered,
-// so one barrier per step);
-// every wavefront then updates its 16 columns, reading the pivot row's
-// entries with v_readlane.  Rows are not moved: A4 ends as the inverse in
-// pivot order, invp = step at which this row was the pivot.  Returns false
-// if the matrix is singular (uniform over the workgroup).
+// so one barrier per step); every wavefront then updates its 16 columns,
+// reading the pivot row's entries with v_readlane (the pivot row is taken
+// into a scalar register once per step, and the 16 reads are issued
+// together).  The owner of column k+1 updates that column first and
+// publishes step k+1's pivot before its other columns, so the next pivot
+// search overlaps the rest of step k (each entry's arithmetic is the same in
+// either order).  Rows are not moved: A4 ends as the inverse in pivot order,
+// invp = step at which this row was the pivot.  Returns false if the matrix
+// is singular (uniform over the workgroup).  LDS only: the barriers leave
+// global loads in flight.
 // ---------------------------------------------------------------------
 template <typename MT>
 __device__ __forceinline__ bool tail_gj(Team<MT> &c, MT (&A4)[16], int &invp) {
   WgLDS<MT> LDSA &S = *c.S;
   bool used = c.lane >= NET::NT;
   invp = c.lane;
+  // the owner of column k = 16 * (its wavefront) + jk publishes step k's pivot
+  auto publish = [&](int jk, int buf) {
+    const MT colk = A4[jk];
+    bool ok;
+    const int p = pivot_row<6, MT>(wave_max_u(used ? 0u : pivot_key<6>(fabs(colk), c.lane)), ok);
+    S.gjcol[buf][c.lane] = colk;
+    if (c.lane == 0) {
+      S.gjp[buf] = ok ? p : -1;
+      S.gjd[buf] = ok ? (MT)1.0 / rdlane(colk, p) : (MT)0.0;
+    }
+  };
+  if (c.wave == 0) publish(0, 0);
 #pragma unroll 1
   for (int wk = 0; wk < 4; wk++) {
 #pragma unroll
@@ -478,30 +474,34 @@ __device__ __forceinline__ bool tail_gj(Team<MT> &c, MT (&A4)[16], int &invp) {
       const int k = 16 * wk + jk;
       if (k >= NET::NT) break;
       const int buf = k & 1;
-      if (c.wave == wk) {
-        const MT colk = A4[jk];
-        bool ok;
-        const int p = pivot_row<6, MT>(wave_max_u(used ? 0u : pivot_key<6>(fabs(colk), c.lane)), ok);
-        S.gjcol[buf][c.lane] = colk;
-        if (c.lane == 0) {
-          S.gjp[buf] = ok ? p : -1;
-          S.gjd[buf] = ok ? (MT)1.0 / rdlane(colk, p) : (MT)0.0;
-        }
-      }
-      __syncthreads();
-      const int p = S.gjp[buf];
+      lds_sync();
+      const int p = uni(S.gjp[buf]);
       if (p < 0) return false;
       const MT d = S.gjd[buf];
       const MT fi = S.gjcol[buf][c.lane];
       const bool isp = (c.lane == p);
       if (isp) { used = true; invp = k; }
+      MT pr[16];
+#pragma unroll
+      for (int j = 0; j < 16; j++) pr[j] = rdlane(A4[j], p);
+      // column k+1 (wavefront wn, slot jn) first, by its owner, which then
+      // publishes step k+1's pivot while the rest of step k proceeds
+      const int jn = (jk + 1) & 15;  // a constant once the jk loop unrolls
+      const int wn = (jk == 15) ? wk + 1 : wk;
+      const bool ahead = (k + 1 < NET::NT) && (c.wave == wn);
+      if (ahead) {
+        const MT pj = pr[jn] * d;
+        A4[jn] = isp ? pj : fma(-fi, pj, A4[jn]);
+        publish(jn, (k + 1) & 1);
+      }
 #pragma unroll
       for (int j = 0; j < 16; j++) {
-        const MT pj = rdlane(A4[j], p) * d;
-        MT nv = isp ? pj : fma(-fi, pj, A4[j]);
-        if (c.wave == wk && j == jk) nv = isp ? d : -fi * d;
-        A4[j] = nv;
+        if (j == jn && ahead) continue;
+        const MT pj = pr[j] * d;
+        A4[j] = isp ? pj : fma(-fi, pj, A4[j]);
       }
+      // the pivot column becomes the elimination's column of the inverse
+      if (c.wave == wk) A4[jk] = isp ? d : -fi * d;
     }
   }
   return true;
@@ -608,85 +608,116 @@ __device__ __forceinline__ int wg_setup(Team<MT> &c, BdfU LDSA &u, BdfWG<MT> &b,
     }
   }
   // Head elimination and head inverses: the setup stream, phase by phase
-  // (one barrier each), in whole groups of SU table rounds.  The first group
-  // is loaded before the barrier; each group's loads are issued before the
-  // previous group is consumed, and the group after a phase's last is the
-  // next phase's first, so table latency overlaps the work and the barriers
-  // (which order LDS only from here on).
+  // (one barrier each), in whole groups of SU table rounds.  The table words
+  // are loaded two groups ahead of their use (the first two before the
+  // barrier), unconditionally, and a phase's last groups are followed in the
+  // stream by the next phase's first, so table latency overlaps the work and
+  // the barriers (which order LDS only from here on).  A word holds its four
+  // LU positions as byte offsets of 4-byte entries (one instruction extracts
+  // each address) and its flags in the offsets' zero low bits; a phase ends
+  // after the group whose first word has bit 0 set.  Stores are branch-free:
+  // an item that stores nothing writes its lane's dummy slot (the tail
+  // block's padding column, which nothing reads).
   const gptr<uint64_t> st = opaque(NET::setup_stream);
-  Ahead<NET::SU, uint64_t> pre;
-  if constexpr (NET::NSP > 0) pre.load(st, 0, NET::SU, c.tid);
+  Ahead<NET::SU, uint64_t> a1, a2;
+  if constexpr (NET::NSP > 0) {
+    a1.load(st, 0, NET::SU, c.tid);
+    a2.load(st, NET::SU, NET::SU, c.tid);
+  }
   lds_sync();
   if constexpr (NET::NSP > 0) {
-    int pos = 0;
+    static_assert(NET::NT == 64 && NET::TS == 65, "the dummy slots are the tail block's padding column");
+    // LU entry at byte offset off4 of a 4-byte-entry layout (position * 4)
+    auto at = [&](uint32_t off4) -> MT LDSA & {
+      return *(MT LDSA *)((unsigned char LDSA *)LU + off4 * (sizeof(MT) / 4));
+    };
+    const uint32_t dummy4 = 4u * (NET::TBASE + (c.tid & 63) * NET::TS + 64);
+    int pos = 0;  // stream round of the group in a1
+    // one group: gather the operands of the words in a1, shift the prefetch
+    // window, consume; true if the phase ends with this group
+    auto group = [&](auto &&gather, auto &&consume) -> bool {
+      uint64_t cur[NET::SU];
+      decltype(gather(uint64_t())) g[NET::SU];
+#pragma unroll
+      for (int u = 0; u < NET::SU; u++) {
+        cur[u] = a1.w[u];
+        g[u] = gather(cur[u]);
+      }
+#pragma unroll
+      for (int u = 0; u < NET::SU; u++) a1.w[u] = a2.w[u];
+      a2.load(st, pos + 2 * NET::SU, NET::SU, c.tid);
+      pos += NET::SU;
+#pragma unroll
+      for (int u = 0; u < NET::SU; u++) consume(cur[u], g[u]);
+      return uni((int)((uint32_t)cur[0] & 1u)) != 0;
+    };
     // Elimination phase lv: the updates of the level-lv pivots and the stores
     // of the level-(lv-1) multipliers and diagonal reciprocals.  An update
     // recomputes its multiplier m = a_ik / a_kk exactly as the store does
     // (same operands, same operation), so the values equal the stored ones.
-    // The head has no pivoting: a zero or non-finite pivot makes the setup
-    // fail (CVODE's lsetup failure, as the tail's singular Gauss-Jordan step
-    // does) instead of passing inf/NaN on.  Every item carries its run's
-    // target, whose old value is gathered with the operands (no target is an
-    // operand within a phase), so the run's last item stores without waiting
-    // for an LDS read.
+    // Every item carries its run's target, whose old value is gathered with
+    // the operands (no target is an operand within a phase), so the run's
+    // last item stores without waiting for an LDS read.  FP32 form: the
+    // hardware reciprocal (1 ulp) in place of an exactly rounded division;
+    // the factors only precondition the FP64 Newton iteration.
 #pragma unroll 1
     for (int ph = 0; ph < NET::NEP; ph++) {
       MT acc = (MT)0.0;
-      const int ng = NET::sp_groups[ph];
-#pragma unroll 1
-      for (int g = 0; g < ng; g++, pos += NET::SU)
-        stream_group(pre, st, pos, c.tid,
-                     [&](uint64_t w) {
-                       return Quad<MT>{LU[w & 0x3fff], LU[(w >> 14) & 0x3fff], LU[(w >> 28) & 0x3fff],
-                                       LU[(w >> 42) & 0x3fff]};
-                     },
-                     [&](uint64_t w, Quad<MT> q) {
-                       // FP32 form: the hardware reciprocal (1 ulp) in place of
-                       // an exactly rounded division; the factors only
-                       // precondition the FP64 Newton iteration.  A pivot whose
-                       // reciprocal is not finite (zero, or denormal and
-                       // flushed) is rejected.
-                       MT m;
-                       bool bad;
-                       if constexpr (sizeof(MT) == 4) {
-                         const MT r = __builtin_amdgcn_rcpf(q.p);
-                         m = q.a * r;
-                         bad = !(isfinite(r) && isfinite(q.p));
-                       } else {
-                         m = q.a / q.p;
-                         bad = !(q.p != (MT)0 && isfinite(q.p));
-                       }
-                       acc = fma(m, q.b, acc);
-                       if ((w >> 56) & 1) {
-                         const bool store = (w >> 57) & 1;
-                         LU[(w >> 42) & 0x3fff] = store ? m : q.t - acc;
-                         acc = (MT)0.0;
-                         if (store && bad) S.badpiv = 1;
-                       }
-                     });
+      bool done;
+      do {
+        done = group(
+            [&](uint64_t w) {
+              const uint32_t lo = (uint32_t)w, hi = (uint32_t)(w >> 32);
+              return Quad<MT>{at(lo & 0xfffcu), at(lo >> 16), at(hi & 0xfffcu), at((hi >> 16) & 0xfffcu)};
+            },
+            [&](uint64_t w, Quad<MT> q) {
+              const uint32_t hi = (uint32_t)(w >> 32);
+              MT m;
+              if constexpr (sizeof(MT) == 4) m = q.a * __builtin_amdgcn_rcpf(q.p);
+              else m = q.a / q.p;
+              acc = fma(m, q.b, acc);
+              const bool last = (hi >> 16) & 1u, store = (hi >> 17) & 1u;
+              at(last ? ((hi >> 16) & 0xfffcu) : dummy4) = store ? m : q.t - acc;
+              acc = last ? (MT)0.0 : acc;
+            });
+      } while (!done);
       lds_sync();
     }
-    if (S.badpiv) return 1;  // uniform: read after the last elimination barrier
+    // The head has no pivoting: a zero or non-finite pivot makes the setup
+    // fail (CVODE's lsetup failure, as the tail's singular Gauss-Jordan step
+    // does) instead of passing inf/NaN on.  Checked once per pivot on its
+    // final diagonal entry and stored reciprocal (FP32: a reciprocal that is
+    // not finite, as for a zero or flushed denormal pivot, is rejected).
+    if (c.tid < NET::NHEAD) {
+      const uint32_t pc = NET::piv_chk[c.tid];
+      const MT pv = at(pc & 0xffffu), rv = at(pc >> 16);
+      bool bad;
+      if constexpr (sizeof(MT) == 4) bad = !(isfinite(rv) && isfinite(pv));
+      else bad = !(pv != (MT)0 && isfinite(pv));
+      if (bad) S.badpiv = 1;
+    }
+    lds_sync();
+    if (S.badpiv) return 1;  // uniform: read after the barrier
     // explicit inverses of the head triangles, level by level:
-    // target <- -(sum of src1*src2) * LU[scale]; the scale is gathered with
-    // the operands
+    // t <- -(sum of a*b) * c; the scale c is gathered with the operands
 #pragma unroll 1
     for (int ph = NET::NEP; ph < NET::NSP; ph++) {
       MT acc = (MT)0.0;
-      const int ng = NET::sp_groups[ph];
-#pragma unroll 1
-      for (int g = 0; g < ng; g++, pos += NET::SU)
-        stream_group(pre, st, pos, c.tid,
-                     [&](uint64_t w) {
-                       return Triple<MT>{LU[w & 0x3fff], LU[(w >> 14) & 0x3fff], LU[(w >> 42) & 0x3fff]};
-                     },
-                     [&](uint64_t w, Triple<MT> g3) {
-                       acc = fma(g3.a, g3.b, acc);
-                       if ((w >> 56) & 1) {
-                         LU[(w >> 28) & 0x3fff] = -acc * g3.t;
-                         acc = (MT)0.0;
-                       }
-                     });
+      bool done;
+      do {
+        done = group(
+            [&](uint64_t w) {
+              const uint32_t lo = (uint32_t)w, hi = (uint32_t)(w >> 32);
+              return Triple<MT>{at(lo & 0xfffcu), at(lo >> 16), at(hi & 0xfffcu)};
+            },
+            [&](uint64_t w, Triple<MT> g3) {
+              const uint32_t hi = (uint32_t)(w >> 32);
+              acc = fma(g3.a, g3.b, acc);
+              const bool last = (hi >> 16) & 1u;
+              at(last ? ((hi >> 16) & 0xfffcu) : dummy4) = -acc * g3.t;
+              acc = last ? (MT)0.0 : acc;
+            });
+      } while (!done);
       lds_sync();
     }
     schur_mfma(c, LU);

@@ -63,26 +63,38 @@ def _lpt(loads: List[Tuple[int, object]], nbins: int) -> List[List[object]]:
     return bins
 
 
+def stream_word(a: int, b: int, c: int, t: int, last: int, store: int = 0) -> int:
+    """A setup-stream word: four LU positions as LDS byte offsets of 4-byte
+    entries (position * 4, 16 bits each: a | b << 16 | c << 32 | t << 48),
+    `last` in bit 48 and `store` in bit 49 (the target offset's low bits,
+    which are zero); bit 0 (the first offset's low bit) is left for the
+    phase-end flag.  Elimination: operands a, b, pivot c; inverse: src1 a,
+    src2 b, scale c."""
+    for x in (a, b, c, t):
+        if not 0 <= x < (1 << POS_BITS):
+            raise ValueError("LU position out of range")
+    return (4 * a) | ((4 * b) << 16) | ((4 * c) << 32) | ((4 * t) << 48) | (last << 48) | (store << 49)
+
+
+def word_fields(w: int):
+    """(a, b, c, t, last, store, phase_end) of a setup-stream word."""
+    return ((w & 0xFFFC) >> 2, ((w >> 16) & 0xFFFC) >> 2, ((w >> 32) & 0xFFFC) >> 2, ((w >> 48) & 0xFFFC) >> 2,
+            (w >> 48) & 1, (w >> 49) & 1, w & 1)
+
+
 def _check_phase(lanes: List[List[int]], elim: bool) -> None:
     """Within one barrier-separated phase no target may be written twice or
     read as an operand by another item (lanes run concurrently)."""
     tgts, srcs = set(), set()
     for lst in lanes:
         for w in lst:
-            if elim:
-                a, b, p_, t = w & 0x3FFF, (w >> 14) & 0x3FFF, (w >> 28) & 0x3FFF, (w >> 42) & 0x3FFF
-                srcs.update(x for x in (a, b, p_) if x != t)   # a store rescales its own entry
-                if (w >> 56) & 1:
-                    if t in tgts:
-                        raise ValueError("a phase writes a position twice")
-                    tgts.add(t)
-            else:
-                a, b, t, sc = w & 0x3FFF, (w >> 14) & 0x3FFF, (w >> 28) & 0x3FFF, (w >> 42) & 0x3FFF
-                srcs.update((a, b, sc))
-                if (w >> 56) & 1:
-                    if t in tgts:
-                        raise ValueError("a phase writes a position twice")
-                    tgts.add(t)
+            a, b, c, t, last, _, _ = word_fields(w)
+            # an elimination store rescales its own entry (reads and writes t)
+            srcs.update(x for x in (a, b, c) if not (elim and x == t))
+            if last:
+                if t in tgts:
+                    raise ValueError("a phase writes a position twice")
+                tgts.add(t)
     if tgts & srcs:
         raise ValueError("a phase reads a position it writes")
 
@@ -314,12 +326,12 @@ class WgTables:
         # the level-(lv-1) pivots' multipliers and diagonal reciprocals.  An
         # update term recomputes its multiplier a_ik / a_kk from the unscaled
         # entry (final after phase lv-1; the store of L_ik happens only in
-        # phase lv+1), so the two never meet in one phase.  Item word:
-        #   a | b << 14 | p << 28 | target << 42 | last << 56 | store << 57
-        # m = a / p (hardware reciprocal in the FP32 form); acc += m * b; at
-        # `last`: target <- store ? m : target - acc.
+        # phase lv+1), so the two never meet in one phase.  Item word (see
+        # stream_word): operands a, b, p and target t; m = a / p (hardware
+        # reciprocal in the FP32 form); acc += m * b; at `last`:
+        # t <- store ? m : t - acc.
         zero, one = self.luzero, self.luone
-        word = lambda a, b, p_, t, last, store: a | (b << 14) | (p_ << 28) | (t << 42) | (last << 56) | (store << 57)
+        word = lambda a, b, p_, t, last, store: stream_word(a, b, p_, t, last, store)
         stores: List[List[Tuple[int, object, List[int]]]] = []   # per level: (weight, key, words)
         runs: List[List[Tuple[int, object, List[int]]]] = []
         for lev in range(self.nlev):
@@ -418,7 +430,10 @@ class WgTables:
         s4 = [[(int(pos_of[head[a], j]), j) for j in tail if F[head[a], j]] for a in range(h)]
         s5 = [[(wi_idx[(a, cc)], head[cc]) for cc in sorted(wi_pat[a])] for a in range(h)]
         self.spmv = [self._spmv_tab(rows) for rows in (s1, s2, s4, s5)]
-        self.n_inv_terms = sum(sum(1 for row in lev for v in row if (v >> 56) & 1) for lev in self.inv_levels)
+        self.n_inv_terms = sum(sum(1 for row in lev for v in row if word_fields(v)[4]) for lev in self.inv_levels)
+        # the head pivots' final diagonal entries and stored reciprocals,
+        # checked after the elimination: dpos * 4 | rpos * 4 << 16
+        self.piv_chk = [4 * int(pos_of[k, k]) | (4 * wi_idx[(hpos[k], hpos[k])]) << 16 for k in head]
         self._build_stream()
         self.diag_pos = [int(pos_of[i, i]) for i in range(NB)]
         self.charges = net.charges()
@@ -427,10 +442,11 @@ class WgTables:
         """The setup's elimination and inverse phases as one [round][lane]
         stream of 64-bit words, every phase padded to whole groups of SU
         rounds (so the kernel's loads are unconditional and the group after a
-        phase's last is the next phase's first), plus one group of padding at
-        the end for the last prefetch.  Empty phases are dropped."""
+        phase's last is the next phase's first), plus two groups of padding at
+        the end for the kernel's prefetch two groups ahead.  Empty phases are
+        dropped."""
         block = self.block
-        pad_i = self.luzero | (self.luzero << 14) | (self.luzero << 28) | (self.luzero << 42)
+        pad_i = stream_word(self.luzero, self.luzero, self.luzero, self.luzero, 0)
         phases = []
         for lanes in self.elim_phases:
             r = max((len(l) for l in lanes), default=0)
@@ -445,9 +461,14 @@ class WgTables:
             _check_phase([[lev[t][l] for t in range(len(lev))] for l in range(block)], elim=False)
             r = SU * math.ceil(len(lev) / SU)
             phases.append([list(lev[t]) if t < len(lev) else [pad_i] * block for t in range(r)])
+        # bit 0 of every lane's word in the first round of a phase's last
+        # group: the phase ends after this group (read by the kernel from the
+        # words it already holds, instead of a per-phase length)
+        for ph in phases:
+            ph[len(ph) - SU] = [w | 1 for w in ph[len(ph) - SU]]
         self.SU = SU
         self.sp_groups = [len(ph) // SU for ph in phases]
-        self.stream = [w for ph in phases for row in ph for w in row] + [self.pad_e] * (SU * block)
+        self.stream = [w for ph in phases for row in ph for w in row] + [self.pad_e] * (2 * SU * block)
 
     def _term_tab(self, per_t):
         """Targets with their (src1, src2) terms -> [round][lane] 64-bit items,
@@ -461,10 +482,10 @@ class WgTables:
                 terms = per_t[key] or [(self.luzero, self.luzero)]
                 for t, (a, b) in enumerate(terms):
                     last = 1 if t == len(terms) - 1 else 0
-                    lst.append(a | (b << 14) | (tgt << 28) | (scale << 42) | (last << 56))
+                    lst.append(stream_word(a, b, scale, tgt, last))
             lanes.append(lst)
         r = max((len(l) for l in lanes), default=0)
-        pad = self.luzero | (self.luzero << 14) | (self.luzero << 28) | (self.luzero << 42)
+        pad = stream_word(self.luzero, self.luzero, self.luzero, self.luzero, 0)
         return [[(lanes[lane][t] if t < len(lanes[lane]) else pad) for lane in range(self.block)]
                 for t in range(r)]
 
@@ -560,15 +581,19 @@ class WgTables:
         arr("uint32_t", "mpos", self.mpos, hx, 8)
         L.append("// setup stream [round][lane]: the elimination phases (NEP), then the head-inverse")
         L.append("// phases; every phase a whole number of SU-round groups (sp_groups), one spare group")
-        L.append("// at the end.  Elimination word: a | b << 14 | p << 28 | target << 42 | last << 56 |")
-        L.append("// store << 57 (m = a / p; acc += m * b; at last: target <- store ? m : target - acc).")
-        L.append("// Inverse word: src1 | src2 << 14 | target << 28 | scale << 42 | last << 56")
-        L.append("// (acc += src1 * src2; at last: target <- -acc * scale).")
+        L.append("// at the end.  Word: four LU positions as byte offsets of 4-byte entries (position * 4,")
+        L.append("// 16 bits each): a | b << 16 | c << 32 | t << 48; last = bit 48, store = bit 49; bit 0")
+        L.append("// (in every lane's word of a group's first round): the phase ends with this group.")
+        L.append("// Elimination: m = a / c; acc += m * b; at last: t <- store ? m : t - acc.")
+        L.append("// Inverse: acc += a * b; at last: t <- -acc * c.")
         c("SU", SU, "rounds per setup-stream load group")
         c("NEP", self.nep, "elimination phases")
         c("NSP", len(self.sp_groups), "setup-stream phases")
         arr("int", "sp_groups", self.sp_groups)
         arr("uint64_t", "setup_stream", self.stream, hx64, 6)
+        L.append("// head pivots: diagonal entry and stored reciprocal, as byte offsets (position * 4):")
+        L.append("// dpos | rpos << 16")
+        arr("uint32_t", "piv_chk", self.piv_chk, hx, 8)
         for nm, sp in zip(("s1", "s2", "s4", "s5"), self.spmv):
             L.append(f"// solve stage {nm} [round][lane]: LU position | B index << 14")
             c(f"{nm.upper()}_ROUNDS", sp["rounds"])

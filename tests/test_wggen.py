@@ -129,27 +129,34 @@ def _spmv(tab, sp, LU, B):
 def _run_setup_stream(tab, LU):
     """The kernel's setup stream on numpy arrays: phase by phase, every lane's
     items in round order, all lanes of a round together (no item of a phase
-    reads what another writes, which the generator checks)."""
+    reads what another writes, which the generator checks).  A phase ends
+    with the group whose first word carries the phase-end flag."""
     st = np.asarray(tab.stream, dtype=np.uint64).reshape(-1, tab.block)
-    f = lambda v, sh: ((v >> np.uint64(sh)) & np.uint64(0x3FFF)).astype(np.int64)
+    off = lambda v, sh: (((v >> np.uint64(sh)) & np.uint64(0xFFFC)) >> np.uint64(2)).astype(np.int64)
+    bit = lambda v, sh: ((v >> np.uint64(sh)) & np.uint64(1)).astype(bool)
     r0 = 0
     for ph, ng in enumerate(tab.sp_groups):
         acc = np.zeros(tab.block)
+        for g in range(ng):
+            assert bool(np.all(bit(st[r0 + g * tab.SU], 0))) == (g == ng - 1), "phase-end flag"
         for rnd in st[r0:r0 + ng * tab.SU]:
-            last = ((rnd >> np.uint64(56)) & np.uint64(1)).astype(bool)
+            a, b, c_, t = off(rnd, 0), off(rnd, 16), off(rnd, 32), off(rnd, 48)
+            last = bit(rnd, 48)
             if ph < tab.nep:
-                a, b, p_, t = f(rnd, 0), f(rnd, 14), f(rnd, 28), f(rnd, 42)
-                store = ((rnd >> np.uint64(57)) & np.uint64(1)).astype(bool)
-                m = LU[a] / LU[p_]
+                store = bit(rnd, 49)
+                m = LU[a] / LU[c_]
                 acc += m * LU[b]
                 new = np.where(store, m, LU[t] - acc)
                 LU[t[last]] = new[last]
             else:
-                a, b, t, sc = f(rnd, 0), f(rnd, 14), f(rnd, 28), f(rnd, 42)
                 acc += LU[a] * LU[b]
-                LU[t[last]] = (-acc * LU[sc])[last]
+                LU[t[last]] = (-acc * LU[c_])[last]
             acc[last] = 0.0
         r0 += ng * tab.SU
+    # the pivot check table names every head pivot's diagonal and reciprocal
+    for w in tab.piv_chk:
+        d, r = (w & 0xFFFF) >> 2, (w >> 16) >> 2
+        assert LU[r] == 1.0 / LU[d]
 
 
 def _factor_and_solve(tab, net, Jb, gamma, w, rhs):
