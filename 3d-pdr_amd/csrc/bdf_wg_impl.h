@@ -335,8 +335,16 @@ __device__ __forceinline__ double wg_rhs(Team<MT> &c, double y, double &xe_out) 
     S.P[p * BLOCK + c.tid] = acc;
   }
   lds_sync();
+  // the row's chunk partials in a fixed order, read 8 at a time (indices
+  // clamped to P[NV] = 0) so that a row waits for LDS once per 8 partials
   double yd = 0.0;
-  for (int u = 0; u < c.rnl; u++) yd += S.P[c.rbase + u];
+  for (int u0 = 0; u0 < c.rnl; u0 += 8) {
+    double p[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) p[u] = S.P[min(c.rbase + u0 + u, NET::NV)];
+#pragma unroll
+    for (int u = 0; u < 8; u++) yd = (u0 + u < c.rnl) ? yd + p[u] : yd;
+  }
   return yd;
 }
 
@@ -752,10 +760,17 @@ __device__ __forceinline__ void spmv_partials(Team<MT> &c, Ahead<8, uint32_t> &p
 }
 
 // sum of a row's chunk partials P[base .. base+cnt), in a fixed order
-template <typename MT>
+// (left to right from 0).  cnt <= MAXC: all MAXC partials are read at once
+// (index clamped to P[NV] = 0) and the first cnt added, so a row waits for
+// LDS once instead of once per partial.
+template <int MAXC, typename MT>
 __device__ __forceinline__ double row_total(const WgLDS<MT> LDSA &S, int base, int cnt) {
+  double p[MAXC > 0 ? MAXC : 1];
+#pragma unroll
+  for (int u = 0; u < MAXC; u++) p[u] = S.P[min(base + u, NET::NV)];
   double v = 0.0;
-  for (int u = 0; u < cnt; u++) v += S.P[base + u];
+#pragma unroll
+  for (int u = 0; u < MAXC; u++) v = (u < cnt) ? v + p[u] : v;
   return v;
 }
 
@@ -778,7 +793,7 @@ __device__ __forceinline__ double wg_solve(Team<MT> &c, const BdfU LDSA &u, cons
     // S1: y_H = b_H + Linv b_H
     spmv_partials<NET::S1_ROUNDS, NET::S2_ROUNDS>(c, pre, NET::s1_tab, NET::s2_tab, LU);
     lds_sync();
-    if (c.tid < NET::NHEAD) S.B[NET::head_row[c.tid]] += row_total(S, NET::s1_base[c.tid], NET::s1_cnt[c.tid]);
+    if (c.tid < NET::NHEAD) S.B[NET::head_row[c.tid]] += row_total<NET::S1_MAXC>(S, NET::s1_base[c.tid], NET::s1_cnt[c.tid]);
     lds_sync();
     // S2: the tail rows' gather L_TH y_H (combined in the tail step)
     spmv_partials<NET::S2_ROUNDS, NET::S4_ROUNDS>(c, pre, NET::s2_tab, NET::s4_tab, LU);
@@ -790,7 +805,7 @@ __device__ __forceinline__ double wg_solve(Team<MT> &c, const BdfU LDSA &u, cons
     double bt = 0.0;
     if (c.lane < NET::NT) {
       bt = S.B[NET::tail_row[c.lane]];
-      if constexpr (NET::NHEAD > 0) bt -= row_total(S, NET::s2_base[c.lane], NET::s2_cnt[c.lane]);
+      if constexpr (NET::NHEAD > 0) bt -= row_total<NET::S2_MAXC>(S, NET::s2_base[c.lane], NET::s2_cnt[c.lane]);
     }
     S.V[b.invp] = bt;
   }
@@ -813,12 +828,12 @@ __device__ __forceinline__ double wg_solve(Team<MT> &c, const BdfU LDSA &u, cons
     // S4: z_H = y_H - U_HT x_T
     spmv_partials<NET::S4_ROUNDS, NET::S5_ROUNDS>(c, pre, NET::s4_tab, NET::s5_tab, LU);
     lds_sync();
-    if (c.tid < NET::NHEAD) S.B[NET::head_row[c.tid]] -= row_total(S, NET::s4_base[c.tid], NET::s4_cnt[c.tid]);
+    if (c.tid < NET::NHEAD) S.B[NET::head_row[c.tid]] -= row_total<NET::S4_MAXC>(S, NET::s4_base[c.tid], NET::s4_cnt[c.tid]);
     lds_sync();
     // S5: x_H = Winv z_H
     spmv_partials<NET::S5_ROUNDS, 0>(c, pre, NET::s5_tab, NET::s5_tab, LU);
     lds_sync();
-    if (c.tid < NET::NHEAD) S.B[NET::head_row[c.tid]] = row_total(S, NET::s5_base[c.tid], NET::s5_cnt[c.tid]);
+    if (c.tid < NET::NHEAD) S.B[NET::head_row[c.tid]] = row_total<NET::S5_MAXC>(S, NET::s5_base[c.tid], NET::s5_cnt[c.tid]);
     lds_sync();
   }
   double x = c.own_ ? S.B[c.tid] / b.wsc : 0.0;

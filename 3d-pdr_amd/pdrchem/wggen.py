@@ -597,6 +597,7 @@ class WgTables:
         for nm, sp in zip(("s1", "s2", "s4", "s5"), self.spmv):
             L.append(f"// solve stage {nm} [round][lane]: LU position | B index << 14")
             c(f"{nm.upper()}_ROUNDS", sp["rounds"])
+            c(f"{nm.upper()}_MAXC", max(sp["cnt"], default=0), "most chunks of one row")
             arr("uint32_t", f"{nm}_tab", [v for row in sp["tab"] for v in row], hx, 8)
             arr("uint16_t", f"{nm}_base", sp["base"])
             arr("uint8_t", f"{nm}_cnt", sp["cnt"])
