@@ -566,7 +566,8 @@ __device__ __forceinline__ int wg_setup(Team<MT> &c, BdfU LDSA &u, BdfWG<MT> &b,
   const gptr<MT> jg = (gptr<MT>)c.Jg;
   auto m_value = [&](uint32_t d, MT j) -> double {
     const int row = d & 0xff, col = (d >> 8) & 0xff, src = d >> 16;
-    if (row == NET::NEQ) return (col == NET::NEQ) ? 0.0 : (double)NET::charge[col];
+    // the bordered row's charges are in the slot too (wg_slot_charges)
+    if (row == NET::NEQ) return (col == NET::NEQ) ? 0.0 : (double)j;
     return (row < 0xfd && src != 0xffff) ? (double)j : 0.0;
   };
   auto m_entry = [&](uint32_t d, double jv) -> double {
@@ -783,6 +784,10 @@ __device__ __forceinline__ double wg_solve(Team<MT> &c, const BdfU LDSA &u, cons
   WgLDS<MT> LDSA &S = *c.S;
   const MT LDSA *LU = regionLU(c.S);
   const double gamrat = u.gamrat;
+  // the lane's row indices, chunk bases and counts, loaded first: their
+  // latency overlaps the first stages instead of opening the row totals
+  const uint32_t sl0 = opaque(NET::solve_lane0)[c.tid], sl1 = opaque(NET::solve_lane1)[c.tid];
+  const int hrow = sl0 & 0xff, trow = sl1 & 0xff;
   // each mat-vec's first table words are loaded during the phase before it
   Ahead<8, uint32_t> pre;
   if constexpr (NET::NHEAD > 0) pre.load(opaque(NET::s1_tab), 0, NET::S1_ROUNDS, c.tid);
@@ -793,7 +798,7 @@ __device__ __forceinline__ double wg_solve(Team<MT> &c, const BdfU LDSA &u, cons
     // S1: y_H = b_H + Linv b_H
     spmv_partials<NET::S1_ROUNDS, NET::S2_ROUNDS>(c, pre, NET::s1_tab, NET::s2_tab, LU);
     lds_sync();
-    if (c.tid < NET::NHEAD) S.B[NET::head_row[c.tid]] += row_total<NET::S1_MAXC>(S, NET::s1_base[c.tid], NET::s1_cnt[c.tid]);
+    if (c.tid < NET::NHEAD) S.B[hrow] += row_total<NET::S1_MAXC>(S, (sl0 >> 8) & 0xff, (sl1 >> 16) & 0xf);
     lds_sync();
     // S2: the tail rows' gather L_TH y_H (combined in the tail step)
     spmv_partials<NET::S2_ROUNDS, NET::S4_ROUNDS>(c, pre, NET::s2_tab, NET::s4_tab, LU);
@@ -804,8 +809,8 @@ __device__ __forceinline__ double wg_solve(Team<MT> &c, const BdfU LDSA &u, cons
   if (c.wave == 0) {
     double bt = 0.0;
     if (c.lane < NET::NT) {
-      bt = S.B[NET::tail_row[c.lane]];
-      if constexpr (NET::NHEAD > 0) bt -= row_total<NET::S2_MAXC>(S, NET::s2_base[c.lane], NET::s2_cnt[c.lane]);
+      bt = S.B[trow];
+      if constexpr (NET::NHEAD > 0) bt -= row_total<NET::S2_MAXC>(S, (sl1 >> 8) & 0xff, (sl1 >> 20) & 0xf);
     }
     S.V[b.invp] = bt;
   }
@@ -821,19 +826,19 @@ __device__ __forceinline__ double wg_solve(Team<MT> &c, const BdfU LDSA &u, cons
     const double acc = (S.P[c.lane] + S.P[64 + c.lane]) + (S.P[128 + c.lane] + S.P[192 + c.lane]);
     S.V2[b.invp] = acc;
     wsync();
-    if (c.lane < NET::NT) S.B[NET::tail_row[c.lane]] = S.V2[c.lane];
+    if (c.lane < NET::NT) S.B[trow] = S.V2[c.lane];
   }
   lds_sync();
   if constexpr (NET::NHEAD > 0) {
     // S4: z_H = y_H - U_HT x_T
     spmv_partials<NET::S4_ROUNDS, NET::S5_ROUNDS>(c, pre, NET::s4_tab, NET::s5_tab, LU);
     lds_sync();
-    if (c.tid < NET::NHEAD) S.B[NET::head_row[c.tid]] -= row_total<NET::S4_MAXC>(S, NET::s4_base[c.tid], NET::s4_cnt[c.tid]);
+    if (c.tid < NET::NHEAD) S.B[hrow] -= row_total<NET::S4_MAXC>(S, (sl0 >> 16) & 0xff, (sl1 >> 24) & 0xf);
     lds_sync();
     // S5: x_H = Winv z_H
     spmv_partials<NET::S5_ROUNDS, 0>(c, pre, NET::s5_tab, NET::s5_tab, LU);
     lds_sync();
-    if (c.tid < NET::NHEAD) S.B[NET::head_row[c.tid]] = row_total<NET::S5_MAXC>(S, NET::s5_base[c.tid], NET::s5_cnt[c.tid]);
+    if (c.tid < NET::NHEAD) S.B[hrow] = row_total<NET::S5_MAXC>(S, sl0 >> 24, sl1 >> 28);
     lds_sync();
   }
   double x = c.own_ ? S.B[c.tid] / b.wsc : 0.0;
@@ -923,6 +928,19 @@ __device__ __forceinline__ void wg_solve_point(Team<MT> &c, const PointArgs &a, 
   __syncthreads();
 }
 
+// The bordered row's entries of M (the species' charges) into the Jacobian
+// slot at their LU positions, which the Jacobian never writes: the M build
+// then reads every entry's value from the slot, with no dependent load of
+// the charge table.  Each lane writes the positions it reads in the M build.
+template <typename MT>
+__device__ __forceinline__ void wg_slot_charges(Team<MT> &c) {
+  for (int t = 0; t < NET::MTMP; t++) {
+    const uint32_t d = NET::mpos[t * BLOCK + c.tid];
+    const int row = d & 0xff, col = (d >> 8) & 0xff;
+    if (row == NET::NEQ && col < NET::NEQ) c.Jg[t * BLOCK + c.tid] = (MT)NET::charge[col];
+  }
+}
+
 // a workgroup's constant state: thread roles, its device slot, tables
 template <typename MT>
 __device__ __forceinline__ void init_team(Team<MT> &c, WgLDS<MT> &lds, unsigned char *ws, double rtol, double atol) {
@@ -954,6 +972,7 @@ bdf_wg_kernel(PointArgs a, unsigned char *ws) {
   __shared__ unsigned int next;
   Team<MT> c;
   init_team(c, lds, ws, a.rtol, a.atol);
+  wg_slot_charges(c);
   const unsigned long long born = rt_ticks();
   for (;;) {
     if (c.tid == 0) next = a.published ? refill_take(a, born) : atomicAdd(a.counter, 1u);

@@ -601,6 +601,30 @@ class WgTables:
             arr("uint32_t", f"{nm}_tab", [v for row in sp["tab"] for v in row], hx, 8)
             arr("uint16_t", f"{nm}_base", sp["base"])
             arr("uint8_t", f"{nm}_cnt", sp["cnt"])
+        # per lane of the solve, packed: head row, the four stages' chunk
+        # bases (lane tid) and the tail row with its S2 base and every count
+        # (tail fields of lane tid & 63)
+        def sp_field(k, key, i, default):
+            v = self.spmv[k][key]
+            return v[i] if i < len(v) else default
+        w0, w1 = [], []
+        for tid in range(self.block):
+            tl = tid & 63
+            hr = self.head_row[tid] if tid < len(self.head_row) else self.nb
+            trow = self.tail[tl] if tl < self.nt else 0
+            b1, b4, b5 = (sp_field(k, "base", tid, 0) for k in (0, 2, 3))
+            c1, c4, c5 = (sp_field(k, "cnt", tid, 0) for k in (0, 2, 3))
+            b2, c2 = sp_field(1, "base", tl, 0), sp_field(1, "cnt", tl, 0)
+            for v, bits in ((hr, 8), (b1, 8), (b4, 8), (b5, 8), (trow, 8), (b2, 8), (c1, 4), (c2, 4), (c4, 4), (c5, 4)):
+                if not 0 <= v < (1 << bits):
+                    raise ValueError("solve lane field overflows")
+            w0.append(hr | (b1 << 8) | (b4 << 16) | (b5 << 24))
+            w1.append(trow | (b2 << 8) | (c1 << 16) | (c2 << 20) | (c4 << 24) | (c5 << 28))
+        L.append("// solve constants per lane: head_row | s1_base << 8 | s4_base << 16 | s5_base << 24 and")
+        L.append("// tail_row | s2_base << 8 | s1_cnt << 16 | s2_cnt << 20 | s4_cnt << 24 | s5_cnt << 28")
+        L.append("// (head fields of lane tid, tail fields of lane tid & 63)")
+        arr("uint32_t", "solve_lane0", w0, hx, 8)
+        arr("uint32_t", "solve_lane1", w1, hx, 8)
         arr("uint8_t", "tail_row", list(self.tail) + [0] * (64 - self.nt))
         arr("uint8_t", "head_row", self.head_row)
         L.append("// LU positions of L[tail t][head k] ([t][k]) and U[head k][tail j] ([k][j])")
