@@ -306,9 +306,20 @@ __device__ __forceinline__ double wg_rhs(Team<MT> &c, double y, double &xe_out) 
     fw[r] = c.fab[r * BLOCK];
     kp[r] = c.Kp[r * BLOCK];
   }
+  // the terms of the lane's VL chunks run interleaved -- item i is term
+  // i / VL of chunk i % VL -- in groups of RG items, so a group's LDS reads
+  // and the chunks' accumulation chains overlap; each chunk still adds its
+  // terms in order
+  constexpr int NI = NET::VL * NET::TM, RG = 2 * NET::VL;
   const gptr<uint32_t> rt = opaque(NET::rhs_term);
-  Ahead<6, uint32_t> pre;
-  pre.load(rt, 0, NET::TM, c.tid);
+  auto term_word = [&](int i) { return rt[(size_t)((i % NET::VL) * NET::TM + i / NET::VL) * BLOCK + c.tid]; };
+  // the words of the next two groups are in flight at any time
+  uint32_t tw[RG], tw2[RG];
+#pragma unroll
+  for (int u = 0; u < RG; u++) {
+    if (u < NI) tw[u] = term_word(u);
+    if (RG + u < NI) tw2[u] = term_word(RG + u);
+  }
   if (c.own_) S.X[c.tid] = y;
   const double xe = sum_lds(c, c.own_ ? c.q * y : 0.0);
   xe_out = xe;
@@ -325,15 +336,32 @@ __device__ __forceinline__ double wg_rhs(Team<MT> &c, double y, double &xe_out) 
   // term word: byte offset of the flux | the coefficient's FP64 high 16
   // bits (its other bits are zero), so decoding is two masks
   const unsigned char LDSA *Fb = (const unsigned char LDSA *)F;
+  double acc[NET::VL];
 #pragma unroll
-  for (int p = 0; p < NET::VL; p++) {
-    double acc = 0.0;
-    // (the last chunk's "next run" is its own first words again: loaded, unused)
-    run_const<6, NET::TM, NET::TM>(pre, rt, p * NET::TM, rt, (p + 1 < NET::VL) ? (p + 1) * NET::TM : 0, c.tid,
-              [&](uint32_t w) { return *(const double LDSA *)(Fb + (w & 0xffff)); },
-              [&](uint32_t w, double f) { acc = fma(__hiloint2double((int)(w & 0xffff0000u), 0), f, acc); });
-    S.P[p * BLOCK + c.tid] = acc;
+  for (int p = 0; p < NET::VL; p++) acc[p] = 0.0;
+#pragma unroll
+  for (int i0 = 0; i0 < NI; i0 += RG) {
+    double f[RG];
+    uint32_t cw[RG];
+#pragma unroll
+    for (int u = 0; u < RG; u++) {
+      cw[u] = tw[u];
+      if (i0 + u < NI) f[u] = *(const double LDSA *)(Fb + (cw[u] & 0xffff));
+    }
+#pragma unroll
+    for (int u = 0; u < RG; u++) {
+      tw[u] = tw2[u];
+      if (i0 + 2 * RG + u < NI) tw2[u] = term_word(i0 + 2 * RG + u);
+    }
+#pragma unroll
+    for (int u = 0; u < RG; u++)
+      if (i0 + u < NI) {
+        const int p = (i0 + u) % NET::VL;
+        acc[p] = fma(__hiloint2double((int)(cw[u] & 0xffff0000u), 0), f[u], acc[p]);
+      }
   }
+#pragma unroll
+  for (int p = 0; p < NET::VL; p++) S.P[p * BLOCK + c.tid] = acc[p];
   lds_sync();
   // the row's chunk partials in a fixed order, read 8 at a time (indices
   // clamped to P[NV] = 0) so that a row waits for LDS once per 8 partials
