@@ -412,29 +412,61 @@ __device__ __forceinline__ void wg_jacobian(Team<MT> &c) {
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
+// Operands come from the schur32 / schur64 tables: per lane one 64-bit word
+// of four LU positions for two k steps (A | B << 16 | A' << 32 | B' << 48),
+// loaded SCHUR_AHEAD groups ahead of use (the tables carry padding words for
+// the look-ahead past the end), so the MFMA chain waits on LDS reads only.  The order of the k steps is that of a plain
+// loop over k, so every output entry sums its products in the same order.
 __device__ __forceinline__ void schur_mfma(const Team<float> &c, float LDSA *LU) {
   // wave w: output tile rows 32*(w>>1).., columns 32*(w&1)..
   // v_mfma_f32_32x32x2f32: lane l supplies A[i = l%32][k = l/32] and
   // B[k = l/32][j = l%32]; result register r of lane l is
   // D[8*(r/4) + 4*(l/32) + r%4][l%32]
+  constexpr int G = NET::SCHUR_G32, NS = NET::KH / 4, NG = NS / G, REM = NS % G;
   const int ti = c.wave >> 1, tj = c.wave & 1;
   const int li = c.lane & 31, lk = c.lane >> 5;
   float LDSA *Tl = LU + NET::TBASE;
+  const gptr<uint64_t> tab = opaque(NET::schur32) + c.wave * NS * 64 + c.lane;
+  uint64_t w[NET::SCHUR_AHEAD + 1][G];
+#pragma unroll
+  for (int a = 0; a < NET::SCHUR_AHEAD; a++)
+#pragma unroll
+    for (int u = 0; u < G; u++) w[a][u] = tab[(a * G + u) * 64];
   f32x16 acc;
 #pragma unroll
   for (int r = 0; r < 16; r++) {
     const int i = 32 * ti + 8 * (r >> 2) + 4 * lk + (r & 3);
     acc[r] = Tl[i * NET::TS + 32 * tj + li];
   }
-  const gptr<uint16_t> lrow = opaque(NET::lth_pos) + (32 * ti + li) * NET::KH;
-  const gptr<uint16_t> ucol = opaque(NET::uht_pos) + 32 * tj + li;
-#pragma unroll 4
-  for (int k0 = 0; k0 < NET::KH; k0 += 2) {
-    const int k = k0 + lk;
-    const float a = -LU[lrow[k]];
-    const float bv = LU[ucol[k * 64]];
-    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a, bv, acc, 0, 0, 0);
+  // n words: every operand read from LDS first, then the MFMA chain (the
+  // scheduling barriers keep the compiler from sinking the table loads and
+  // the LDS reads into the chain, which would serialise their latencies)
+  auto steps = [&](const uint64_t *v, auto n) {
+    float x[4 * (decltype(n)::value > 0 ? decltype(n)::value : 1)];  // n = 0: a network without a head
+#pragma unroll
+    for (int u = 0; u < decltype(n)::value; u++) {
+      x[4 * u] = -LU[v[u] & 0xffff];
+      x[4 * u + 1] = LU[(v[u] >> 16) & 0xffff];
+      x[4 * u + 2] = -LU[(v[u] >> 32) & 0xffff];
+      x[4 * u + 3] = LU[v[u] >> 48];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int u = 0; u < 2 * decltype(n)::value; u++)
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(x[2 * u], x[2 * u + 1], acc, 0, 0, 0);
+  };
+  // fully unrolled, so that the buffers rotate by name: a register copy of a
+  // word still in flight would wait for its load
+  constexpr int NB = NET::SCHUR_AHEAD + 1;
+#pragma unroll
+  for (int g = 0; g < NG; g++) {
+#pragma unroll
+    for (int u = 0; u < G; u++) w[(g + NET::SCHUR_AHEAD) % NB][u] = tab[((g + NET::SCHUR_AHEAD) * G + u) * 64];
+    __builtin_amdgcn_sched_barrier(0);
+    steps(w[g % NB], std::integral_constant<int, G>());
+    __builtin_amdgcn_sched_barrier(0);
   }
+  if constexpr (REM > 0) steps(w[NG % NB], std::integral_constant<int, REM>());
 #pragma unroll
   for (int r = 0; r < 16; r++) {
     const int i = 32 * ti + 8 * (r >> 2) + 4 * lk + (r & 3);
@@ -445,27 +477,49 @@ __device__ __forceinline__ void schur_mfma(const Team<float> &c, float LDSA *LU)
 __device__ __forceinline__ void schur_mfma(const Team<double> &c, double LDSA *LU) {
   // v_mfma_f64_16x16x4f64: lane l supplies A[i = l%16][k = l/16] and
   // B[k = l/16][j = l%16]; result register r of lane l is D[4*r + l/16][l%16].
-  // wave w computes the four 16x16 tiles of row band w.
+  // wave w computes the four 16x16 tiles of row band w, interleaved: one A
+  // operand (the same in every tile's word) feeds four MFMA chains.
+  constexpr int NS = (NET::KH + 7) / 8;
   const int li = c.lane & 15, lk = c.lane >> 4;
   double LDSA *Tl = LU + NET::TBASE;
+  const int i0 = 16 * c.wave;
+  const gptr<uint64_t> tab = opaque(NET::schur64) + c.wave * NS * 4 * 64 + c.lane;
+  uint64_t w[NET::SCHUR_AHEAD + 1][4];
 #pragma unroll
-  for (int tj = 0; tj < 4; tj++) {
-    const int i0 = 16 * c.wave, j0 = 16 * tj;
-    f64x4 acc;
+  for (int a = 0; a < NET::SCHUR_AHEAD; a++)
 #pragma unroll
-    for (int r = 0; r < 4; r++) acc[r] = Tl[(i0 + 4 * r + lk) * NET::TS + j0 + li];
-    const gptr<uint16_t> lrow = opaque(NET::lth_pos) + (i0 + li) * NET::KH;
-    const gptr<uint16_t> ucol = opaque(NET::uht_pos) + j0 + li;
-#pragma unroll 2
-    for (int k0 = 0; k0 < NET::KH; k0 += 4) {
-      const int k = k0 + lk;
-      const double a = -LU[lrow[k]];
-      const double bv = LU[ucol[k * 64]];
-      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bv, acc, 0, 0, 0);
+    for (int t = 0; t < 4; t++) w[a][t] = tab[(a * 4 + t) * 64];
+  f64x4 acc[4];
+#pragma unroll
+  for (int t = 0; t < 4; t++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) acc[t][r] = Tl[(i0 + 4 * r + lk) * NET::TS + 16 * t + li];
+  // fully unrolled, so that the buffers rotate by name
+  constexpr int NB = NET::SCHUR_AHEAD + 1;
+#pragma unroll
+  for (int s = 0; s < NS; s++) {
+#pragma unroll
+    for (int t = 0; t < 4; t++) w[(s + NET::SCHUR_AHEAD) % NB][t] = tab[((s + NET::SCHUR_AHEAD) * 4 + t) * 64];
+    __builtin_amdgcn_sched_barrier(0);
+    const uint64_t *v = w[s % NB];
+    const double a0 = -LU[v[0] & 0xffff], a1 = -LU[(v[0] >> 32) & 0xffff];
+    double b0[4], b1[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      b0[t] = LU[(v[t] >> 16) & 0xffff];
+      b1[t] = LU[v[t] >> 48];
     }
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int r = 0; r < 4; r++) Tl[(i0 + 4 * r + lk) * NET::TS + j0 + li] = acc[r];
+    for (int t = 0; t < 4; t++) acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0[t], acc[t], 0, 0, 0);
+#pragma unroll
+    for (int t = 0; t < 4; t++) acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1[t], acc[t], 0, 0, 0);
+    __builtin_amdgcn_sched_barrier(0);
   }
+#pragma unroll
+  for (int t = 0; t < 4; t++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) Tl[(i0 + 4 * r + lk) * NET::TS + 16 * t + li] = acc[t][r];
 }
 
 // ---------------------------------------------------------------------

@@ -47,6 +47,8 @@ BLOCK = 256
 POS_BITS = 14          # LU positions (NLU + 1 < 16384)
 MGROUP = 8             # rounds per load group of the M build
 SU = 4                 # rounds per load group of the setup stream
+SCHUR_G32 = 4          # words per load group of the FP32 Schur loop
+SCHUR_AHEAD = 2        # load groups in flight ahead of the Schur loops' use
 COEF_BITS = 4          # signed stoichiometric coefficients (-8..7)
 
 
@@ -387,6 +389,7 @@ class WgTables:
                     self.lth[t * self.kh + a] = int(pos_of[i, k])
                 if F[k, i]:
                     self.uht[a * 64 + t] = int(pos_of[k, i])
+        self.schur32, self.schur64 = self._schur_tables()
         # ------------------------ explicit head inverses, computed in the setup
         # Linv[a][c] = -(L[a][c] + sum_b L[a][b] Linv[b][c])  (rows by forward level)
         # Winv[a][c] = -Winv[a][a] * sum_b U[a][b] Winv[b][c]  (rows by backward level)
@@ -548,6 +551,8 @@ class WgTables:
         c("MTMP", self.mtmp, "M-build rounds padded to whole load groups (the saved Jacobian's size in rounds)")
         c("MGROUP", MGROUP, "rounds per M-build load group")
         c("KH", self.kh, "head pivots padded to a multiple of 4")
+        c("SCHUR_G32", SCHUR_G32, "words per load group of the FP32 Schur loop")
+        c("SCHUR_AHEAD", SCHUR_AHEAD, "Schur load groups in flight ahead of use")
         c("NHEAD", self.nh_piv); c("NLEV", self.nlev, "head elimination levels")
         c("LUONE", self.luone, "LU[LUONE] = 1")
         c("NHI", self.nhi, "entries of the head inverses Linv, Winv")
@@ -630,12 +635,56 @@ class WgTables:
         L.append("// LU positions of L[tail t][head k] ([t][k]) and U[head k][tail j] ([k][j])")
         arr("uint16_t", "lth_pos", self.lth, str, 16)
         arr("uint16_t", "uht_pos", self.uht, str, 16)
+        L.append("// Schur operand positions per lane, two k steps a word (A | B << 16 | A' << 32 | B' << 48):")
+        L.append("// FP32 [wave][word][lane], FP64 [wave][word][column tile][lane], padding words after each")
+        arr("uint64_t", "schur32", self.schur32, hx64, 6)
+        arr("uint64_t", "schur64", self.schur64, hx64, 6)
         arr("uint16_t", "diag_pos", self.diag_pos)
         arr("int8_t", "charge", self.charges + [0] * (self.block - self.neq))
         emit_rate_params(L, net)
         L.append(f"}}  // namespace {ns}")
         with open(path, "w") as fh:
             fh.write("\n".join(L) + "\n")
+
+    def _schur_tables(self):
+        """Operand positions of the MFMA Schur complement, one u64 word per lane
+        for two k steps: A = L[t][k] | B = U[k][j] << 16 of the first step,
+        the same for the second << 32.  A k past the head reads LU[luzero] = 0.
+
+        FP32 (v_mfma_f32_32x32x2f32, wave w -> tile rows 32 (w >> 1), columns
+        32 (w & 1); lane l = A row / B column l % 32, k offset l / 32): word s
+        of wave w holds k = 4s + l/32 and 4s + 2 + l/32; [w][s][lane].
+        FP64 (v_mfma_f64_16x16x4f64, wave w -> row band 16w, four column tiles
+        jt; lane l = row / column l % 16, k offset l / 16): word s of tile jt
+        holds k = 8s + l/16 and 8s + 4 + l/16; [w][s][jt][lane].
+        Each table is followed by SCHUR_AHEAD groups of padding words, which
+        the loops' look-ahead loads read and never use."""
+        kh = self.kh
+        z = self.luzero
+        lth = lambda t, k: self.lth[t * kh + k] if k < kh else z
+        uht = lambda k, j: self.uht[k * 64 + j] if k < kh else z
+
+        def word(t, j, k0, k1):
+            return lth(t, k0) | uht(k0, j) << 16 | lth(t, k1) << 32 | uht(k1, j) << 48
+
+        t32, t64 = [], []
+        if kh:
+            for w in range(4):
+                ti, tj = w >> 1, w & 1
+                for s in range(kh // 4):
+                    for lane in range(64):
+                        li, lk = lane & 31, lane >> 5
+                        t32.append(word(32 * ti + li, 32 * tj + li, 4 * s + lk, 4 * s + 2 + lk))
+            for w in range(4):
+                for s in range((kh + 7) // 8):
+                    for jt in range(4):
+                        for lane in range(64):
+                            li, lk = lane & 15, lane >> 4
+                            t64.append(word(16 * w + li, 16 * jt + li, 8 * s + lk, 8 * s + 4 + lk))
+            pad = z | z << 16 | z << 32 | z << 48
+            t32 += [pad] * (SCHUR_AHEAD * SCHUR_G32 * 64)
+            t64 += [pad] * (SCHUR_AHEAD * 4 * 64)
+        return t32, t64
 
     def summary(self) -> str:
         ne = sum(self.sp_groups[:self.nep]) * SU

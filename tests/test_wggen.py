@@ -229,6 +229,61 @@ def test_factor_solve_tables(case, gamma):
     assert np.max(err) < 1e-6 * max(1.0, np.max(np.abs(ref) * w[:n])), (name, gamma, np.max(err))
 
 
+def test_schur_tables(case):
+    """The MFMA Schur loops' per-lane operand words (schur32, schur64), run
+    the way the kernels run them, give T - L_TH U_HT of the position tables;
+    in schur64 every column tile's word carries the same A operands (the
+    kernel reads A from the first tile's word only)."""
+    name, net, tab, *_ = case
+    KH = tab.kh
+    if not KH:
+        assert tab.schur32 == [] and tab.schur64 == []
+        return
+    rng = np.random.default_rng(3)
+    LU = rng.standard_normal(tab.nlu + 1)
+    LU[tab.luzero] = 0.0
+    Lth = np.array(tab.lth, dtype=np.int64).reshape(64, KH)
+    Uht = np.array(tab.uht, dtype=np.int64).reshape(KH, 64)
+    ref = -LU[Lth] @ LU[Uht]
+    f = lambda w, sh: (w >> sh) & 0xFFFF
+    # FP32: wave w -> tile (w >> 1, w & 1) of 32 x 32; v_mfma_f32_32x32x2f32
+    ns = KH // 4
+    t32 = np.array(tab.schur32, dtype=np.uint64)
+    assert len(t32) == 4 * ns * 64 + wggen.SCHUR_AHEAD * wggen.SCHUR_G32 * 64
+    got = np.zeros((64, 64))
+    for w in range(4):
+        ti, tj = w >> 1, w & 1
+        words = t32[w * ns * 64:(w + 1) * ns * 64].reshape(ns, 64).astype(np.int64)
+        for s in range(ns):
+            for half in (0, 32):
+                A = np.zeros((32, 2)); B = np.zeros((2, 32))
+                for lane in range(64):
+                    li, lk = lane & 31, lane >> 5
+                    A[li, lk] = -LU[f(words[s, lane], half)]
+                    B[lk, li] = LU[f(words[s, lane], half + 16)]
+                got[32 * ti:32 * ti + 32, 32 * tj:32 * tj + 32] += A @ B
+    np.testing.assert_allclose(got, ref, rtol=1e-12, atol=1e-12)
+    # FP64: wave w -> row band 16w, four 16 x 16 column tiles; v_mfma_f64_16x16x4f64
+    ns = (KH + 7) // 8
+    t64 = np.array(tab.schur64, dtype=np.uint64)
+    assert len(t64) == 4 * ns * 4 * 64 + wggen.SCHUR_AHEAD * 4 * 64
+    words = t64[:4 * ns * 4 * 64].reshape(4, ns, 4, 64).astype(np.int64)
+    got = np.zeros((64, 64))
+    for w in range(4):
+        for s in range(ns):
+            for jt in range(4):
+                assert np.array_equal(f(words[w, s, jt], 0), f(words[w, s, 0], 0))
+                assert np.array_equal(f(words[w, s, jt], 32), f(words[w, s, 0], 32))
+                for half in (0, 32):
+                    A = np.zeros((16, 4)); B = np.zeros((4, 16))
+                    for lane in range(64):
+                        li, lk = lane & 15, lane >> 4
+                        A[li, lk] = -LU[f(words[w, s, jt, lane], half)]
+                        B[lk, li] = LU[f(words[w, s, jt, lane], half + 16)]
+                    got[16 * w:16 * w + 16, 16 * jt:16 * jt + 16] += A @ B
+    np.testing.assert_allclose(got, ref, rtol=1e-12, atol=1e-12)
+
+
 def test_analysis_structure():
     """Structural facts the kernel relies on."""
     for name in NETS:
