@@ -62,6 +62,13 @@ struct RegionBytes {
   static_assert(NET::TBASE % 4 == 0, "F and Tl start 16-byte aligned");
 };
 
+// Gauss-Jordan pivot of one step: one LDS read gives a wavefront both
+template <typename MT>
+struct alignas(2 * sizeof(MT)) GjPivot {
+  MT d;
+  int p;
+};
+
 template <typename MT>
 struct WgLDS {
   double X[NET::NB + 1];     // factor vector: x_0..x_{NEQ-1}, x_e, 1
@@ -71,8 +78,7 @@ struct WgLDS {
   double P[NET::NV + 1];     // RHS chunk partials (P[NV] = 0), tail gather partials
   double V[64], V2[64];      // tail inverse exchange
   MT gjcol[2][64];           // Gauss-Jordan: pivot column (double-buffered by step)
-  MT gjd[2];                 //   reciprocal of the pivot
-  int gjp[2];                //   pivot row, -1 if singular
+  GjPivot<MT> gjpv[2];       //   reciprocal of the pivot and its row (-1: singular)
   double red[2][NWAVE];      // team reductions (double-buffered)
   int badpiv;                // head elimination met a zero or non-finite pivot
   BdfU u;
@@ -546,14 +552,20 @@ __device__ __forceinline__ bool tail_gj(Team<MT> &c, MT (&A4)[16], int &invp) {
   bool used = c.lane >= NET::NT;
   invp = c.lane;
   // the owner of column k = 16 * (its wavefront) + jk publishes step k's pivot
+  // (every lane divides while the search runs; the pivot lane's quotient
+  // is the pivot's reciprocal, read with every lane active -- a lane-read
+  // inside the lane-0 branch could see quotients the other lanes never
+  // computed, once the compiler sinks the division into that branch)
   auto publish = [&](int jk, int buf) {
     const MT colk = A4[jk];
+    const MT rc = (MT)1.0 / colk;
     bool ok;
     const int p = pivot_row<6, MT>(wave_max_u(used ? 0u : pivot_key<6>(fabs(colk), c.lane)), ok);
+    const MT d = rdlane(rc, p);
     S.gjcol[buf][c.lane] = colk;
     if (c.lane == 0) {
-      S.gjp[buf] = ok ? p : -1;
-      S.gjd[buf] = ok ? (MT)1.0 / rdlane(colk, p) : (MT)0.0;
+      S.gjpv[buf].d = ok ? d : (MT)0.0;
+      S.gjpv[buf].p = ok ? p : -1;
     }
   };
   if (c.wave == 0) publish(0, 0);
@@ -565,9 +577,10 @@ __device__ __forceinline__ bool tail_gj(Team<MT> &c, MT (&A4)[16], int &invp) {
       if (k >= NET::NT) break;
       const int buf = k & 1;
       lds_sync();
-      const int p = uni(S.gjp[buf]);
-      if (p < 0) return false;
-      const MT d = S.gjd[buf];
+      // the pivot and the column in one LDS round trip; a singular step is
+      // left after its (meaningless) updates, by every wavefront alike
+      const MT d = S.gjpv[buf].d;
+      const int p = uni(S.gjpv[buf].p);
       const MT fi = S.gjcol[buf][c.lane];
       const bool isp = (c.lane == p);
       if (isp) { used = true; invp = k; }
@@ -592,6 +605,7 @@ __device__ __forceinline__ bool tail_gj(Team<MT> &c, MT (&A4)[16], int &invp) {
       }
       // the pivot column becomes the elimination's column of the inverse
       if (c.wave == wk) A4[jk] = isp ? d : -fi * d;
+      if (p < 0) return false;
     }
   }
   return true;
