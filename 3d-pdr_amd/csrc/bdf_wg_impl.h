@@ -653,31 +653,30 @@ __device__ __forceinline__ int wg_setup(Team<MT> &c, BdfU LDSA &u, BdfWG<MT> &b,
   }
   __syncthreads();
   if (c.tid == 0) S.badpiv = 0;
-  // M = I - gamma J, scaled, in LU storage order.  Position entry d: row,
-  // column, whether J has an entry there (the saved Jacobian holds it at the
-  // same position); the bordered row holds the charges.  The table entries
-  // and J values of the next group of positions are loaded while the current
-  // group is scaled and stored.
+  // M = I - gamma J, scaled, in LU storage order, without branches.  Position
+  // word d: row | col << 8 | flags -- MF_J: the saved Jacobian holds a value
+  // at the same position (a J entry, or a charge of the bordered row, which
+  // wg_slot_charges put there); MF_BORD: the bordered row, coefficient -1 in
+  // place of -gamma; MF_DIAG: add 1; MF_FIXED: the zero or one slot (MF_ONE),
+  // unscaled; MF_SKIP: padding.  A position without a J value gets -gamma * 0
+  // (a signed zero).  The table entries and J values of the next group of
+  // positions are loaded while the current group is scaled and stored.
   const gptr<uint32_t> mp = opaque(NET::mpos);
   const gptr<MT> jg = (gptr<MT>)c.Jg;
-  auto m_value = [&](uint32_t d, MT j) -> double {
-    const int row = d & 0xff, col = (d >> 8) & 0xff, src = d >> 16;
-    // the bordered row's charges are in the slot too (wg_slot_charges)
-    if (row == NET::NEQ) return (col == NET::NEQ) ? 0.0 : (double)j;
-    return (row < 0xfd && src != 0xffff) ? (double)j : 0.0;
-  };
-  auto m_entry = [&](uint32_t d, double jv) -> double {
-    const int row = d & 0xff, col = (d >> 8) & 0xff;
-    if (row == 0xfe) return 0.0;  // the zero slot
-    if (row == 0xfd) return 1.0;  // the one slot
-    double m;
-    if (row == NET::NEQ) {
-      m = (col == NET::NEQ) ? 1.0 : -jv;
-    } else {
-      m = -gamma * jv;
-      if (row == col) m += 1.0;
-    }
-    return m * S.W[row] * S.Winv[col];
+  // a padding position stores into its lane's slot of the tail block's
+  // padding column (itself padding, never read), so that no store, and no
+  // computation sunk into one, depends on a branch
+  static_assert(NET::TS == NET::NT + 1, "the tail block has a padding column");
+  const int mdummy = NET::TBASE + (c.tid % NET::NT) * NET::TS + NET::NT;
+  auto m_entry = [&](uint32_t d, MT j, double w, double wi) -> MT {
+    const double jv = (d & NET::MF_J) ? (double)j : 0.0;
+    const double jt = ((d & NET::MF_BORD) ? -1.0 : -gamma) * jv;
+    const double m = (d & NET::MF_DIAG) ? jt + 1.0 : jt;
+    double v = m * w * wi;
+    // an opaque step: the value is computed for every position, so the
+    // compiler does not turn the select below into a branch around the reads
+    asm volatile("" : "+v"(v));
+    return (MT)((d & NET::MF_FIXED) ? ((d & NET::MF_ONE) ? 1.0 : 0.0) : v);
   };
   {
     // the position table and the Jacobian slot are padded to whole groups, so
@@ -707,9 +706,17 @@ __device__ __forceinline__ int wg_setup(Team<MT> &c, BdfU LDSA &u, BdfWG<MT> &b,
         d[u] = mp[(tn + u) * BLOCK + c.tid];
         j[u] = jg[(tn + u) * BLOCK + c.tid];
       }
+      // the group's scale factors first: every LDS read ahead of the stores,
+      // which the compiler cannot tell apart from them
+      double wr[U], wc[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        wr[u] = S.W[dc[u] & 0xff];
+        wc[u] = S.Winv[(dc[u] >> 8) & 0xff];
+      }
 #pragma unroll
       for (int u = 0; u < U; u++)
-        if ((dc[u] & 0xff) != 0xff) LU[(t0 + u) * BLOCK + c.tid] = (MT)m_entry(dc[u], m_value(dc[u], jc[u]));
+        LU[(dc[u] & NET::MF_SKIP) ? mdummy : (t0 + u) * BLOCK + c.tid] = m_entry(dc[u], jc[u], wr[u], wc[u]);
     }
   }
   // Head elimination and head inverses: the setup stream, phase by phase
@@ -1108,8 +1115,8 @@ __global__ void __launch_bounds__(BLOCK) wg_jac_kernel(const double *x, const do
   double *o = out + (size_t)sys * NET::NEQ * (NET::NEQ + 1);
   for (int t = 0; t < NET::MTM; t++) {
     const uint32_t d = NET::mpos[t * BLOCK + c.tid];
-    const int row = d & 0xff, col = (d >> 8) & 0xff, src = d >> 16;
-    if (row < NET::NEQ && col <= NET::NEQ && src != 0xffff)
+    const int row = d & 0xff, col = (d >> 8) & 0xff;
+    if ((d & NET::MF_J) && row < NET::NEQ && col <= NET::NEQ)
       o[(size_t)row * (NET::NEQ + 1) + col] = (double)c.Jg[t * BLOCK + c.tid];
   }
 }

@@ -47,6 +47,13 @@ BLOCK = 256
 POS_BITS = 14          # LU positions (NLU + 1 < 16384)
 MGROUP = 8             # rounds per load group of the M build
 SU = 4                 # rounds per load group of the setup stream
+# M-build position flags (bits 16.. of an mpos word; row | col << 8 below):
+MF_J = 1 << 16         #   the Jacobian slot holds a value here (J entry / charge)
+MF_BORD = 1 << 17      #   the bordered row: coefficient -1 in place of -gamma
+MF_DIAG = 1 << 18      #   add 1 (M = I - gamma J)
+MF_FIXED = 1 << 19     #   the zero or one slot: unscaled constant
+MF_ONE = 1 << 20       #   ... which is 1
+MF_SKIP = 1 << 21      #   padding: nothing to store
 SCHUR_G32 = 4          # words per load group of the FP32 Schur loop
 SCHUR_AHEAD = 2        # load groups in flight ahead of the Schur loops' use
 COEF_BITS = 4          # signed stoichiometric coefficients (-8..7)
@@ -297,27 +304,31 @@ class WgTables:
         for i in tail:
             allpos += [(i, j) for j in tail] + ["pad"]
         assert len(allpos) == self.nlu
-        # M entry descriptors in position order: row | col << 8 | src << 16
-        # (src 0xffff: no J term); row 0xfe: the zero slot, 0xfd: the one slot,
-        # 0xff: padding (and the inverse block, filled later in the setup)
+        # M entry descriptors in position order: row | col << 8 | flags (see
+        # MF_*; the zero and one slots and padding have row = col = 0)
         self.mpos = []
         for e in allpos:
             if e is None:
-                self.mpos.append(0xFE)
+                self.mpos.append(MF_FIXED)
                 continue
             if e == "pad":
-                self.mpos.append(0xFF)
+                self.mpos.append(MF_SKIP)
                 continue
             if e == "one":
-                self.mpos.append(0xFD)
+                self.mpos.append(MF_FIXED | MF_ONE)
                 continue
             i, j = e
-            src = jid.get((i, j), 0xFFFF) if i < n else 0xFFFF
-            self.mpos.append(i | (j << 8) | (src << 16))
+            f = 0
+            if i < n:
+                f |= MF_J if (i, j) in jid else 0
+            else:                     # the bordered row: charges, and 1 at x_e
+                f |= MF_BORD | (MF_J if j < n else 0)
+            f |= MF_DIAG if i == j else 0
+            self.mpos.append(i | (j << 8) | f)
         self.mtm = math.ceil(self.nlu / block)
         # padded to whole groups of the M build's loads (MGROUP rounds)
         self.mtmp = MGROUP * math.ceil(self.mtm / MGROUP)
-        self.mpos += [0xFF] * (self.mtmp * block - self.nlu)
+        self.mpos += [MF_SKIP] * (self.mtmp * block - self.nlu)
         # ------------------------------------------ head elimination levels
         hl = an.head_level
         self.nlev = int(hl.max()) + 1 if an.nhead else 0
@@ -550,6 +561,9 @@ class WgTables:
         c("NLU", self.nlu); c("MTM", self.mtm, "M-build rounds")
         c("MTMP", self.mtmp, "M-build rounds padded to whole load groups (the saved Jacobian's size in rounds)")
         c("MGROUP", MGROUP, "rounds per M-build load group")
+        for nm_, v_ in (("MF_J", MF_J), ("MF_BORD", MF_BORD), ("MF_DIAG", MF_DIAG), ("MF_FIXED", MF_FIXED),
+                        ("MF_ONE", MF_ONE), ("MF_SKIP", MF_SKIP)):
+            L.append(f"constexpr uint32_t {nm_} = 0x{v_:x}u;  // M-build position flag")
         c("KH", self.kh, "head pivots padded to a multiple of 4")
         c("SCHUR_G32", SCHUR_G32, "words per load group of the FP32 Schur loop")
         c("SCHUR_AHEAD", SCHUR_AHEAD, "Schur load groups in flight ahead of use")

@@ -163,20 +163,22 @@ def _factor_and_solve(tab, net, Jb, gamma, w, rhs):
     """The kernel's setup + solve, in the kernel's order, on numpy arrays."""
     n, NB = net.neq, tab.nb
     q = np.asarray(net.charges(), dtype=float)
-    Jg = np.array([Jb[i, j] for (i, j) in tab.jent])
+    # the saved Jacobian by LU position, the bordered row's charges beside it
+    Jg = np.zeros(tab.mtmp * tab.block)
+    for e, (i, j) in enumerate(tab.jent):
+        Jg[tab.jpos[e]] = Jb[i, j]
     LU = np.zeros(tab.nlu)
     for p in range(tab.nlu):
         d = tab.mpos[p]
-        row, col, src = d & 0xFF, (d >> 8) & 0xFF, d >> 16
-        if row in (0xFF, 0xFE):
+        row, col = d & 0xFF, (d >> 8) & 0xFF
+        if d & wggen.MF_SKIP:
             continue
-        if row == 0xFD:
-            LU[p] = 1.0
+        if d & wggen.MF_FIXED:
+            LU[p] = 1.0 if d & wggen.MF_ONE else 0.0
             continue
-        if row == n:
-            m = 1.0 if col == n else -q[col]
-        else:
-            m = (-gamma * Jg[src] if src != 0xFFFF else 0.0) + (1.0 if row == col else 0.0)
+        assert bool(d & wggen.MF_BORD) == (row == n) and bool(d & wggen.MF_DIAG) == (row == col)
+        jv = (q[col] if row == n else Jg[p]) if d & wggen.MF_J else 0.0
+        m = (-1.0 if d & wggen.MF_BORD else -gamma) * jv + (1.0 if d & wggen.MF_DIAG else 0.0)
         LU[p] = m * w[row] / w[col]
     _run_setup_stream(tab, LU)
     NT, TS, KH = tab.nt, tab.tstride, tab.kh
