@@ -591,12 +591,13 @@ struct pdrchem_cloud {
     return true;
   }
   ~pdrchem_cloud() {
-    if (stream) hipStreamSynchronize(stream);
+    // teardown: nothing to report to, so the return codes are dropped
+    if (stream) (void)hipStreamSynchronize(stream);
     delete th;
-    for (void *q : allocs) hipFree(q);
-    if (ev0) hipEventDestroy(ev0);
-    if (ev1) hipEventDestroy(ev1);
-    if (stream) hipStreamDestroy(stream);
+    for (void *q : allocs) (void)hipFree(q);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (stream) (void)hipStreamDestroy(stream);
   }
   bool fail(const char *what) {
     err = t_err.empty() ? std::string(what) : std::string(what) + ": " + t_err;
@@ -984,37 +985,38 @@ int pdrchem_cloud_iterate(pdrchem_cloud *c, double t_end_years, double conv_tol,
       server_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
       if (!good) {
         c->err = std::string("solve through the request loop: ") + pdrchem_server_last_error(c->server);
-        hipEventDestroy(e2);
-        hipEventDestroy(e3);
+        (void)hipEventDestroy(e2);
+        (void)hipEventDestroy(e3);
         return -1;
       }
     } else if (pdrchem_solve_device(c->net, c->c_ab, c->c_rate, c->c_rho, nact, rtol, atol, 0.0, t_end_years,
                                     c->nelect, c->c_status, c->c_stats, c->d_counter, precision, s) != 0) {
       c->err = std::string("solve: ") + pdrchem_last_error();
-      hipEventDestroy(e2);
-      hipEventDestroy(e3);
+      (void)hipEventDestroy(e2);
+      (void)hipEventDestroy(e3);
       return -1;
     }
-    hipEventRecord(e3, s);
+    const bool recorded = ok(hipEventRecord(e3, s), "event");
     // 3. convergence, write-back
     hipLaunchKernelGGL(retire_kernel, dim3(blocks_for(nact, 256)), dim3(256), 0, s, nact, c->nspec, c->d_idx,
                        c->c_ab, c->c_status, c->d_ab, c->d_conv, conv_tol, 1.0e-20,
                        c->d_nconv, c->d_nfail, c->c_stats, c->d_ctr);
     unsigned cnt[3] = {0, 0, 0};
     unsigned long long ctr[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    bool good = ok(hipGetLastError(), "retire kernel") && columns(c, true) &&
+    bool good = recorded && ok(hipGetLastError(), "retire kernel") && columns(c, true) &&
                 ok(hipMemcpyAsync(&cnt[0], c->d_nconv, 4, hipMemcpyDeviceToHost, s), "copy") &&
                 ok(hipMemcpyAsync(&cnt[1], c->d_nfail, 4, hipMemcpyDeviceToHost, s), "copy") &&
                 ok(hipMemcpyAsync(&cnt[2], c->d_neg, 4, hipMemcpyDeviceToHost, s), "copy") &&
                 ok(hipMemcpyAsync(ctr, c->d_ctr, sizeof(ctr), hipMemcpyDeviceToHost, s), "copy") &&
                 ok(hipStreamSynchronize(s), "sync");
     if (good) {
-      hipEventElapsedTime(&ms_rates, c->ev0, c->ev1);
-      hipEventElapsedTime(&ms_solve, e2, e3);
+      // timings only: a failed query leaves -1 (not measured)
+      if (hipEventElapsedTime(&ms_rates, c->ev0, c->ev1) != hipSuccess) ms_rates = -1.0f;
+      if (hipEventElapsedTime(&ms_solve, e2, e3) != hipSuccess) ms_solve = -1.0f;
       if (server_ms >= 0) ms_solve = (float)server_ms;
     }
-    hipEventDestroy(e2);
-    hipEventDestroy(e3);
+    (void)hipEventDestroy(e2);
+    (void)hipEventDestroy(e3);
     if (!good) {
       c->fail("iteration");
       return -1;
